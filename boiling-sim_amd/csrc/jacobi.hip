@@ -1,0 +1,352 @@
+// Masked 7-point Jacobi pressure sweep on gfx950 and its fused n_iter driver.
+//
+// Arithmetic follows jacobi_pressure_step (python/boilingsim/fluid.py:243-276)
+// exactly: s = ((((L+R)+D)+U)+B)+F over the neighbours i-1, i+1, j-1, j+1,
+// k-1, k+1; rhs = (((rho*dx)*dx)*div)/dt; p_new = (s - rhs)/6, each operation
+// rounded separately (the library is built with -ffp-contract=off and f32
+// division is IEEE correctly rounded), so a sweep is bit-identical to the
+// reference arithmetic.  Out-of-bounds neighbours are air, as in
+// crates/cuda-kernels/src/jacobi_pressure.cu:121-124.
+//
+// Data movement.  One thread owns VEC consecutive k cells (dwordx4 loads when
+// nz % 4 == 0), one workgroup of 256 threads owns a contiguous chunk of one
+// (j,k) plane, and blockIdx.y walks i.  Neighbour rows/planes are re-read
+// through L1/L2; the XCD remap keeps each XCD on a contiguous slab of planes
+// so those re-reads hit its own L2.  HBM then sees each array once per sweep.
+//
+// Mask-free sweeps.  Inside the fused driver, sweeps after the first carry the
+// class of each non-fluid cell in the pressure array itself (air and solid
+// sentinels, see common.hpp), so those sweeps stream only p_in, div and p_out:
+// 12 B/cell instead of the 16 B/cell of a sweep that also reads the int32
+// mask.  The first sweep reads the mask and writes the encoded form; the last
+// sweep writes the decoded form (0 at every non-fluid cell, as the reference
+// does), and the other ping-pong buffer is decoded in place, so both caller
+// buffers end exactly as the reference leaves them.
+#include "common.hpp"
+
+namespace bs {
+
+template <int VEC> struct VecF;
+template <> struct VecF<1> { using T = float; };
+template <> struct VecF<2> { using T = float2; };
+template <> struct VecF<4> { using T = float4; };
+template <int VEC> struct VecI;
+template <> struct VecI<1> { using T = int; };
+template <> struct VecI<2> { using T = int2; };
+template <> struct VecI<4> { using T = int4; };
+
+template <int VEC>
+__device__ __forceinline__ void load_vec(const float* __restrict__ p, int64_t off, float (&v)[VEC]) {
+    using T = typename VecF<VEC>::T;
+    const T t = *reinterpret_cast<const T*>(p + off);
+    const float* tv = reinterpret_cast<const float*>(&t);
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) v[e] = tv[e];
+}
+
+template <int VEC>
+__device__ __forceinline__ void load_vec(const int* __restrict__ p, int64_t off, int (&v)[VEC]) {
+    using T = typename VecI<VEC>::T;
+    const T t = *reinterpret_cast<const T*>(p + off);
+    const int* tv = reinterpret_cast<const int*>(&t);
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) v[e] = tv[e];
+}
+
+template <int VEC>
+__device__ __forceinline__ void store_vec(float* __restrict__ p, int64_t off, const float (&v)[VEC]) {
+    using T = typename VecF<VEC>::T;
+    T t;
+    float* tv = reinterpret_cast<float*>(&t);
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) tv[e] = v[e];
+    *reinterpret_cast<T*>(p + off) = t;
+}
+
+enum : int { kInMask = 0, kInEncoded = 1, kInZero = 2 };
+enum : int { kOutDecoded = 0, kOutEncoded = 1 };
+
+// Neighbour contribution in encoded form: air sentinel -> 0, solid sentinel ->
+// p_self (Neumann ghost), anything else is a fluid value.
+__device__ __forceinline__ float contrib_enc(float v, float ps) {
+    const uint32_t b = __float_as_uint(v);
+    float r = (b == kSentSolid) ? ps : v;
+    return (b == kSentAir) ? 0.0f : r;
+}
+
+// Neighbour contribution from the class id (fluid.py:_pressure_neighbour).
+__device__ __forceinline__ float contrib_mat(int m, float v, float ps, int mf, int ma) {
+    if (m == mf) return v;
+    if (m == ma) return 0.0f;
+    return ps;
+}
+
+template <int VEC, int IN, int OUT>
+__global__ __launch_bounds__(256) void jacobi_sweep_kernel(
+    float* __restrict__ p_new, const float* __restrict__ p_old, const float* __restrict__ div,
+    const int* __restrict__ mat, int nx, int ny, int nz, FastDiv nzv_div, uint32_t chunks,
+    float c2, float dt, int mf, int ma) {
+    const uint32_t nblk = gridDim.x * gridDim.y;
+    const uint32_t lb = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, nblk);
+    const uint32_t chunk = lb % chunks;
+    const int i = (int)(lb / chunks);
+    const uint32_t nzv = nzv_div.d;
+    const uint32_t q = chunk * 256u + threadIdx.x;
+    if (q >= (uint32_t)ny * nzv) return;
+    const int j = (int)fdiv(q, nzv_div);
+    const int k0 = (int)(q - (uint32_t)j * nzv) * VEC;
+    const int64_t plane = (int64_t)ny * nz;
+    const int64_t c0 = (int64_t)i * plane + (int64_t)j * nz + k0;
+
+    float pc[VEC], dv[VEC], out[VEC];
+    if constexpr (IN != kInZero) load_vec<VEC>(p_old, c0, pc);
+    load_vec<VEC>(div, c0, dv);
+
+    if constexpr (IN == kInEncoded) {
+        const float air = __uint_as_float(kSentAir);
+        float pl[VEC], pr[VEC], pd[VEC], pu[VEC];
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) { pl[e] = air; pr[e] = air; pd[e] = air; pu[e] = air; }
+        if (i > 0) load_vec<VEC>(p_old, c0 - plane, pl);
+        if (i < nx - 1) load_vec<VEC>(p_old, c0 + plane, pr);
+        if (j > 0) load_vec<VEC>(p_old, c0 - nz, pd);
+        if (j < ny - 1) load_vec<VEC>(p_old, c0 + nz, pu);
+        const float pb_edge = (k0 > 0) ? p_old[c0 - 1] : air;
+        const float pf_edge = (k0 + VEC < nz) ? p_old[c0 + VEC] : air;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            const uint32_t cb = __float_as_uint(pc[e]);
+            if (cb == kSentAir || cb == kSentSolid) {
+                out[e] = (OUT == kOutEncoded) ? pc[e] : 0.0f;
+                continue;
+            }
+            const float ps = pc[e];
+            const float vb = (e > 0) ? pc[e - 1] : pb_edge;
+            const float vf = (e < VEC - 1) ? pc[e + 1] : pf_edge;
+            float s = contrib_enc(pl[e], ps) + contrib_enc(pr[e], ps);
+            s = s + contrib_enc(pd[e], ps);
+            s = s + contrib_enc(pu[e], ps);
+            s = s + contrib_enc(vb, ps);
+            s = s + contrib_enc(vf, ps);
+            const float rhs = (c2 * dv[e]) / dt;
+            out[e] = (s - rhs) / 6.0f;
+        }
+    } else if constexpr (IN == kInZero) {
+        // p_old == 0 everywhere (the projection's cold start, fluid.py:627-629):
+        // every neighbour term is +0, so s = +0 and only the class is needed.
+        int mc[VEC];
+        load_vec<VEC>(mat, c0, mc);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            if (mc[e] != mf) {
+                if (OUT == kOutEncoded)
+                    out[e] = __uint_as_float(mc[e] == ma ? kSentAir : kSentSolid);
+                else
+                    out[e] = 0.0f;
+                continue;
+            }
+            const float rhs = (c2 * dv[e]) / dt;
+            out[e] = (0.0f - rhs) / 6.0f;
+        }
+    } else {
+        int mc[VEC], ml[VEC], mr[VEC], md[VEC], mu[VEC];
+        float pl[VEC], pr[VEC], pd[VEC], pu[VEC];
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            ml[e] = ma; mr[e] = ma; md[e] = ma; mu[e] = ma;
+            pl[e] = 0.0f; pr[e] = 0.0f; pd[e] = 0.0f; pu[e] = 0.0f;
+        }
+        load_vec<VEC>(mat, c0, mc);
+        if (i > 0) { load_vec<VEC>(p_old, c0 - plane, pl); load_vec<VEC>(mat, c0 - plane, ml); }
+        if (i < nx - 1) { load_vec<VEC>(p_old, c0 + plane, pr); load_vec<VEC>(mat, c0 + plane, mr); }
+        if (j > 0) { load_vec<VEC>(p_old, c0 - nz, pd); load_vec<VEC>(mat, c0 - nz, md); }
+        if (j < ny - 1) { load_vec<VEC>(p_old, c0 + nz, pu); load_vec<VEC>(mat, c0 + nz, mu); }
+        float pb_edge = 0.0f, pf_edge = 0.0f;
+        int mb_edge = ma, mf_edge = ma;
+        if (k0 > 0) { pb_edge = p_old[c0 - 1]; mb_edge = mat[c0 - 1]; }
+        if (k0 + VEC < nz) { pf_edge = p_old[c0 + VEC]; mf_edge = mat[c0 + VEC]; }
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            if (mc[e] != mf) {
+                if (OUT == kOutEncoded)
+                    out[e] = __uint_as_float(mc[e] == ma ? kSentAir : kSentSolid);
+                else
+                    out[e] = 0.0f;
+                continue;
+            }
+            const float ps = pc[e];
+            const float vb = (e > 0) ? pc[e - 1] : pb_edge;
+            const int mbv = (e > 0) ? mc[e - 1] : mb_edge;
+            const float vf = (e < VEC - 1) ? pc[e + 1] : pf_edge;
+            const int mfv = (e < VEC - 1) ? mc[e + 1] : mf_edge;
+            float s = contrib_mat(ml[e], pl[e], ps, mf, ma) + contrib_mat(mr[e], pr[e], ps, mf, ma);
+            s = s + contrib_mat(md[e], pd[e], ps, mf, ma);
+            s = s + contrib_mat(mu[e], pu[e], ps, mf, ma);
+            s = s + contrib_mat(mbv, vb, ps, mf, ma);
+            s = s + contrib_mat(mfv, vf, ps, mf, ma);
+            const float rhs = (c2 * dv[e]) / dt;
+            out[e] = (s - rhs) / 6.0f;
+        }
+    }
+    store_vec<VEC>(p_new, c0, out);
+}
+
+// Sentinel -> 0 in place (restores the reference's zero at non-fluid cells).
+template <int VEC>
+__global__ __launch_bounds__(256) void decode_kernel(float* __restrict__ p, int64_t nvec) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= nvec) return;
+    float v[VEC];
+    load_vec<VEC>(p, t * VEC, v);
+    bool any = false;
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) {
+        const uint32_t b = __float_as_uint(v[e]);
+        if (b == kSentAir || b == kSentSolid) { v[e] = 0.0f; any = true; }
+    }
+    if (any) store_vec<VEC>(p, t * VEC, v);
+}
+
+struct SweepShape {
+    int vec;
+    FastDiv nzv;
+    uint32_t chunks;
+    dim3 grid;
+};
+
+static int pick_vec(int nz, std::initializer_list<const void*> ptrs) {
+    int vec = (nz % 4 == 0) ? 4 : (nz % 2 == 0) ? 2 : 1;
+    for (const void* p : ptrs) {
+        const uintptr_t a = (uintptr_t)p;
+        while (vec > 1 && (a % (uintptr_t)(4 * vec)) != 0) vec >>= 1;
+    }
+    return vec;
+}
+
+static SweepShape sweep_shape(int nx, int ny, int nz, int vec) {
+    SweepShape s;
+    s.vec = vec;
+    const uint32_t nzv = (uint32_t)(nz / vec);
+    s.nzv = make_fastdiv(nzv);
+    const uint64_t per_plane = (uint64_t)ny * nzv;
+    s.chunks = (uint32_t)((per_plane + 255) / 256);
+    s.grid = dim3(s.chunks, (unsigned)nx, 1);
+    return s;
+}
+
+template <int IN, int OUT>
+static void launch_sweep_t(const SweepShape& sh, hipStream_t st, float* p_new, const float* p_old,
+                           const float* div, const int* mat, int nx, int ny, int nz, float c2,
+                           float dt, int mf, int ma) {
+    switch (sh.vec) {
+        case 4:
+            jacobi_sweep_kernel<4, IN, OUT><<<sh.grid, 256, 0, st>>>(p_new, p_old, div, mat, nx, ny, nz,
+                                                                      sh.nzv, sh.chunks, c2, dt, mf, ma);
+            break;
+        case 2:
+            jacobi_sweep_kernel<2, IN, OUT><<<sh.grid, 256, 0, st>>>(p_new, p_old, div, mat, nx, ny, nz,
+                                                                      sh.nzv, sh.chunks, c2, dt, mf, ma);
+            break;
+        default:
+            jacobi_sweep_kernel<1, IN, OUT><<<sh.grid, 256, 0, st>>>(p_new, p_old, div, mat, nx, ny, nz,
+                                                                      sh.nzv, sh.chunks, c2, dt, mf, ma);
+            break;
+    }
+}
+
+static void launch_decode(hipStream_t st, float* p, int64_t n, int vec) {
+    const int64_t nvec = n / vec;
+    const unsigned blocks = (unsigned)((nvec + 255) / 256);
+    switch (vec) {
+        case 4: decode_kernel<4><<<blocks, 256, 0, st>>>(p, nvec); break;
+        case 2: decode_kernel<2><<<blocks, 256, 0, st>>>(p, nvec); break;
+        default: decode_kernel<1><<<blocks, 256, 0, st>>>(p, nvec); break;
+    }
+}
+
+// (rho*dx)*dx in float32, the leading factor of the reference's rhs.
+static inline float rhs_coef(float dx, float rho) {
+    volatile float a = rho * dx;
+    volatile float b = a * dx;
+    return b;
+}
+
+}  // namespace bs
+
+using namespace bs;
+
+extern "C" int bs_jacobi_pressure_launch_stream(float* p_new, const float* p_old, const float* div_u,
+                                                const int* mat, int nx, int ny, int nz, float dx,
+                                                float dt, float rho, int mat_fluid, int mat_air,
+                                                hipStream_t stream) {
+    if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
+    const int vec = pick_vec(nz, {p_new, p_old, div_u, mat});
+    const SweepShape sh = sweep_shape(nx, ny, nz, vec);
+    launch_sweep_t<kInMask, kOutDecoded>(sh, stream, p_new, p_old, div_u, mat, nx, ny, nz,
+                                         rhs_coef(dx, rho), dt, mat_fluid, mat_air);
+    return (int)hipGetLastError();
+}
+
+extern "C" int bs_jacobi_pressure_launch(float* p_new, const float* p_old, const float* div_u,
+                                         const int* mat, int nx, int ny, int nz, float dx, float dt,
+                                         float rho, int mat_fluid, int mat_air) {
+    return bs_jacobi_pressure_launch_stream(p_new, p_old, div_u, mat, nx, ny, nz, dx, dt, rho,
+                                            mat_fluid, mat_air, nullptr);
+}
+
+// Fused n_iter Jacobi solve (pressure_solve_launch, jacobi_pressure.cu:228-270).
+// zero_init != 0 promises p == 0 on entry (the projection's cold start) and
+// lets the first sweep skip reading p.
+extern "C" int bs_jacobi_solve_launch(float* p, float* p_tmp, const float* div_u, const int* mat,
+                                      int nx, int ny, int nz, float dx, float dt, float rho,
+                                      int n_iter, int mat_fluid, int mat_air, int zero_init,
+                                      hipStream_t st) {
+    if (nx <= 0 || ny <= 0 || nz <= 0 || n_iter <= 0) return hipSuccess;
+    const int vec = pick_vec(nz, {p, p_tmp, div_u, mat});
+    const SweepShape sh = sweep_shape(nx, ny, nz, vec);
+    const float c2 = rhs_coef(dx, rho);
+    const int64_t n = (int64_t)nx * ny * nz;
+    if (n_iter == 1) {
+        if (zero_init)
+            launch_sweep_t<kInZero, kOutDecoded>(sh, st, p_tmp, p, div_u, mat, nx, ny, nz, c2, dt,
+                                                 mat_fluid, mat_air);
+        else
+            launch_sweep_t<kInMask, kOutDecoded>(sh, st, p_tmp, p, div_u, mat, nx, ny, nz, c2, dt,
+                                                 mat_fluid, mat_air);
+    } else {
+        // Iteration `it` reads p when even and p_tmp when odd (fluid.py:688-700).
+        if (zero_init)
+            launch_sweep_t<kInZero, kOutEncoded>(sh, st, p_tmp, p, div_u, mat, nx, ny, nz, c2, dt,
+                                                 mat_fluid, mat_air);
+        else
+            launch_sweep_t<kInMask, kOutEncoded>(sh, st, p_tmp, p, div_u, mat, nx, ny, nz, c2, dt,
+                                                 mat_fluid, mat_air);
+        for (int it = 1; it < n_iter - 1; ++it) {
+            float* dst = (it & 1) ? p : p_tmp;
+            const float* src = (it & 1) ? p_tmp : p;
+            launch_sweep_t<kInEncoded, kOutEncoded>(sh, st, dst, src, div_u, mat, nx, ny, nz, c2,
+                                                    dt, mat_fluid, mat_air);
+        }
+        const int it = n_iter - 1;
+        float* dst = (it & 1) ? p : p_tmp;
+        const float* src = (it & 1) ? p_tmp : p;
+        launch_sweep_t<kInEncoded, kOutDecoded>(sh, st, dst, src, div_u, mat, nx, ny, nz, c2, dt,
+                                                mat_fluid, mat_air);
+        if ((n_iter & 1) == 0) launch_decode(st, p_tmp, n, vec);
+    }
+    BS_RETURN_IF_ERR(hipGetLastError());
+    if (n_iter & 1) {
+        // Odd count: the last sweep wrote p_tmp; the result must end in p
+        // (jacobi_pressure.cu:262-268).
+        BS_RETURN_IF_ERR(hipMemcpyAsync(p, p_tmp, sizeof(float) * (size_t)n,
+                                        hipMemcpyDeviceToDevice, st));
+    }
+    return (int)hipGetLastError();
+}
+
+extern "C" int bs_pressure_solve_launch(float* p, float* p_tmp, const float* div_u, const int* mat,
+                                        int nx, int ny, int nz, float dx, float dt, float rho,
+                                        int n_iter, int mat_fluid, int mat_air) {
+    return bs_jacobi_solve_launch(p, p_tmp, div_u, mat, nx, ny, nz, dx, dt, rho, n_iter, mat_fluid,
+                                  mat_air, 0, nullptr);
+}

@@ -1,0 +1,85 @@
+"""ctypes binding of libboilsim_hip.so, the gfx950 pressure-projection library.
+
+Every entry point takes raw device addresses (Python ints) and returns a
+hipError_t; :func:`check` turns a nonzero code into ``RuntimeError`` the way the
+reference maps ``cudaError_t`` to ``anyhow`` to ``PyRuntimeError``
+(crates/cuda-kernels/src/lib.rs:267-269, crates/sim-core/src/cuda.rs:123).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_float, c_int, c_longlong, c_uint, c_void_p
+
+LIB_NAME = "libboilsim_hip.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+_lib = None
+
+VP = c_void_p  # device pointer
+I = c_int
+F = c_float
+S = c_void_p  # hipStream_t
+
+# name -> argtypes (restype is always c_int = hipError_t unless listed in _RESTYPES)
+SIGNATURES = {
+    "bs_abi_version": [],
+    "bs_error_string": [I],
+    "bs_device_open": [I, c_char_p, I],
+    "bs_device_synchronize": [],
+    "bs_scale_inplace_launch": [VP, I, F],
+    "bs_jacobi_pressure_launch": [VP, VP, VP, VP, I, I, I, F, F, F, I, I],
+    "bs_jacobi_pressure_launch_stream": [VP, VP, VP, VP, I, I, I, F, F, F, I, I, S],
+    "bs_pressure_solve_launch": [VP, VP, VP, VP, I, I, I, F, F, F, I, I, I],
+    "bs_jacobi_solve_launch": [VP, VP, VP, VP, I, I, I, F, F, F, I, I, I, I, S],
+    "bs_divergence_launch": [VP, VP, VP, VP, VP, I, I, I, F, I, S],
+    "bs_subtract_gradient_launch": [VP, VP, VP, VP, VP, I, I, I, F, F, F, I, S],
+    "bs_max_abs_launch": [VP, c_longlong, VP, S],
+    "bs_projection_jacobi_launch": [VP, VP, VP, VP, VP, VP, VP, I, I, I, F, F, F, I, I, I, VP, S],
+}
+
+_RESTYPES = {"bs_error_string": c_char_p}
+
+
+def load():
+    """Load the library once.  Raises RuntimeError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -c \"import __graft_entry__ as g; g.build()\"` "
+            "or `make -C boiling-sim_amd/csrc`."
+        )
+    lib = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPES.get(name, c_int)
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    return sorted(SIGNATURES)
+
+
+def error_string(code: int) -> str:
+    try:
+        s = load().bs_error_string(int(code))
+        return s.decode() if s else "unknown"
+    except Exception:  # pragma: no cover - only on a broken install
+        return "unknown"
+
+
+def check(name: str, rc: int) -> None:
+    if rc != 0:
+        raise RuntimeError(f"{name} returned hipError_t {rc}: {error_string(rc)}")
+
+
+def call(name: str, *args):
+    """Call entry point `name` and raise RuntimeError on a nonzero hipError_t."""
+    rc = getattr(load(), name)(*args)
+    check(name, rc)
+    return rc

@@ -1,0 +1,140 @@
+"""``sim_core.cuda.SimCore`` on MI355X: same Python signatures as the reference's
+PyO3 class (crates/sim-core/src/cuda.rs:17-412), backed by libboilsim_hip.so.
+
+Pointer contract (crates/sim-core/src/cuda.rs:52-60, python/boilingsim/fluid.py:
+646-651): every ``*_ptr`` is a raw device address as a Python int — for example
+``tensor.data_ptr()`` of a ``torch`` tensor on the same device — that points to
+at least ``nx*ny*nz`` elements of the right type.  The caller owns every
+buffer; no method allocates device memory.  Work is queued on the null stream,
+asynchronously, except ``scale_array`` (synchronous, scale.cu:22-34) and
+``pressure_solve_pcg`` (returns the iteration count, so it completes before
+returning).
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+from . import _lib
+
+_I32_MAX = 2**31 - 1
+
+
+def _usize(name: str, v) -> int:
+    """PyO3 ``usize`` extraction followed by the reference's ``try_into::<i32>``
+    check (crates/cuda-kernels/src/lib.rs:250-252)."""
+    v = int(v)
+    if v < 0:
+        raise OverflowError(f"can't convert negative int to unsigned ({name}={v})")
+    if v > _I32_MAX:
+        raise RuntimeError(f"dim {v} overflows i32")
+    return v
+
+
+def _ptr(name: str, v) -> int:
+    v = int(v)
+    if v < 0 or v >= 2**64:
+        raise OverflowError(f"{name}={v} is not a valid u64 device pointer")
+    return v
+
+
+def _i32(name: str, v) -> int:
+    v = int(v)
+    if not (-(2**31) <= v <= _I32_MAX):
+        raise OverflowError(f"{name}={v} does not fit i32")
+    return v
+
+
+class SimCore:
+    """Handle on one HIP device (reference: ``SimCore(device_id)``)."""
+
+    def __init__(self, device_id: int = 0):
+        device_id = _usize("device_id", device_id)
+        buf = ctypes.create_string_buffer(256)
+        rc = _lib.load().bs_device_open(device_id, buf, len(buf))
+        if rc != 0:
+            raise RuntimeError(
+                f"HIP device {device_id} unavailable: hipError_t {rc} ({_lib.error_string(rc)}). "
+                "If you are on a CPU-only machine, sim_core.cuda cannot be used."
+            )
+        self.device_id = device_id
+        self._info = buf.value.decode()
+
+    def device_info(self) -> str:
+        return self._info
+
+    def __repr__(self) -> str:
+        return f"SimCore(device_id={self.device_id})"
+
+    # -- FFI smoke (cuda.rs:51-68) -------------------------------------------------
+    def scale_array(self, device_ptr: int, n: int, scale: float) -> None:
+        n = _usize("n", n)
+        if n == 0:
+            return
+        _lib.call("bs_scale_inplace_launch", _ptr("device_ptr", device_ptr), n, float(scale))
+
+    # -- one Jacobi sweep (cuda.rs:83-126) ----------------------------------------
+    def jacobi_pressure_step(self, p_new_ptr, p_old_ptr, div_u_ptr, mat_ptr, nx, ny, nz,
+                             dx, dt, rho, mat_fluid=0, mat_air=2) -> None:
+        _lib.call(
+            "bs_jacobi_pressure_launch",
+            _ptr("p_new_ptr", p_new_ptr), _ptr("p_old_ptr", p_old_ptr),
+            _ptr("div_u_ptr", div_u_ptr), _ptr("mat_ptr", mat_ptr),
+            _usize("nx", nx), _usize("ny", ny), _usize("nz", nz),
+            float(dx), float(dt), float(rho), _i32("mat_fluid", mat_fluid), _i32("mat_air", mat_air),
+        )
+
+    # -- fused n_iter Jacobi solve (cuda.rs:136-171) -------------------------------
+    def pressure_solve(self, p_ptr, p_tmp_ptr, div_u_ptr, mat_ptr, nx, ny, nz, dx, dt, rho,
+                       n_iter, mat_fluid=0, mat_air=2) -> None:
+        _lib.call(
+            "bs_pressure_solve_launch",
+            _ptr("p_ptr", p_ptr), _ptr("p_tmp_ptr", p_tmp_ptr),
+            _ptr("div_u_ptr", div_u_ptr), _ptr("mat_ptr", mat_ptr),
+            _usize("nx", nx), _usize("ny", ny), _usize("nz", nz),
+            float(dx), float(dt), float(rho), _usize("n_iter", n_iter),
+            _i32("mat_fluid", mat_fluid), _i32("mat_air", mat_air),
+        )
+
+    # -- extensions: the Warp-side kernels of pressure_projection ------------------
+    def compute_divergence(self, div_u_ptr, ux_ptr, uy_ptr, uz_ptr, mat_ptr, nx, ny, nz, dx,
+                           mat_fluid=0, stream=0) -> None:
+        """compute_divergence (python/boilingsim/fluid.py:201-219)."""
+        _lib.call(
+            "bs_divergence_launch",
+            _ptr("div_u_ptr", div_u_ptr), _ptr("ux_ptr", ux_ptr), _ptr("uy_ptr", uy_ptr),
+            _ptr("uz_ptr", uz_ptr), _ptr("mat_ptr", mat_ptr),
+            _usize("nx", nx), _usize("ny", ny), _usize("nz", nz), float(dx),
+            _i32("mat_fluid", mat_fluid), _ptr("stream", stream),
+        )
+
+    def subtract_pressure_gradient(self, ux_ptr, uy_ptr, uz_ptr, p_ptr, mat_ptr, nx, ny, nz,
+                                   dx, dt, rho, mat_fluid=0, stream=0) -> None:
+        """subtract_pressure_gradient_{x,y,z} (fluid.py:279-330) in one pass."""
+        _lib.call(
+            "bs_subtract_gradient_launch",
+            _ptr("ux_ptr", ux_ptr), _ptr("uy_ptr", uy_ptr), _ptr("uz_ptr", uz_ptr),
+            _ptr("p_ptr", p_ptr), _ptr("mat_ptr", mat_ptr),
+            _usize("nx", nx), _usize("ny", ny), _usize("nz", nz),
+            float(dx), float(dt), float(rho), _i32("mat_fluid", mat_fluid), _ptr("stream", stream),
+        )
+
+    def max_abs(self, x_ptr, n, out_bits_ptr, stream=0) -> None:
+        """max|x| left on the device as float bits in a uint32 at out_bits_ptr."""
+        _lib.call("bs_max_abs_launch", _ptr("x_ptr", x_ptr), int(n), _ptr("out_bits_ptr", out_bits_ptr),
+                  _ptr("stream", stream))
+
+    def projection_jacobi(self, p_ptr, p_tmp_ptr, div_u_ptr, ux_ptr, uy_ptr, uz_ptr, mat_ptr,
+                          nx, ny, nz, dx, dt, rho, n_iter, max_div_bits_ptr, mat_fluid=0,
+                          mat_air=2, stream=0) -> None:
+        """The whole Jacobi pressure_projection (fluid.py:612-734) in one call;
+        max|div| is left on the device (uint32 float bits at max_div_bits_ptr)."""
+        _lib.call(
+            "bs_projection_jacobi_launch",
+            _ptr("p_ptr", p_ptr), _ptr("p_tmp_ptr", p_tmp_ptr), _ptr("div_u_ptr", div_u_ptr),
+            _ptr("ux_ptr", ux_ptr), _ptr("uy_ptr", uy_ptr), _ptr("uz_ptr", uz_ptr),
+            _ptr("mat_ptr", mat_ptr), _usize("nx", nx), _usize("ny", ny), _usize("nz", nz),
+            float(dx), float(dt), float(rho), _usize("n_iter", n_iter),
+            _i32("mat_fluid", mat_fluid), _i32("mat_air", mat_air),
+            _ptr("max_div_bits_ptr", max_div_bits_ptr), _ptr("stream", stream),
+        )
