@@ -1,0 +1,220 @@
+"""pressure_projection on MI355X — the host driver of the reference's
+python/boilingsim/fluid.py:500-734, over torch device tensors and the HIP
+library behind ``sim_core.cuda.SimCore``.
+
+Solver selection keeps the reference's environment contract
+(fluid.py:585-609): ``BOILINGSIM_USE_RUST_PRESSURE=1`` selects the fused
+Jacobi solve, ``BOILINGSIM_PRESSURE_SOLVER`` is ``jacobi`` (default) or
+``cg-rust`` (the reference-semantics PCG), and setting both raises
+``ValueError``.  Two values are added: ``pcg-hip`` (Jacobi-PCG to a true
+relative residual ``pressure_tol``, FP64 residual check) and ``lpcg-hip``
+(flexible PCG with the learned preconditioner, falling back to ``pcg-hip``).
+The Warp kernels of the reference (divergence, gradient subtraction) run as
+HIP kernels here, and the returned max|div| is reduced on the device instead of
+copying the whole field to the host (fluid.py:733-734).
+"""
+
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .geometry import MAT_AIR, MAT_FLUID
+
+VALID_SOLVERS = ("jacobi", "cg-rust", "pcg-hip", "lpcg-hip")
+
+
+@dataclass
+class SolverConfig:
+    """The two knobs the projection reads (python/boilingsim/config.py:161-190)."""
+
+    pressure_tol: float = 1e-5
+    pressure_max_iter: int = 200
+
+
+@dataclass
+class ScenarioConfig:
+    solver: SolverConfig = field(default_factory=SolverConfig)
+
+
+@dataclass
+class Grid:
+    """Cell fields (nx,ny,nz) and MAC faces, as torch tensors on one device
+    (python/boilingsim/geometry.py:254-315)."""
+
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    p: object
+    mat: object
+    ux: object
+    uy: object
+    uz: object
+
+    @property
+    def shape(self) -> tuple[int, int, int]:
+        return (self.nx, self.ny, self.nz)
+
+
+@dataclass
+class FluidWorkspace:
+    """Projection scratch (fluid.py:500-525, the fields the projection uses)."""
+
+    div_u: object
+    p_tmp: object
+    max_bits: object  # int32[1]: max|div| as float bits
+
+
+def make_grid(mat: np.ndarray, ux: np.ndarray, uy: np.ndarray, uz: np.ndarray, dx: float,
+              device: str = "cuda") -> Grid:
+    import torch
+
+    nx, ny, nz = mat.shape
+    t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(device)
+    return Grid(nx, ny, nz, float(dx), torch.zeros((nx, ny, nz), dtype=torch.float32, device=device),
+                t(mat, np.int32), t(ux, np.float32), t(uy, np.float32), t(uz, np.float32))
+
+
+def allocate_fluid_workspace(grid: Grid) -> FluidWorkspace:
+    import torch
+
+    dev = grid.p.device
+    return FluidWorkspace(
+        div_u=torch.zeros(grid.shape, dtype=torch.float32, device=dev),
+        p_tmp=torch.zeros(grid.shape, dtype=torch.float32, device=dev),
+        max_bits=torch.zeros(1, dtype=torch.int32, device=dev),
+    )
+
+
+_SIM_CORE = None
+
+
+def _get_sim_core():
+    """Process-global SimCore(0), as _get_rust_sim (fluid.py:533-547)."""
+    global _SIM_CORE
+    if _SIM_CORE is None:
+        import sim_core
+
+        if sim_core.cuda is None:
+            raise RuntimeError(
+                "The HIP pressure path needs sim_core.cuda, which is unavailable (no HIP device, "
+                "or libboilsim_hip.so not built). Build it with __graft_entry__.build()."
+            )
+        _SIM_CORE = sim_core.cuda.SimCore(0)
+    return _SIM_CORE
+
+
+_PCG_WORKSPACE = {"shape": None, "buffers": None}
+
+
+def _get_pcg_workspace(shape, device="cuda"):
+    """The reference's PCG workspace (fluid.py:557-582): 5 n-vectors, a
+    1024-float dot workspace and 7 one-float scalars, cached by shape."""
+    import torch
+
+    if _PCG_WORKSPACE["shape"] == tuple(shape) and _PCG_WORKSPACE["buffers"] is not None:
+        return _PCG_WORKSPACE["buffers"]
+    z = lambda *s: torch.zeros(s, dtype=torch.float32, device=device)
+    bufs = {k: z(*shape) for k in ("b", "r", "z", "p_search", "Ap")}
+    bufs["dot_ws"] = z(1024)
+    for k in ("alpha", "beta", "rzold", "rznew", "bsq", "rsq", "pAp"):
+        bufs[k] = z(1)
+    _PCG_WORKSPACE["shape"] = tuple(shape)
+    _PCG_WORKSPACE["buffers"] = bufs
+    return bufs
+
+
+_SOLVE_WORKSPACE = {"shape": None, "buf": None}
+
+
+def _get_solve_workspace(core, shape, device="cuda"):
+    import torch
+
+    if _SOLVE_WORKSPACE["shape"] == tuple(shape):
+        return _SOLVE_WORKSPACE["buf"]
+    nb = core.pcg_workspace_bytes(*shape)
+    buf = torch.empty(nb, dtype=torch.uint8, device=device)
+    _SOLVE_WORKSPACE["shape"] = tuple(shape)
+    _SOLVE_WORKSPACE["buf"] = buf
+    return buf
+
+
+def _select_pressure_solver() -> str:
+    """"jacobi", "rust-jacobi", "cg-rust", "pcg-hip" or "lpcg-hip"
+    (fluid.py:585-609, extended)."""
+    use_rust_jacobi = os.environ.get("BOILINGSIM_USE_RUST_PRESSURE") == "1"
+    solver = os.environ.get("BOILINGSIM_PRESSURE_SOLVER", "jacobi").lower()
+    if solver not in VALID_SOLVERS:
+        raise ValueError(
+            f"BOILINGSIM_PRESSURE_SOLVER={solver!r} is not recognised; valid values are "
+            "'jacobi' (default), 'cg-rust', 'pcg-hip' or 'lpcg-hip'."
+        )
+    if use_rust_jacobi and solver != "jacobi":
+        raise ValueError(
+            "BOILINGSIM_USE_RUST_PRESSURE=1 and BOILINGSIM_PRESSURE_SOLVER="
+            f"{solver} are mutually exclusive -- they route through different code paths. "
+            "Pick exactly one."
+        )
+    if solver != "jacobi":
+        return solver
+    if use_rust_jacobi:
+        return "rust-jacobi"
+    return "jacobi"
+
+
+LAST_SOLVE_STATS: dict = {}
+
+
+def pressure_projection(grid: Grid, ws: FluidWorkspace, cfg: ScenarioConfig, dt: float, rho: float,
+                        device: str = "cuda", preconditioner=None) -> float:
+    """Enforce incompressibility (fluid.py:612-734); returns max|div u| after
+    the correction.  ``preconditioner`` is the trained network used by
+    ``lpcg-hip``."""
+    core = _get_sim_core()
+    nx, ny, nz = grid.shape
+    dx = grid.dx
+    solver = _select_pressure_solver()
+    mi = int(cfg.solver.pressure_max_iter)
+    if solver in ("jacobi", "rust-jacobi"):
+        # One fused call: zero p, divergence, sweeps, gradient, divergence, max.
+        core.projection_jacobi(grid.p.data_ptr(), ws.p_tmp.data_ptr(), ws.div_u.data_ptr(),
+                               grid.ux.data_ptr(), grid.uy.data_ptr(), grid.uz.data_ptr(),
+                               grid.mat.data_ptr(), nx, ny, nz, dx, dt, rho, mi, ws.max_bits.data_ptr(),
+                               MAT_FLUID, MAT_AIR)
+    else:
+        core.compute_divergence(ws.div_u.data_ptr(), grid.ux.data_ptr(), grid.uy.data_ptr(),
+                                grid.uz.data_ptr(), grid.mat.data_ptr(), nx, ny, nz, dx, MAT_FLUID)
+        if solver == "cg-rust":
+            w = _get_pcg_workspace(grid.shape, grid.p.device)
+            iters = core.pressure_solve_pcg(
+                grid.p.data_ptr(), ws.div_u.data_ptr(), grid.mat.data_ptr(), nx, ny, nz, dx, dt, rho,
+                float(cfg.solver.pressure_tol), mi,
+                *(w[k].data_ptr() for k in ("b", "r", "z", "p_search", "Ap", "dot_ws", "alpha", "beta",
+                                            "rzold", "rznew", "bsq", "rsq", "pAp")),
+                MAT_FLUID, MAT_AIR)
+            LAST_SOLVE_STATS.clear()
+            LAST_SOLVE_STATS.update({"solver": solver, "iterations": iters})
+        elif solver == "pcg-hip":
+            buf = _get_solve_workspace(core, grid.shape, grid.p.device)
+            st = core.pcg_solve(grid.p.data_ptr(), ws.div_u.data_ptr(), grid.mat.data_ptr(), nx, ny, nz,
+                                dx, dt, rho, float(cfg.solver.pressure_tol), mi, buf.data_ptr(),
+                                buf.numel(), warm_start=False)
+            LAST_SOLVE_STATS.clear()
+            LAST_SOLVE_STATS.update({"solver": solver, **st})
+        else:  # lpcg-hip
+            from .learned import learned_pcg_solve
+
+            st = learned_pcg_solve(core, grid, ws, dt, rho, float(cfg.solver.pressure_tol), mi,
+                                   preconditioner)
+            LAST_SOLVE_STATS.clear()
+            LAST_SOLVE_STATS.update({"solver": solver, **st})
+        core.subtract_pressure_gradient(grid.ux.data_ptr(), grid.uy.data_ptr(), grid.uz.data_ptr(),
+                                        grid.p.data_ptr(), grid.mat.data_ptr(), nx, ny, nz, dx, dt, rho,
+                                        MAT_FLUID)
+        core.compute_divergence(ws.div_u.data_ptr(), grid.ux.data_ptr(), grid.uy.data_ptr(),
+                                grid.uz.data_ptr(), grid.mat.data_ptr(), nx, ny, nz, dx, MAT_FLUID)
+        core.max_abs(ws.div_u.data_ptr(), grid.nx * grid.ny * grid.nz, ws.max_bits.data_ptr())
+    return float(ws.max_bits.cpu().numpy().view(np.float32)[0])

@@ -22,6 +22,7 @@
 // sweep writes the decoded form (0 at every non-fluid cell, as the reference
 // does), and the other ping-pong buffer is decoded in place, so both caller
 // buffers end exactly as the reference leaves them.
+#include "boilsim_hip.h"
 #include "common.hpp"
 
 namespace bs {

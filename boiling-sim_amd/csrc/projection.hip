@@ -7,6 +7,7 @@
 //   divergence: ux, uy, uz, mat read, div written   -> 16 + 4 B/cell
 //   gradient:   p, mat read, 3 faces read+written   -> 28 + 4 B/cell
 //   max|div|:   div read                            ->  4 B/cell
+#include "boilsim_hip.h"
 #include "common.hpp"
 
 namespace bs {

@@ -1,5 +1,6 @@
 // Boundary helpers: the FFI smoke kernel (crates/cuda-kernels/src/scale.cu:11-34),
 // error strings and device queries for the Python shim.
+#include "boilsim_hip.h"
 #include "common.hpp"
 
 #include <stdio.h>

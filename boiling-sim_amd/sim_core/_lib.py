@@ -10,12 +10,29 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, c_char_p, c_float, c_int, c_longlong, c_uint, c_void_p
+from ctypes import POINTER, Structure, c_char_p, c_double, c_float, c_int, c_longlong, c_size_t, c_void_p
 
 LIB_NAME = "libboilsim_hip.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
 _lib = None
+
+
+class SolveStats(Structure):
+    """bs_solve_stats (include/boilsim_hip.h)."""
+
+    _fields_ = [
+        ("iterations", c_int),
+        ("converged", c_int),
+        ("restarts", c_int),
+        ("reserved", c_int),
+        ("relres_recursive", c_double),
+        ("relres_true", c_double),
+        ("bnorm", c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 VP = c_void_p  # device pointer
 I = c_int
@@ -37,9 +54,21 @@ SIGNATURES = {
     "bs_subtract_gradient_launch": [VP, VP, VP, VP, VP, I, I, I, F, F, F, I, S],
     "bs_max_abs_launch": [VP, c_longlong, VP, S],
     "bs_projection_jacobi_launch": [VP, VP, VP, VP, VP, VP, VP, I, I, I, F, F, F, I, I, I, VP, S],
+    "bs_pressure_solve_pcg_launch": [VP, VP, VP, I, I, I, F, F, F, I, I, F, I] + [VP] * 13 + [POINTER(c_int)],
+    "bs_pcg_workspace_bytes": [I, I, I],
+    "bs_pcg_solve": [VP, VP, VP, I, I, I, F, F, F, I, I, c_double, I, I, I, VP, c_size_t, S, c_void_p],
+    "bs_laplacian_spmv_launch": [VP, VP, VP, I, I, I, I, I],
+    "bs_diag_inverse_apply_launch": [VP, VP, VP, I, I, I, I, I],
+    "bs_dot_launch": [VP, VP, I, VP, VP],
+    "bs_dot_launch_f64": [VP, VP, I, VP, VP, VP],
+    "bs_axpy_device_launch": [VP, VP, VP, I, I],
+    "bs_scaled_axpy_launch": [VP, VP, VP, VP, I],
+    "bs_divide_scalars_launch": [VP, VP, VP],
+    "bs_compute_b_launch": [VP, VP, VP, I, I, I, F, I],
+    "bs_zero_buffer_launch": [VP, I],
 }
 
-_RESTYPES = {"bs_error_string": c_char_p}
+_RESTYPES = {"bs_error_string": c_char_p, "bs_pcg_workspace_bytes": c_size_t}
 
 
 def load():

@@ -138,3 +138,50 @@ class SimCore:
             _i32("mat_fluid", mat_fluid), _i32("mat_air", mat_air),
             _ptr("max_div_bits_ptr", max_div_bits_ptr), _ptr("stream", stream),
         )
+
+    # -- PCG (cuda.rs:368-412) ------------------------------------------------------
+    def pressure_solve_pcg(self, p_out_ptr, div_u_ptr, mat_ptr, nx, ny, nz, dx, dt, rho,
+                           pressure_tol, max_iter, ws_b_ptr, ws_r_ptr, ws_z_ptr,
+                           ws_p_search_ptr, ws_ap_ptr, dot_workspace_ptr, dev_alpha_ptr,
+                           dev_beta_ptr, dev_rzold_ptr, dev_rznew_ptr, dev_bsq_ptr, dev_rsq_ptr,
+                           dev_pap_ptr, mat_fluid=0, mat_air=2) -> int:
+        """Jacobi-PCG with the reference's workspace layout and control flow;
+        returns the iteration count.  FP64 dots replace the reference's FP32
+        ones; ws_z carries the class codes and reduction partials."""
+        iters = ctypes.c_int(0)
+        _lib.call(
+            "bs_pressure_solve_pcg_launch",
+            _ptr("p_out_ptr", p_out_ptr), _ptr("div_u_ptr", div_u_ptr), _ptr("mat_ptr", mat_ptr),
+            _usize("nx", nx), _usize("ny", ny), _usize("nz", nz),
+            float(dx), float(dt), float(rho), _i32("mat_fluid", mat_fluid), _i32("mat_air", mat_air),
+            float(pressure_tol), _usize("max_iter", max_iter),
+            _ptr("ws_b_ptr", ws_b_ptr), _ptr("ws_r_ptr", ws_r_ptr), _ptr("ws_z_ptr", ws_z_ptr),
+            _ptr("ws_p_search_ptr", ws_p_search_ptr), _ptr("ws_ap_ptr", ws_ap_ptr),
+            _ptr("dot_workspace_ptr", dot_workspace_ptr), _ptr("dev_alpha_ptr", dev_alpha_ptr),
+            _ptr("dev_beta_ptr", dev_beta_ptr), _ptr("dev_rzold_ptr", dev_rzold_ptr),
+            _ptr("dev_rznew_ptr", dev_rznew_ptr), _ptr("dev_bsq_ptr", dev_bsq_ptr),
+            _ptr("dev_rsq_ptr", dev_rsq_ptr), _ptr("dev_pap_ptr", dev_pap_ptr),
+            ctypes.byref(iters),
+        )
+        return int(iters.value)
+
+    @staticmethod
+    def pcg_workspace_bytes(nx, ny, nz) -> int:
+        return int(_lib.load().bs_pcg_workspace_bytes(_usize("nx", nx), _usize("ny", ny), _usize("nz", nz)))
+
+    def pcg_solve(self, x_ptr, div_u_ptr, mat_ptr, nx, ny, nz, dx, dt, rho, tol, max_iter,
+                  workspace_ptr, workspace_bytes, warm_start=False, max_restarts=4, mat_fluid=0,
+                  mat_air=2, stream=0) -> dict:
+        """Jacobi-PCG to a true relative residual ``tol`` (FP64 residual check,
+        residual replacement).  Returns the solve statistics as a dict."""
+        st = _lib.SolveStats()
+        _lib.call(
+            "bs_pcg_solve",
+            _ptr("x_ptr", x_ptr), _ptr("div_u_ptr", div_u_ptr), _ptr("mat_ptr", mat_ptr),
+            _usize("nx", nx), _usize("ny", ny), _usize("nz", nz),
+            float(dx), float(dt), float(rho), _i32("mat_fluid", mat_fluid), _i32("mat_air", mat_air),
+            float(tol), _usize("max_iter", max_iter), 1 if warm_start else 0, int(max_restarts),
+            _ptr("workspace_ptr", workspace_ptr), int(workspace_bytes), _ptr("stream", stream),
+            ctypes.byref(st),
+        )
+        return st.as_dict()

@@ -1,0 +1,154 @@
+/*
+ * libboilsim_hip.so — C ABI of the MI355X (gfx950) pressure projection.
+ *
+ * Plain pointers and sizes only.  Every pointer argument is a device address
+ * in the calling process on the current HIP device; cell fields are
+ * C-contiguous (nx, ny, nz) with k fastest, faces are ux (nx+1,ny,nz),
+ * uy (nx,ny+1,nz), uz (nx,ny,nz+1) — the layout of the reference's Warp arrays
+ * (python/boilingsim/geometry.py:254-282).  The library allocates no device
+ * memory: the caller owns every buffer, as in the reference
+ * (crates/cuda-kernels/src/pressure_solve_pcg.cu:82-88).  Return values are
+ * hipError_t codes (0 = success), as the reference's launchers return
+ * cudaError_t.  Entry points without a `stream` argument use the null stream,
+ * like the reference's default-stream launchers.
+ */
+#ifndef BOILSIM_HIP_H
+#define BOILSIM_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Opaque stream handle; identical to hipStream_t. NULL = the null stream. */
+typedef struct ihipStream_t* bs_stream_t;
+
+/* ---- boundary helpers ---------------------------------------------------- */
+
+/* ABI revision of this header. */
+int bs_abi_version(void);
+/* hipGetErrorString for a code returned by this library. */
+const char* bs_error_string(int code);
+/* hipSetDevice(device_id) and a one-line description (SimCore::new / device_info,
+ * crates/sim-core/src/cuda.rs:25-40). */
+int bs_device_open(int device_id, char* buf, int buflen);
+int bs_device_synchronize(void);
+/* In-place d_a[i] *= scale, synchronous.  Replaces scale_inplace_launch
+ * (crates/cuda-kernels/src/scale.cu:22-34, lib.rs:23). */
+int bs_scale_inplace_launch(float* d_a, int n, float scale);
+
+/* ---- Jacobi pressure solve ------------------------------------------------- */
+
+/* One masked 7-point Jacobi sweep, bit-identical to jacobi_pressure_step
+ * (python/boilingsim/fluid.py:243-276).  Replaces jacobi_pressure_launch
+ * (crates/cuda-kernels/src/jacobi_pressure.cu:184-208, lib.rs:32-45).
+ * p_new and p_old must not overlap.  Asynchronous. */
+int bs_jacobi_pressure_launch(float* p_new, const float* p_old, const float* div_u, const int* mat,
+                              int nx, int ny, int nz, float dx, float dt, float rho, int mat_fluid,
+                              int mat_air);
+int bs_jacobi_pressure_launch_stream(float* p_new, const float* p_old, const float* div_u,
+                                     const int* mat, int nx, int ny, int nz, float dx, float dt,
+                                     float rho, int mat_fluid, int mat_air, bs_stream_t stream);
+
+/* n_iter ping-pong sweeps; the result ends in p whatever the parity of n_iter,
+ * and p_tmp ends as the reference leaves it.  Replaces pressure_solve_launch
+ * (crates/cuda-kernels/src/jacobi_pressure.cu:228-270, lib.rs:52-66).
+ * p and p_tmp must be disjoint.  Asynchronous. */
+int bs_pressure_solve_launch(float* p, float* p_tmp, const float* div_u, const int* mat, int nx,
+                             int ny, int nz, float dx, float dt, float rho, int n_iter,
+                             int mat_fluid, int mat_air);
+/* Same with a stream; zero_init != 0 promises p == 0 on entry (the
+ * projection's cold start, fluid.py:627-629) so the first sweep skips reading
+ * it. */
+int bs_jacobi_solve_launch(float* p, float* p_tmp, const float* div_u, const int* mat, int nx,
+                           int ny, int nz, float dx, float dt, float rho, int n_iter, int mat_fluid,
+                           int mat_air, int zero_init, bs_stream_t stream);
+
+/* ---- projection kernels (the Warp side of pressure_projection) ------------ */
+
+/* compute_divergence (fluid.py:201-219). */
+int bs_divergence_launch(float* div_u, const float* ux, const float* uy, const float* uz,
+                         const int* mat, int nx, int ny, int nz, float dx, int mat_fluid,
+                         bs_stream_t stream);
+/* subtract_pressure_gradient_{x,y,z} (fluid.py:279-330), one launch. */
+int bs_subtract_gradient_launch(float* ux, float* uy, float* uz, const float* p, const int* mat,
+                                int nx, int ny, int nz, float dx, float dt, float rho, int mat_fluid,
+                                bs_stream_t stream);
+/* max|x| as float bits in *out_bits (device uint32); replaces the full-field
+ * copy of fluid.py:733-734. */
+int bs_max_abs_launch(const float* x, long long n, unsigned int* out_bits, bs_stream_t stream);
+/* pressure_projection with the Jacobi solver (fluid.py:612-734): zero p/p_tmp,
+ * divergence, n_iter sweeps, gradient, divergence, max|div| -> *max_div_bits. */
+int bs_projection_jacobi_launch(float* p, float* p_tmp, float* div_u, float* ux, float* uy, float* uz,
+                                const int* mat, int nx, int ny, int nz, float dx, float dt, float rho,
+                                int n_iter, int mat_fluid, int mat_air, unsigned int* max_div_bits,
+                                bs_stream_t stream);
+
+/* ---- PCG ------------------------------------------------------------------ */
+
+/* Jacobi-preconditioned CG with the reference's control flow and workspace
+ * layout; FP32 vectors, FP64 dots/scalars.  Replaces pressure_solve_pcg_launch
+ * (crates/cuda-kernels/src/pressure_solve_pcg.cu:89-118, lib.rs:131-153).
+ * ws_* are n floats each, dot_workspace >= 1024 floats, the seven scalar slots
+ * 1 float each (they receive float copies of the final scalars).  Synchronous:
+ * returns after the solve with the iteration count in *iter_count_host. */
+int bs_pressure_solve_pcg_launch(float* p_out, const float* div_u, const int* mat, int nx, int ny,
+                                 int nz, float dx, float dt, float rho, int mat_fluid, int mat_air,
+                                 float pressure_tol, int max_iter, float* ws_b, float* ws_r,
+                                 float* ws_z, float* ws_p_search, float* ws_Ap, float* dot_workspace,
+                                 float* dev_alpha, float* dev_beta, float* dev_rzold,
+                                 float* dev_rznew, float* dev_bsq, float* dev_rsq, float* dev_pAp,
+                                 int* iter_count_host);
+
+/* Statistics of bs_pcg_solve. */
+typedef struct {
+    int iterations;
+    int converged;
+    int restarts;
+    int reserved;
+    double relres_recursive; /* sqrt(r.r / b.b) of the recursive residual */
+    double relres_true;      /* ||b - A x||_2 / ||b||_2, FP64 accumulation */
+    double bnorm;            /* ||b||_2 */
+} bs_solve_stats;
+
+/* Bytes of workspace bs_pcg_solve needs for an nx*ny*nz grid. */
+size_t bs_pcg_workspace_bytes(int nx, int ny, int nz);
+/* Jacobi-PCG to a TRUE relative residual tol (new; the reference tests only
+ * its recursive FP32 residual, pressure_solve_pcg.cu:205-216): convergence is
+ * tested every iteration on device, then ||b - A x|| is formed in FP64 and the
+ * solve restarts from x (residual replacement) while it is above tol, at most
+ * max_restarts times.  warm_start != 0 starts from x's contents. */
+int bs_pcg_solve(float* x, const float* div_u, const int* mat, int nx, int ny, int nz, float dx,
+                 float dt, float rho, int mat_fluid, int mat_air, double tol, int max_iter,
+                 int warm_start, int max_restarts, void* workspace, size_t workspace_bytes,
+                 bs_stream_t stream, bs_solve_stats* stats);
+
+/* ---- the reference PCG's building blocks, standalone ---------------------- */
+
+/* Ap = 6p - S(p), non-fluid rows 0 (laplacian_spmv.cu:145-163). */
+int bs_laplacian_spmv_launch(float* Ap, const float* p, const int* mat, int nx, int ny, int nz,
+                             int mat_fluid, int mat_air);
+/* z = r / (6 - #solid neighbours), z = r if that is <= 0, non-fluid 0
+ * (diag_inverse_apply.cu:91-111). */
+int bs_diag_inverse_apply_launch(float* z, const float* r, const int* mat, int nx, int ny, int nz,
+                                 int mat_fluid, int mat_air);
+/* *result = sum x*y accumulated in FP64 (dot_reduce.cu:93-115, which sums in
+ * FP32); workspace >= 1024 floats. */
+int bs_dot_launch(const float* x, const float* y, int n, float* workspace, float* result);
+int bs_dot_launch_f64(const float* x, const float* y, int n, float* workspace, float* result,
+                      double* result64);
+/* axpy_device.cu:34-45, 66-77, 93-100, 126-138, 142-145. */
+int bs_axpy_device_launch(float* y, const float* x, const float* alpha_ptr, int n, int negate_alpha);
+int bs_scaled_axpy_launch(float* p_out, const float* z, const float* p_old, const float* beta_ptr,
+                          int n);
+int bs_divide_scalars_launch(const float* num_ptr, const float* den_ptr, float* result_ptr);
+int bs_compute_b_launch(float* b, const float* div_u, const int* mat, int nx, int ny, int nz,
+                        float scale, int mat_fluid);
+int bs_zero_buffer_launch(float* buf, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BOILSIM_HIP_H */
