@@ -1,0 +1,239 @@
+// Tile mapping, vector loads/stores, the one-byte cell code and the
+// deterministic partial-sum helpers shared by the Krylov and multigrid kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace bs {
+
+// A tile is 256 threads x VEC consecutive cells of one (j,k) plane chunk at
+// one i.  Grid-stride kernels give each workgroup a contiguous run of tiles
+// ordered i-fastest, so a workgroup walks down i for a fixed plane chunk and
+// the i-1 neighbour plane it needs was loaded by itself one tile earlier.
+struct Tiles {
+    int nx, ny, nz;
+    FastDiv nzv;      // nz / VEC
+    uint32_t chunks;  // 256-thread chunks per plane
+    uint32_t ntiles;  // chunks * nx
+};
+
+inline Tiles make_tiles(int nx, int ny, int nz, int vec) {
+    Tiles T;
+    T.nx = nx; T.ny = ny; T.nz = nz;
+    const uint32_t nzv = (uint32_t)(nz / vec);
+    T.nzv = make_fastdiv(nzv);
+    T.chunks = (uint32_t)(((uint64_t)ny * nzv + 255) / 256);
+    T.ntiles = T.chunks * (uint32_t)nx;
+    return T;
+}
+
+inline int vec_for(int nz, std::initializer_list<const void*> ptrs) {
+    int vec = (nz % 4 == 0) ? 4 : (nz % 2 == 0) ? 2 : 1;
+    for (const void* q : ptrs) {
+        const uintptr_t a = (uintptr_t)q;
+        while (vec > 1 && (a % (uintptr_t)(4 * vec)) != 0) vec >>= 1;
+    }
+    return vec;
+}
+
+__device__ __forceinline__ void tile_span(uint32_t ntiles, uint32_t& t0, uint32_t& t1) {
+    const uint32_t w = xcd_remap(blockIdx.x, gridDim.x);
+    t0 = (uint32_t)(((uint64_t)w * ntiles) / gridDim.x);
+    t1 = (uint32_t)(((uint64_t)(w + 1) * ntiles) / gridDim.x);
+}
+
+// Tile t -> (i, j, k0) for this thread; false when the thread is past the plane.
+template <int VEC>
+__device__ __forceinline__ bool tile_cell(const Tiles& T, uint32_t t, int& i, int& j, int& k0) {
+    const uint32_t chunk = t / (uint32_t)T.nx;
+    i = (int)(t - chunk * (uint32_t)T.nx);
+    const uint32_t q = chunk * 256u + threadIdx.x;
+    if (q >= (uint32_t)T.ny * T.nzv.d) return false;
+    j = (int)fdiv(q, T.nzv);
+    k0 = (int)(q - (uint32_t)j * T.nzv.d) * VEC;
+    return true;
+}
+
+template <int VEC>
+__device__ __forceinline__ void ldf(const float* __restrict__ p, int64_t off, float (&v)[VEC]) {
+    if constexpr (VEC == 4) {
+        const float4 t = *reinterpret_cast<const float4*>(p + off);
+        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+    } else if constexpr (VEC == 2) {
+        const float2 t = *reinterpret_cast<const float2*>(p + off);
+        v[0] = t.x; v[1] = t.y;
+    } else {
+        v[0] = p[off];
+    }
+}
+
+template <int VEC>
+__device__ __forceinline__ void stf(float* __restrict__ p, int64_t off, const float (&v)[VEC]) {
+    if constexpr (VEC == 4) {
+        *reinterpret_cast<float4*>(p + off) = make_float4(v[0], v[1], v[2], v[3]);
+    } else if constexpr (VEC == 2) {
+        *reinterpret_cast<float2*>(p + off) = make_float2(v[0], v[1]);
+    } else {
+        p[off] = v[0];
+    }
+}
+
+template <int VEC>
+__device__ __forceinline__ void ldc(const uint8_t* __restrict__ p, int64_t off, uint8_t (&v)[VEC]) {
+    if constexpr (VEC == 4) {
+        const uint32_t t = *reinterpret_cast<const uint32_t*>(p + off);
+        v[0] = t & 0xFF; v[1] = (t >> 8) & 0xFF; v[2] = (t >> 16) & 0xFF; v[3] = t >> 24;
+    } else if constexpr (VEC == 2) {
+        const uint16_t t = *reinterpret_cast<const uint16_t*>(p + off);
+        v[0] = t & 0xFF; v[1] = t >> 8;
+    } else {
+        v[0] = p[off];
+    }
+}
+
+template <int VEC>
+__device__ __forceinline__ void stc(uint8_t* __restrict__ p, int64_t off, const uint8_t (&v)[VEC]) {
+    if constexpr (VEC == 4) {
+        *reinterpret_cast<uint32_t*>(p + off) =
+            (uint32_t)v[0] | ((uint32_t)v[1] << 8) | ((uint32_t)v[2] << 16) | ((uint32_t)v[3] << 24);
+    } else if constexpr (VEC == 2) {
+        *reinterpret_cast<uint16_t*>(p + off) = (uint16_t)(v[0] | (v[1] << 8));
+    } else {
+        p[off] = v[0];
+    }
+}
+
+// z = M^-1 r for one cell from its code (diag_inverse_apply.cu:63-89).
+__device__ __forceinline__ float precond(float r, uint8_t code) {
+    if (code < kCodeFluid0) return 0.0f;
+    const int dg = 8 - (int)code;  // 6 - n_solid
+    return dg > 0 ? r / (float)dg : r;
+}
+
+// Neighbour term of the SpMV from the neighbour's code (fluid.py:222-240).
+__device__ __forceinline__ float nbr_term(uint8_t c, float v, float ps) {
+    return c >= kCodeFluid0 ? v : (c == kCodeSolid ? ps : 0.0f);
+}
+
+// Loads of a 7-point neighbourhood for VEC cells (values and codes); out-of-
+// bounds neighbours read as air.
+template <int VEC>
+struct Nbhd {
+    uint8_t cc[VEC], cl[VEC], cr[VEC], cd[VEC], cu[VEC], cb, cf;
+    float vc[VEC], vl[VEC], vr[VEC], vd[VEC], vu[VEC], vb, vf;
+
+    __device__ __forceinline__ void load(const Tiles& T, const uint8_t* __restrict__ code,
+                                         const float* __restrict__ v, int i, int j, int k0, int64_t c0) {
+        const int64_t plane = (int64_t)T.ny * T.nz;
+        ldc<VEC>(code, c0, cc);
+        ldf<VEC>(v, c0, vc);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            cl[e] = cr[e] = cd[e] = cu[e] = kCodeAir;
+            vl[e] = vr[e] = vd[e] = vu[e] = 0.0f;
+        }
+        if (i > 0) { ldc<VEC>(code, c0 - plane, cl); ldf<VEC>(v, c0 - plane, vl); }
+        if (i < T.nx - 1) { ldc<VEC>(code, c0 + plane, cr); ldf<VEC>(v, c0 + plane, vr); }
+        if (j > 0) { ldc<VEC>(code, c0 - T.nz, cd); ldf<VEC>(v, c0 - T.nz, vd); }
+        if (j < T.ny - 1) { ldc<VEC>(code, c0 + T.nz, cu); ldf<VEC>(v, c0 + T.nz, vu); }
+        cb = cf = kCodeAir;
+        vb = vf = 0.0f;
+        if (k0 > 0) { cb = code[c0 - 1]; vb = v[c0 - 1]; }
+        if (k0 + VEC < T.nz) { cf = code[c0 + VEC]; vf = v[c0 + VEC]; }
+    }
+
+    // 6*v - S(v) at element e in the reference's float order
+    // (laplacian_spmv.cu:120-142); 0 for a non-fluid centre.
+    __device__ __forceinline__ float apply(int e) const {
+        if (cc[e] < kCodeFluid0) return 0.0f;
+        const float ps = vc[e];
+        const uint8_t c_b = e > 0 ? cc[e - 1] : cb;
+        const float v_b = e > 0 ? vc[e - 1] : vb;
+        const uint8_t c_f = e < VEC - 1 ? cc[e + 1] : cf;
+        const float v_f = e < VEC - 1 ? vc[e + 1] : vf;
+        float s = nbr_term(cl[e], vl[e], ps) + nbr_term(cr[e], vr[e], ps);
+        s = s + nbr_term(cd[e], vd[e], ps);
+        s = s + nbr_term(cu[e], vu[e], ps);
+        s = s + nbr_term(c_b, v_b, ps);
+        s = s + nbr_term(c_f, v_f, ps);
+        return 6.0f * ps - s;
+    }
+
+    // Same operator with the neighbour sum and the product in FP64.
+    __device__ __forceinline__ double apply64(int e) const {
+        if (cc[e] < kCodeFluid0) return 0.0;
+        const float ps = vc[e];
+        const uint8_t c_b = e > 0 ? cc[e - 1] : cb;
+        const float v_b = e > 0 ? vc[e - 1] : vb;
+        const uint8_t c_f = e < VEC - 1 ? cc[e + 1] : cf;
+        const float v_f = e < VEC - 1 ? vc[e + 1] : vf;
+        double s = (double)nbr_term(cl[e], vl[e], ps) + (double)nbr_term(cr[e], vr[e], ps);
+        s += (double)nbr_term(cd[e], vd[e], ps);
+        s += (double)nbr_term(cu[e], vu[e], ps);
+        s += (double)nbr_term(c_b, v_b, ps);
+        s += (double)nbr_term(c_f, v_f, ps);
+        return 6.0 * (double)ps - s;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Deterministic reductions without inter-workgroup fences: a kernel writes one
+// FP64 partial per workgroup (value k of workgroup w at k*G + w); the kernel
+// that needs the total sums the G partials itself, every workgroup in the same
+// fixed order, so all agree bit for bit.  The stream order between the two
+// kernels provides visibility, which avoids the per-workgroup L2 write-back an
+// agent-scope fence costs on a multi-XCD device.
+// ---------------------------------------------------------------------------
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ double wave_sum64(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Workgroup sums of NV values; results valid in every thread.
+template <int NV>
+__device__ __forceinline__ void block_sums(double (&v)[NV]) {
+    __shared__ double red[NV][kThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const double s = wave_sum64(v[k]);
+        if (lane == 0) red[k][wave] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double s = red[k][0];
+#pragma unroll
+        for (int w = 1; w < kThreads / 64; ++w) s += red[k][w];
+        v[k] = s;
+    }
+    __syncthreads();
+}
+
+// Writes this workgroup's NV partials.
+template <int NV>
+__device__ __forceinline__ void write_partials(double (&v)[NV], double* __restrict__ partials) {
+    block_sums<NV>(v);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) partials[(size_t)k * gridDim.x + blockIdx.x] = v[k];
+    }
+}
+
+// Sums G partials per value (written by an earlier kernel) into out[].
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double* __restrict__ partials, unsigned G,
+                                             double (&out)[NV]) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double acc = 0.0;
+        for (unsigned w = threadIdx.x; w < G; w += kThreads) acc += partials[(size_t)k * G + w];
+        out[k] = acc;
+    }
+    block_sums<NV>(out);
+}
+
+}  // namespace bs
