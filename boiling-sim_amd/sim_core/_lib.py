@@ -66,9 +66,23 @@ SIGNATURES = {
     "bs_divide_scalars_launch": [VP, VP, VP],
     "bs_compute_b_launch": [VP, VP, VP, I, I, I, F, I],
     "bs_zero_buffer_launch": [VP, I],
+    "bs_unet_num_params": [],
+    "bs_unet_packed_bytes": [],
+    "bs_unet_workspace_bytes": [I, I, I, I],
+    "bs_unet_pack": [VP, VP, S],
+    "bs_build_codes": [VP, I, I, I, I, I, VP, S],
+    "bs_unet_forward": [VP, VP, VP, VP, I, I, I, c_double, VP, VP, c_size_t, I, S],
+    "bs_unet_backward": [VP, VP, VP, VP, VP, I, I, I, VP, VP, VP, c_size_t, S],
+    "bs_unet_adam": [VP, VP, VP, VP, I, F, F, S],
 }
 
-_RESTYPES = {"bs_error_string": c_char_p, "bs_pcg_workspace_bytes": c_size_t}
+_RESTYPES = {
+    "bs_error_string": c_char_p,
+    "bs_pcg_workspace_bytes": c_size_t,
+    "bs_unet_num_params": c_size_t,
+    "bs_unet_packed_bytes": c_size_t,
+    "bs_unet_workspace_bytes": c_size_t,
+}
 
 
 def load():

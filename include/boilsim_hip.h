@@ -147,6 +147,39 @@ int bs_compute_b_launch(float* b, const float* div_u, const int* mat, int nx, in
                         float scale, int mat_fluid);
 int bs_zero_buffer_launch(float* buf, int n);
 
+
+/* ---- learned preconditioner (no counterpart in the reference) -------------- */
+
+/* A 3-level 3D conv U-Net (channels 8/16/32, 3x3x3, BF16 activations and
+ * weights, FP32 accumulation on MFMA) mapping a residual r to a correction z;
+ * the parameter layout is, per conv, W[27][cin][cout] then bias[cout] (FP32). */
+size_t bs_unet_num_params(void);
+/* Bytes of the packed BF16 weight fragments (forward and backward). */
+size_t bs_unet_packed_bytes(void);
+/* Workspace bytes for an nx*ny*nz grid; training != 0 adds the backward buffers. */
+size_t bs_unet_workspace_bytes(int nx, int ny, int nz, int training);
+/* Packs FP32 params into the MFMA fragment order (after every weight update). */
+int bs_unet_pack(const float* params, void* packed, bs_stream_t stream);
+/* One-byte class codes from the int32 mask: 0 air, 1 solid, 2 + #solid
+ * neighbours for fluid (out of bounds = air). */
+int bs_build_codes(const int* mat, int nx, int ny, int nz, int mat_fluid, int mat_air,
+                   unsigned char* code, bs_stream_t stream);
+/* z = rms(r) * U-Net(r / rms(r), classes) on fluid cells, 0 elsewhere;
+ * n_fluid = number of fluid cells. */
+int bs_unet_forward(const float* params, const void* packed, const float* r, const unsigned char* code,
+                    int nx, int ny, int nz, double n_fluid, float* z, void* workspace,
+                    size_t workspace_bytes, int training, bs_stream_t stream);
+/* After bs_unet_forward(training = 1): accumulates into grads the gradient of
+ * (z-e)^T A (z-e) / (e.r) for the target correction e (A e = r); loss_dev
+ * (2 device doubles) receives the numerator and e.r. */
+int bs_unet_backward(const float* params, const void* packed, const float* r, const float* e,
+                     const unsigned char* code, int nx, int ny, int nz, float* grads,
+                     double* loss_dev, void* workspace, size_t workspace_bytes, bs_stream_t stream);
+/* Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on FP32 params; gscale
+ * multiplies the gradient (1/batch). */
+int bs_unet_adam(float* params, const float* grads, float* m, float* v, int step, float lr,
+                 float gscale, bs_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
