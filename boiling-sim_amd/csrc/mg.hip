@@ -1,0 +1,392 @@
+// Multigrid-augmented learned preconditioner for the masked pressure Poisson
+// operator (A = 6I - S with the _pressure_neighbour rules, python/boilingsim/
+// fluid.py:222-240): one V-cycle over the fine grid and its 2x coarsening
+// with damped-Jacobi smoothing, the 4x-coarsened grid solved approximately by
+// the trained U-Net (or, for comparison, by damped Jacobi sweeps).
+//
+// Coarse classes: a coarse cell is fluid if any child is fluid, else air if
+// any child is air, else solid; the coarse operator is the same stencil on
+// the coarse classes.  Restriction r_c = 4 * mean over the 8 children (the
+// 4 = (2h/h)^2 keeps the unscaled stencil consistent across levels);
+// prolongation is cell-centred trilinear (weights 3/4, 1/4 per axis).
+// Smoother: z += w D^-1 (r - A z), w = 0.8, two sweeps before and after.
+// The network's correction on the coarsest grid is polished with damped-Jacobi
+// sweeps there, which keeps the preconditioner positive where the network
+// alone is not.
+#include "boilsim_hip.h"
+#include "grid.hpp"
+#include "precond_api.hpp"
+
+#include <string.h>
+
+namespace bs {
+
+struct MLvl {
+    int nx, ny, nz;
+    int64_t n;
+    FastDiv fz, fyz;
+};
+
+static MLvl mk(int nx, int ny, int nz) {
+    MLvl L;
+    L.nx = nx; L.ny = ny; L.nz = nz;
+    L.n = (int64_t)nx * ny * nz;
+    L.fz = make_fastdiv((uint32_t)nz);
+    L.fyz = make_fastdiv((uint32_t)(ny * nz));
+    return L;
+}
+
+__device__ __forceinline__ void ijk(const MLvl& L, uint32_t c, int& i, int& j, int& k) {
+    i = (int)fdiv(c, L.fyz);
+    const uint32_t rem = c - (uint32_t)i * (uint32_t)(L.ny * L.nz);
+    j = (int)fdiv(rem, L.fz);
+    k = (int)(rem - (uint32_t)j * (uint32_t)L.nz);
+}
+
+constexpr float kOmega = 0.8f;
+constexpr unsigned kStepWg = 512;
+
+__device__ __forceinline__ float inv_diag(uint8_t code) {
+    const int dg = 8 - (int)code;
+    return dg > 0 ? 1.0f / (float)dg : 1.0f;
+}
+
+// (A v)[c] with codes, out of bounds = air.
+__device__ __forceinline__ float apply_a(const MLvl& L, const uint8_t* __restrict__ code,
+                                         const float* __restrict__ v, int64_t c, int i, int j, int k) {
+    const uint8_t cc = code[c];
+    if (cc < kCodeFluid0) return 0.0f;
+    const float ps = v[c];
+    const int64_t plane = (int64_t)L.ny * L.nz;
+    float s = 0.0f;
+    if (i > 0) s += nbr_term(code[c - plane], v[c - plane], ps);
+    if (i < L.nx - 1) s += nbr_term(code[c + plane], v[c + plane], ps);
+    if (j > 0) s += nbr_term(code[c - L.nz], v[c - L.nz], ps);
+    if (j < L.ny - 1) s += nbr_term(code[c + L.nz], v[c + L.nz], ps);
+    if (k > 0) s += nbr_term(code[c - 1], v[c - 1], ps);
+    if (k < L.nz - 1) s += nbr_term(code[c + 1], v[c + 1], ps);
+    return 6.0f * ps - s;
+}
+
+// Coarse class ids (0 fluid, 1 solid, 2 air) from the fine codes.
+__global__ __launch_bounds__(256) void coarse_class_kernel(MLvl F, MLvl C, const uint8_t* __restrict__ fcode,
+                                                           int* __restrict__ cmat) {
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t cc = (int64_t)blockIdx.x * 256 + threadIdx.x; cc < C.n; cc += stride) {
+        int I, J, K;
+        ijk(C, (uint32_t)cc, I, J, K);
+        bool fl = false, air = false;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int d = 0; d < 2; ++d) {
+                    const int i = 2 * I + a, j = 2 * J + b, k = 2 * K + d;
+                    if (i >= F.nx || j >= F.ny || k >= F.nz) continue;
+                    const uint8_t q = fcode[((int64_t)i * F.ny + j) * F.nz + k];
+                    fl |= q >= kCodeFluid0;
+                    air |= q == kCodeAir;
+                }
+        cmat[cc] = fl ? 0 : (air ? 2 : 1);
+    }
+}
+
+__global__ __launch_bounds__(256) void codes_from_mat_kernel(MLvl L, const int* __restrict__ mat, int mf,
+                                                             int ma, uint8_t* __restrict__ code,
+                                                             unsigned int* __restrict__ nfluid) {
+    const int64_t plane = (int64_t)L.ny * L.nz;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    unsigned cnt = 0;
+    for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < L.n; c += stride) {
+        const int m = mat[c];
+        if (m != mf) { code[c] = m == ma ? kCodeAir : kCodeSolid; continue; }
+        ++cnt;
+        int i, j, k;
+        ijk(L, (uint32_t)c, i, j, k);
+        int ns = 0;
+        auto solid = [&](int64_t o) { const int q = mat[o]; return (q != mf && q != ma) ? 1 : 0; };
+        if (i > 0) ns += solid(c - plane);
+        if (i < L.nx - 1) ns += solid(c + plane);
+        if (j > 0) ns += solid(c - L.nz);
+        if (j < L.ny - 1) ns += solid(c + L.nz);
+        if (k > 0) ns += solid(c - 1);
+        if (k < L.nz - 1) ns += solid(c + 1);
+        code[c] = (uint8_t)(kCodeFluid0 + ns);
+    }
+    if (nfluid) atomicAdd(nfluid, cnt);
+}
+
+// z = w D^-1 r (first sweep from z = 0).
+__global__ __launch_bounds__(256) void jacobi0_kernel(MLvl L, const uint8_t* __restrict__ code,
+                                                      const float* __restrict__ r, float* __restrict__ z,
+                                                      const int* __restrict__ done) {
+    if (done && *done) return;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < L.n; c += stride) {
+        const uint8_t cc = code[c];
+        z[c] = cc >= kCodeFluid0 ? kOmega * r[c] * inv_diag(cc) : 0.0f;
+    }
+}
+
+// zo = zi + w D^-1 (r - A zi).
+__global__ __launch_bounds__(256) void jacobi_kernel(MLvl L, const uint8_t* __restrict__ code,
+                                                     const float* __restrict__ r, const float* __restrict__ zi,
+                                                     float* __restrict__ zo, const int* __restrict__ done) {
+    if (done && *done) return;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < L.n; c += stride) {
+        const uint8_t cc = code[c];
+        if (cc < kCodeFluid0) { zo[c] = 0.0f; continue; }
+        int i, j, k;
+        ijk(L, (uint32_t)c, i, j, k);
+        const float az = apply_a(L, code, zi, c, i, j, k);
+        zo[c] = zi[c] + kOmega * (r[c] - az) * inv_diag(cc);
+    }
+}
+
+// rc[parent] = 4 * mean over its children of (r - A z) (fluid children; the
+// others contribute 0).  Coarse-centred, so no atomics.
+__global__ __launch_bounds__(256) void restrict_kernel(MLvl F, MLvl C, const uint8_t* __restrict__ fcode,
+                                                       const float* __restrict__ r, const float* __restrict__ z,
+                                                       float* __restrict__ rc, const int* __restrict__ done) {
+    if (done && *done) return;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t cc = (int64_t)blockIdx.x * 256 + threadIdx.x; cc < C.n; cc += stride) {
+        int I, J, K;
+        ijk(C, (uint32_t)cc, I, J, K);
+        float s = 0.0f;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int d = 0; d < 2; ++d) {
+                    const int i = 2 * I + a, j = 2 * J + b, k = 2 * K + d;
+                    if (i >= F.nx || j >= F.ny || k >= F.nz) continue;
+                    const int64_t f = ((int64_t)i * F.ny + j) * F.nz + k;
+                    if (fcode[f] < kCodeFluid0) continue;
+                    s += r[f] - (z ? apply_a(F, fcode, z, f, i, j, k) : 0.0f);
+                }
+        rc[cc] = 0.5f * s;
+    }
+}
+
+// res = r - A z (fluid cells), 0 elsewhere.
+__global__ __launch_bounds__(256) void residual_kernel(MLvl L, const uint8_t* __restrict__ code,
+                                                      const float* __restrict__ r, const float* __restrict__ z,
+                                                      float* __restrict__ res, const int* __restrict__ done) {
+    if (done && *done) return;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < L.n; c += stride) {
+        if (code[c] < kCodeFluid0) { res[c] = 0.0f; continue; }
+        int i, j, k;
+        ijk(L, (uint32_t)c, i, j, k);
+        res[c] = r[c] - apply_a(L, code, z, c, i, j, k);
+    }
+}
+
+// z += e on every cell.
+__global__ __launch_bounds__(256) void add_kernel(int64_t n, const float* __restrict__ e, float* __restrict__ z,
+                                                  const int* __restrict__ done) {
+    if (done && *done) return;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < n; c += stride) z[c] += e[c];
+}
+
+// Partials of r.e and e.(A e) for the optimal step along e.
+__global__ __launch_bounds__(256) void step_partials_kernel(MLvl L, const uint8_t* __restrict__ code,
+                                                            const float* __restrict__ r,
+                                                            const float* __restrict__ e,
+                                                            double* __restrict__ part, const int* __restrict__ done) {
+    if (done && *done) return;
+    double re = 0.0, eae = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < L.n; c += stride) {
+        if (code[c] < kCodeFluid0) continue;
+        int i, j, k;
+        ijk(L, (uint32_t)c, i, j, k);
+        const double ev = e[c];
+        re += (double)r[c] * ev;
+        eae += ev * (double)apply_a(L, code, e, c, i, j, k);
+    }
+    double v[2] = {re, eae};
+    write_partials<2>(v, part);
+}
+
+// e *= gamma, gamma = (r.e)/(e.A e), the A-norm-optimal multiple of e as a
+// correction for A x = r; gamma = 0 when either product is not positive.
+__global__ __launch_bounds__(256) void step_scale_kernel(int64_t n, const double* __restrict__ part, unsigned G,
+                                                         float* __restrict__ e, const int* __restrict__ done) {
+    if (done && *done) return;
+    double v[2];
+    sum_partials<2>(part, G, v);
+    const float g = (v[0] > 0.0 && v[1] > 0.0) ? (float)(v[0] / v[1]) : 0.0f;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < n; c += stride) e[c] *= g;
+}
+
+// z[fine] += trilinear(ec)[fine] on fluid cells (cell-centred weights 3/4, 1/4).
+__global__ __launch_bounds__(256) void prolong_kernel(MLvl F, MLvl C, const uint8_t* __restrict__ fcode,
+                                                      const float* __restrict__ ec, float* __restrict__ z,
+                                                      const int* __restrict__ done) {
+    if (done && *done) return;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    const int64_t cplane = (int64_t)C.ny * C.nz;
+    for (int64_t f = (int64_t)blockIdx.x * 256 + threadIdx.x; f < F.n; f += stride) {
+        if (fcode[f] < kCodeFluid0) continue;
+        int i, j, k;
+        ijk(F, (uint32_t)f, i, j, k);
+        const int I = i >> 1, J = j >> 1, K = k >> 1;
+        int I2 = I + ((i & 1) ? 1 : -1), J2 = J + ((j & 1) ? 1 : -1), K2 = K + ((k & 1) ? 1 : -1);
+        I2 = I2 < 0 ? 0 : (I2 >= C.nx ? C.nx - 1 : I2);
+        J2 = J2 < 0 ? 0 : (J2 >= C.ny ? C.ny - 1 : J2);
+        K2 = K2 < 0 ? 0 : (K2 >= C.nz ? C.nz - 1 : K2);
+        const int ii[2] = {I, I2}, jj[2] = {J, J2}, kk[2] = {K, K2};
+        const float w[2] = {0.75f, 0.25f};
+        float v = 0.0f;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int d = 0; d < 2; ++d)
+                    v += w[a] * w[b] * w[d] * ec[ii[a] * cplane + (int64_t)jj[b] * C.nz + kk[d]];
+        z[f] += v;
+    }
+}
+
+static inline size_t al(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct MgWs {
+    MLvl L0, L1, L2;
+    float* zt0;               // fine temporary
+    uint8_t *code1, *code2;
+    int *mat1, *mat2;
+    float *r1, *z1, *zt1, *r2, *e2, *zt2, *rs2, *en2;
+    unsigned int* nfl;
+    double* part;  // 2 x kStepWg partials of the optimal-step products
+    void* unet;
+    size_t bytes;
+};
+
+static MgWs carve_mg(void* base, int nx, int ny, int nz) {
+    MgWs W;
+    memset(&W, 0, sizeof(W));
+    W.L0 = mk(nx, ny, nz);
+    W.L1 = mk((nx + 1) / 2, (ny + 1) / 2, (nz + 1) / 2);
+    W.L2 = mk((W.L1.nx + 1) / 2, (W.L1.ny + 1) / 2, (W.L1.nz + 1) / 2);
+    size_t off = 0;
+    auto take = [&](size_t b) -> char* {
+        char* p = base ? (char*)base + off : nullptr;
+        off += al(b);
+        return p;
+    };
+    W.zt0 = (float*)take(W.L0.n * 4);
+    W.code1 = (uint8_t*)take(W.L1.n);
+    W.mat1 = (int*)take(W.L1.n * 4);
+    W.r1 = (float*)take(W.L1.n * 4);
+    W.z1 = (float*)take(W.L1.n * 4);
+    W.zt1 = (float*)take(W.L1.n * 4);
+    W.code2 = (uint8_t*)take(W.L2.n);
+    W.mat2 = (int*)take(W.L2.n * 4);
+    W.r2 = (float*)take(W.L2.n * 4);
+    W.e2 = (float*)take(W.L2.n * 4);
+    W.zt2 = (float*)take(W.L2.n * 4);
+    W.rs2 = (float*)take(W.L2.n * 4);
+    W.en2 = (float*)take(W.L2.n * 4);
+    W.nfl = (unsigned int*)take(64);
+    W.part = (double*)take(2 * kStepWg * sizeof(double));
+    W.unet = take(unet_ws_bytes(W.L2.nx, W.L2.ny, W.L2.nz));
+    W.bytes = off;
+    return W;
+}
+
+static unsigned grid_for(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    if (b > 8192) b = 8192;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+size_t mg_ws_bytes(int nx, int ny, int nz) { return carve_mg(nullptr, nx, ny, nz).bytes; }
+
+static void optimal_step(hipStream_t st, const MLvl& L, const uint8_t* code, const float* r, float* e, double* part,
+                         const int* done) {
+    step_partials_kernel<<<kStepWg, 256, 0, st>>>(L, code, r, e, part, done);
+    step_scale_kernel<<<grid_for(L.n), 256, 0, st>>>(L.n, part, kStepWg, e, done);
+}
+
+size_t step_ws_bytes() { return 2 * kStepWg * sizeof(double); }
+
+void scale_to_optimal(hipStream_t st, int nx, int ny, int nz, const uint8_t* code, const float* r, float* e,
+                      double* part, const int* done) {
+    optimal_step(st, mk(nx, ny, nz), code, r, e, part, done);
+}
+
+__global__ void zero_u32(unsigned int* p) { if (threadIdx.x == 0) *p = 0u; }
+
+int mg_setup(hipStream_t st, void* ws, int nx, int ny, int nz, const int* mat, int mf, int ma,
+             double* n_fluid_coarse) {
+    MgWs W = carve_mg(ws, nx, ny, nz);
+    // Fine codes into a temporary (zt0 is free during set-up), then the two
+    // coarse levels' classes and codes.
+    uint8_t* fcode = (uint8_t*)W.zt0;
+    zero_u32<<<1, 64, 0, st>>>(W.nfl);
+    codes_from_mat_kernel<<<grid_for(W.L0.n), 256, 0, st>>>(W.L0, mat, mf, ma, fcode, nullptr);
+    coarse_class_kernel<<<grid_for(W.L1.n), 256, 0, st>>>(W.L0, W.L1, fcode, W.mat1);
+    codes_from_mat_kernel<<<grid_for(W.L1.n), 256, 0, st>>>(W.L1, W.mat1, 0, 2, W.code1, nullptr);
+    coarse_class_kernel<<<grid_for(W.L2.n), 256, 0, st>>>(W.L1, W.L2, W.code1, W.mat2);
+    codes_from_mat_kernel<<<grid_for(W.L2.n), 256, 0, st>>>(W.L2, W.mat2, 0, 2, W.code2, W.nfl);
+    BS_RETURN_IF_ERR(hipGetLastError());
+    unsigned int h = 0;
+    BS_RETURN_IF_ERR(hipMemcpyAsync(&h, W.nfl, sizeof(h), hipMemcpyDeviceToHost, st));
+    BS_RETURN_IF_ERR(hipStreamSynchronize(st));
+    if (n_fluid_coarse) *n_fluid_coarse = (double)h;
+    return hipSuccess;
+}
+
+int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* code0, const float* r, float* z,
+             int coarse_kind, const float* params, const void* packed, double n_fluid_coarse, const int* done) {
+    MgWs W = carve_mg(ws, nx, ny, nz);
+    const unsigned g0 = grid_for(W.L0.n), g1 = grid_for(W.L1.n), g2 = grid_for(W.L2.n);
+    // fine pre-smoothing: z = wD^-1 r, then one sweep into zt0 -> z
+    jacobi0_kernel<<<g0, 256, 0, st>>>(W.L0, code0, r, W.zt0, done);
+    jacobi_kernel<<<g0, 256, 0, st>>>(W.L0, code0, r, W.zt0, z, done);
+    restrict_kernel<<<g1, 256, 0, st>>>(W.L0, W.L1, code0, r, z, W.r1, done);
+    // level 1
+    jacobi0_kernel<<<g1, 256, 0, st>>>(W.L1, W.code1, W.r1, W.zt1, done);
+    jacobi_kernel<<<g1, 256, 0, st>>>(W.L1, W.code1, W.r1, W.zt1, W.z1, done);
+    restrict_kernel<<<g2, 256, 0, st>>>(W.L1, W.L2, W.code1, W.r1, W.z1, W.r2, done);
+    // level 2: coarse solve.  coarse_kind 0: 11 damped-Jacobi sweeps;
+    // 1: the network, then 10 sweeps polishing its correction; 2: 5 sweeps,
+    // the network on their residual, then 5 sweeps.
+    auto sweeps = [&](int pairs) {
+        for (int s = 0; s < pairs; ++s) {
+            jacobi_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, W.zt2, done);
+            jacobi_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.zt2, W.e2, done);
+        }
+    };
+    if (coarse_kind == 0) {
+        jacobi0_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, done);
+        sweeps(5);
+    } else if (coarse_kind == 1) {
+        const int rc = unet_apply(st, W.unet, W.L2.nx, W.L2.ny, W.L2.nz, params, packed, W.r2, W.code2,
+                                  n_fluid_coarse, nullptr, 0, W.e2, nullptr, nullptr, done);
+        if (rc != hipSuccess) return rc;
+        optimal_step(st, W.L2, W.code2, W.r2, W.e2, W.part, done);
+        sweeps(5);
+    } else {
+        jacobi0_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, done);
+        jacobi_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, W.zt2, done);
+        jacobi_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.zt2, W.e2, done);
+        residual_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, W.rs2, done);
+        const int rc = unet_apply(st, W.unet, W.L2.nx, W.L2.ny, W.L2.nz, params, packed, W.rs2, W.code2,
+                                  n_fluid_coarse, nullptr, 0, W.en2, nullptr, nullptr, done);
+        if (rc != hipSuccess) return rc;
+        optimal_step(st, W.L2, W.code2, W.rs2, W.en2, W.part, done);
+        add_kernel<<<g2, 256, 0, st>>>(W.L2.n, W.en2, W.e2, done);
+        sweeps(2);
+    }
+    // back up: level 1
+    prolong_kernel<<<g1, 256, 0, st>>>(W.L1, W.L2, W.code1, W.e2, W.z1, done);
+    jacobi_kernel<<<g1, 256, 0, st>>>(W.L1, W.code1, W.r1, W.z1, W.zt1, done);
+    jacobi_kernel<<<g1, 256, 0, st>>>(W.L1, W.code1, W.r1, W.zt1, W.z1, done);
+    // fine
+    prolong_kernel<<<g0, 256, 0, st>>>(W.L0, W.L1, code0, W.z1, z, done);
+    jacobi_kernel<<<g0, 256, 0, st>>>(W.L0, code0, r, z, W.zt0, done);
+    jacobi_kernel<<<g0, 256, 0, st>>>(W.L0, code0, r, W.zt0, z, done);
+    return (int)hipGetLastError();
+}
+
+}  // namespace bs

@@ -28,6 +28,7 @@
 // pressure_solve_pcg.cu:205-216).
 #include "boilsim_hip.h"
 #include "grid.hpp"
+#include "precond_api.hpp"
 
 #include <math.h>
 #include <stddef.h>
@@ -51,7 +52,10 @@ struct PcgCtrl {
     int converged;
     int max_iter;
     int mode;        // 0: reference cadence (iter >= 8 && iter % 5 == 0), 1: every iteration
-    int pad[3];
+    int best_it;     // flexible CG: iteration of the best residual so far
+    int fallback;    // flexible CG: stalled or lost positivity
+    int pad;
+    double best_rr;
 };
 static_assert(sizeof(PcgCtrl) <= 256, "control block must fit 256 bytes");
 
@@ -586,6 +590,364 @@ extern "C" int bs_pcg_solve(float* x, const float* div_u, const int* mat, int nx
     }
     if (stats) {
         stats->converged = conv;
+        stats->relres_recursive = h.bsq > 0.0 ? sqrt(h.rr / h.bsq) : 0.0;
+    }
+    return hipSuccess;
+}
+
+// ===========================================================================
+// Flexible CG with a nonlinear preconditioner (the learned one)
+//   z_{k+1} = M(r_{k+1}),  beta = r_{k+1}.(z_{k+1} - z_k) / (r_k.z_k)
+// (Polak-Ribiere form, which keeps CG's short recurrence usable when M varies
+// between iterations), p = z + beta p.  The solve falls back to plain PCG when
+// the residual has not dropped by 2 % for 20 iterations or r.z loses
+// positivity, and always ends with the FP64 true-residual test.
+// ===========================================================================
+namespace bs {
+
+// x += alpha p, r -= alpha Ap; partials of r.r.
+template <int VEC>
+__global__ __launch_bounds__(256) void fcg_update_kernel(Tiles T, const float* __restrict__ p,
+                                                         const float* __restrict__ ap, float* __restrict__ x,
+                                                         float* __restrict__ r, const double* __restrict__ partA,
+                                                         double* __restrict__ partB, PcgCtrl* ctrl, int it) {
+    if (ctrl->done) return;
+    double pap[1];
+    sum_partials<1>(partA, gridDim.x, pap);
+    const double rz = ctrl->rz[it & 1];
+    const double alpha = pap[0] != 0.0 ? rz / pap[0] : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctrl->alpha = alpha;
+        ctrl->pAp = pap[0];
+    }
+    const float af = (float)alpha;
+    uint32_t t0, t1;
+    tile_span(T.ntiles, t0, t1);
+    double rr = 0.0;
+    const int64_t plane = (int64_t)T.ny * T.nz;
+    for (uint32_t t = t0; t < t1; ++t) {
+        int i, j, k0;
+        if (!tile_cell<VEC>(T, t, i, j, k0)) continue;
+        const int64_t c0 = (int64_t)i * plane + (int64_t)j * T.nz + k0;
+        float pv[VEC], av[VEC], xv[VEC], rv[VEC];
+        ldf<VEC>(p, c0, pv);
+        ldf<VEC>(ap, c0, av);
+        ldf<VEC>(x, c0, xv);
+        ldf<VEC>(r, c0, rv);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            xv[e] = xv[e] + af * pv[e];
+            rv[e] = rv[e] - af * av[e];
+            rr += (double)rv[e] * (double)rv[e];
+        }
+        stf<VEC>(x, c0, xv);
+        stf<VEC>(r, c0, rv);
+    }
+    double v[1] = {rr};
+    write_partials<1>(v, partB);
+}
+
+// One workgroup: convergence and stall test after the update of iteration it.
+__global__ __launch_bounds__(256) void fcg_check_kernel(const double* __restrict__ partB, unsigned G, PcgCtrl* ctrl,
+                                                        int it) {
+    if (ctrl->done) return;
+    double v[1];
+    sum_partials<1>(partB, G, v);
+    if (threadIdx.x != 0) return;
+    const double rr = v[0];
+    ctrl->rr = rr;
+    if (rr < ctrl->stop) {
+        ctrl->iter = it + 1;
+        ctrl->converged = 1;
+        ctrl->done = 1;
+        return;
+    }
+    if (rr < 0.98 * ctrl->best_rr) {
+        ctrl->best_rr = rr;
+        ctrl->best_it = it;
+    } else if (it - ctrl->best_it >= 20) {
+        ctrl->iter = it + 1;
+        ctrl->fallback = 1;
+        ctrl->done = 1;
+    }
+}
+
+// Partials of r.z_new and r.z_old (z_old may be null).
+__global__ __launch_bounds__(256) void fcg_dots_kernel(int64_t n, const float* __restrict__ r,
+                                                       const float* __restrict__ z, const float* __restrict__ z_old,
+                                                       double* __restrict__ partZ, const PcgCtrl* ctrl) {
+    if (ctrl->done) return;
+    double a = 0.0, b = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += stride) {
+        a += (double)r[t] * (double)z[t];
+        if (z_old) b += (double)r[t] * (double)z_old[t];
+    }
+    double v[2] = {a, b};
+    write_partials<2>(v, partZ);
+}
+
+// beta = (r.z_new - r.z_old) / r.z; p = z_new + beta p.  init: p = z, r.z only.
+template <int VEC>
+__global__ __launch_bounds__(256) void fcg_direction_kernel(Tiles T, const float* __restrict__ z,
+                                                            float* __restrict__ p, const double* __restrict__ partZ,
+                                                            unsigned G, PcgCtrl* ctrl, int it, int init) {
+    if (ctrl->done) return;
+    double v[2];
+    sum_partials<2>(partZ, G, v);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (!(v[0] > 0.0)) {  // preconditioner not positive on this residual
+        if (lead) {
+            ctrl->iter = it + (init ? 0 : 1);
+            ctrl->fallback = 1;
+            ctrl->done = 1;
+        }
+        return;
+    }
+    double beta = 0.0;
+    if (!init) {
+        const double rz_old = ctrl->rz[it & 1];
+        beta = rz_old != 0.0 ? (v[0] - v[1]) / rz_old : 0.0;
+    }
+    const int slot = init ? (it & 1) : ((it + 1) & 1);
+    if (lead) {
+        ctrl->rz[slot] = v[0];
+        ctrl->rz_new = v[0];
+        ctrl->beta = beta;
+        if (!init) {
+            ctrl->iter = it + 1;
+            if (it + 1 >= ctrl->max_iter) ctrl->done = 1;
+        }
+    }
+    const float bf = (float)beta;
+    uint32_t t0, t1;
+    tile_span(T.ntiles, t0, t1);
+    const int64_t plane = (int64_t)T.ny * T.nz;
+    for (uint32_t t = t0; t < t1; ++t) {
+        int i, j, k0;
+        if (!tile_cell<VEC>(T, t, i, j, k0)) continue;
+        const int64_t c0 = (int64_t)i * plane + (int64_t)j * T.nz + k0;
+        float zv[VEC], pv[VEC];
+        ldf<VEC>(z, c0, zv);
+        if (init) {
+            stf<VEC>(p, c0, zv);
+        } else {
+            ldf<VEC>(p, c0, pv);
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) pv[e] = zv[e] + bf * pv[e];
+            stf<VEC>(p, c0, pv);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void count_fluid_kernel(int64_t n, const uint8_t* __restrict__ code, int* out) {
+    unsigned cnt = 0;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += stride) cnt += code[t] >= kCodeFluid0;
+    atomicAdd((unsigned int*)out, cnt);
+}
+
+__global__ void fcg_prepare_kernel(PcgCtrl* ctrl, int it0) {
+    if (threadIdx.x != 0) return;
+    ctrl->best_rr = 1e300;
+    ctrl->best_it = it0;
+    ctrl->fallback = 0;
+    ctrl->iter = it0;
+}
+
+struct FcgBuffers {
+    PcgBuffers B;
+    float* z[2];
+    double* partZ;
+    void* pws;  // preconditioner workspace
+    int kind;
+    const float* params;
+    const void* packed;
+    double n_fluid;  // of the level the network runs on
+    int nx, ny, nz;
+};
+
+// z = M(r) (+ partials of r.z and r.z_old into partZ).
+static int fcg_precond(hipStream_t st, const FcgBuffers& F, const float* r, float* z, const float* z_old) {
+    const PcgBuffers& B = F.B;
+    const int* done = &B.ctrl->done;
+    const int64_t n = (int64_t)F.nx * F.ny * F.nz;
+    if (F.kind == 1) {
+        // z = gamma N(r) with the A-norm-optimal gamma, so r.z >= 0 always.
+        const int rc = unet_apply(st, F.pws, F.nx, F.ny, F.nz, F.params, F.packed, r, B.code, F.n_fluid, nullptr, 0,
+                                  z, nullptr, nullptr, done);
+        if (rc != hipSuccess) return rc;
+        scale_to_optimal(st, F.nx, F.ny, F.nz, B.code, r, z, F.partZ, done);
+        fcg_dots_kernel<<<kUNetPartWg, 256, 0, st>>>(n, r, z, z_old, F.partZ, B.ctrl);
+        return (int)hipGetLastError();
+    }
+    const int coarse = F.kind == 2 ? 1 : (F.kind == 4 ? 2 : 0);
+    const int rc = mg_apply(st, F.pws, F.nx, F.ny, F.nz, B.code, r, z, coarse, F.params, F.packed, F.n_fluid, done);
+    if (rc != hipSuccess) return rc;
+    fcg_dots_kernel<<<kUNetPartWg, 256, 0, st>>>(n, r, z, z_old, F.partZ, B.ctrl);
+    return (int)hipGetLastError();
+}
+
+static int fcg_iterate(const Tiles& T, int vec, const FcgBuffers& F, int it0, int max_iter, hipStream_t st) {
+    const PcgBuffers& B = F.B;
+    BS_RETURN_IF_ERR((hipError_t)ensure_poll());
+    hipEvent_t ev[2];
+    BS_RETURN_IF_ERR(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    BS_RETURN_IF_ERR(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    const size_t done_off = offsetof(PcgCtrl, done);
+    int it = it0, batch = 2, slot = 0;
+    bool pending[2] = {false, false};
+    int rc = hipSuccess;
+    g_poll.slots[0] = g_poll.slots[2] = 0;
+    while (it < max_iter) {
+        const int nb = (max_iter - it) < batch ? (max_iter - it) : batch;
+        for (int q = 0; q < nb && rc == hipSuccess; ++q, ++it) {
+            BS_VEC_LAUNCH(pcg_spmv_kernel, vec, B.nwg, st, T, B.code, B.p, B.ap, B.partA, B.ctrl);
+            BS_VEC_LAUNCH(fcg_update_kernel, vec, B.nwg, st, T, B.p, B.ap, B.x, B.r, B.partA, B.partB, B.ctrl, it);
+            fcg_check_kernel<<<1, 256, 0, st>>>(B.partB, B.nwg, B.ctrl, it);
+            rc = fcg_precond(st, F, B.r, F.z[(it + 1) & 1], F.z[it & 1]);
+            BS_VEC_LAUNCH(fcg_direction_kernel, vec, B.nwg, st, T, F.z[(it + 1) & 1], B.p, F.partZ, kUNetPartWg,
+                          B.ctrl, it, 0);
+        }
+        if (rc == hipSuccess) rc = hipGetLastError();
+        if (rc != hipSuccess) break;
+        rc = hipMemcpyAsync(g_poll.slots + 2 * slot, (const char*)B.ctrl + done_off, 2 * sizeof(int),
+                            hipMemcpyDeviceToHost, st);
+        if (rc != hipSuccess) break;
+        rc = hipEventRecord(ev[slot], st);
+        if (rc != hipSuccess) break;
+        pending[slot] = true;
+        const int prev = slot ^ 1;
+        if (pending[prev]) {
+            rc = hipEventSynchronize(ev[prev]);
+            if (rc != hipSuccess) break;
+            pending[prev] = false;
+            if (g_poll.slots[2 * prev] != 0) break;
+        }
+        slot ^= 1;
+        if (batch < 16) batch *= 2;
+    }
+    if (rc == hipSuccess) {
+        rc = hipMemcpyAsync(g_poll.final, B.ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, st);
+        if (rc == hipSuccess) rc = hipStreamSynchronize(st);
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    return rc;
+}
+
+static size_t precond_ws_bytes(int nx, int ny, int nz, int kind) {
+    if (kind == 1) return unet_ws_bytes(nx, ny, nz);
+    if (kind >= 2) return mg_ws_bytes(nx, ny, nz);
+    return 0;
+}
+
+}  // namespace bs
+
+extern "C" size_t bs_fcg_workspace_bytes(int nx, int ny, int nz, int precond) {
+    if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
+    const size_t n = (size_t)nx * ny * nz;
+    return bs_pcg_workspace_bytes(nx, ny, nz) + 2 * align256(n * 4) + align256(2 * kUNetPartWg * sizeof(double)) +
+           align256(precond_ws_bytes(nx, ny, nz, precond));
+}
+
+// Flexible CG with preconditioner `precond` (1: U-Net on this grid, 2: V-cycle
+// with the U-Net on the 4x-coarsened grid followed by Jacobi polishing, 3: the
+// same V-cycle with Jacobi sweeps there instead of the network, 4: Jacobi
+// sweeps, the U-Net on their residual, Jacobi sweeps), to a true relative
+// residual tol; on a stall it continues with plain Jacobi-PCG from x.
+extern "C" int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx, int ny, int nz, float dx,
+                            float dt, float rho, int mat_fluid, int mat_air, double tol, int max_iter,
+                            int warm_start, int precond, const float* unet_params, const void* unet_packed,
+                            void* workspace, size_t workspace_bytes, hipStream_t st, bs_fcg_stats* stats) {
+    if (stats) memset(stats, 0, sizeof(*stats));
+    if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
+    if (precond < 1 || precond > 4) return (int)hipErrorInvalidValue;
+    if (precond != 3 && (!unet_params || !unet_packed)) return (int)hipErrorInvalidValue;
+    if (workspace_bytes < bs_fcg_workspace_bytes(nx, ny, nz, precond)) return (int)hipErrorInvalidValue;
+    BS_RETURN_IF_ERR((hipError_t)ensure_poll());
+    const size_t n = (size_t)nx * ny * nz;
+    FcgBuffers F;
+    carve(workspace, n, F.B);
+    PcgBuffers& B = F.B;
+    char* c = (char*)workspace + bs_pcg_workspace_bytes(nx, ny, nz);
+    F.z[0] = (float*)c; c += align256(n * 4);
+    F.z[1] = (float*)c; c += align256(n * 4);
+    F.partZ = (double*)c; c += align256(2 * kUNetPartWg * sizeof(double));
+    F.pws = c;
+    F.kind = precond;
+    F.params = unet_params;
+    F.packed = unet_packed;
+    F.nx = nx; F.ny = ny; F.nz = nz;
+    B.x = x;
+    const int vec = vec_for(nz, {x, div_u, mat, B.b, B.r, B.p, B.ap, B.code, F.z[0], F.z[1]});
+    const Tiles T = make_tiles(nx, ny, nz, vec);
+    B.nwg = pick_nwg(T, kMaxWg);
+    B.partB = B.partA + B.nwg;
+    B.partS = B.partA + 3 * B.nwg;
+    const float bscale = -rho * dx * dx / dt;
+    pcg_ctrl_init_kernel<<<1, 64, 0, st>>>(B.ctrl, tol * tol, max_iter, 1);
+    BS_VEC_LAUNCH(pcg_setup_kernel, vec, B.nwg, st, T, mat, div_u, bscale, mat_fluid, mat_air,
+                  warm_start ? 0 : 1, B.code, B.b, B.x, B.r, B.p, B.partS);
+    pcg_setup_finalize<<<1, 256, 0, st>>>(B.partS, B.nwg, B.ctrl, warm_start ? 0 : 1);
+    if (warm_start) {
+        BS_VEC_LAUNCH(pcg_residual_kernel, vec, B.nwg, st, T, B.code, B.b, B.x, B.r, B.p, B.partS, 1);
+        pcg_residual_finalize<<<1, 256, 0, st>>>(B.partS, B.nwg, B.ctrl, 1, 0);
+    }
+    BS_RETURN_IF_ERR(hipGetLastError());
+    if (precond == 1) {
+        // The network normalises r by its RMS over the fluid cells of this grid.
+        count_fluid_kernel<<<256, 256, 0, st>>>((int64_t)n, B.code, &B.ctrl->pad);
+        BS_RETURN_IF_ERR(hipMemcpyAsync(g_poll.final, B.ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, st));
+        BS_RETURN_IF_ERR(hipStreamSynchronize(st));
+        F.n_fluid = (double)(unsigned)g_poll.final->pad;
+    }
+    if (precond >= 2) {
+        const int rc = mg_setup(st, F.pws, nx, ny, nz, mat, mat_fluid, mat_air, &F.n_fluid);
+        if (rc != hipSuccess) return rc;
+    }
+    fcg_prepare_kernel<<<1, 64, 0, st>>>(B.ctrl, 0);
+    int rc = fcg_precond(st, F, B.r, F.z[0], nullptr);
+    if (rc != hipSuccess) return rc;
+    BS_VEC_LAUNCH(fcg_direction_kernel, vec, B.nwg, st, T, F.z[0], B.p, F.partZ, kUNetPartWg, B.ctrl, 0, 1);
+    BS_RETURN_IF_ERR(hipGetLastError());
+    rc = fcg_iterate(T, vec, F, 0, max_iter, st);
+    if (rc != hipSuccess) return rc;
+    PcgCtrl h = *g_poll.final;
+    const int fcg_iters = h.iter;
+    const int fell_back = h.fallback;
+    // True residual; plain PCG (residual replacement) while it is above tol.
+    int restarts = 0, it0 = h.iter;
+    for (;;) {
+        BS_VEC_LAUNCH(pcg_residual_kernel, vec, B.nwg, st, T, B.code, B.b, B.x, B.ap, B.p, B.partS, 0);
+        pcg_residual_finalize<<<1, 256, 0, st>>>(B.partS, B.nwg, B.ctrl, 0, 0);
+        BS_RETURN_IF_ERR(hipGetLastError());
+        BS_RETURN_IF_ERR(hipMemcpyAsync(g_poll.final, B.ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, st));
+        BS_RETURN_IF_ERR(hipStreamSynchronize(st));
+        h = *g_poll.final;
+        const double rel_true = h.bsq > 0.0 ? sqrt(h.rr_true / h.bsq) : 0.0;
+        if (stats) {
+            stats->relres_true = rel_true;
+            stats->bnorm = sqrt(h.bsq);
+        }
+        if (h.bsq == 0.0 || rel_true <= tol) {
+            if (stats) stats->converged = 1;
+            break;
+        }
+        if (restarts >= 8 || it0 >= max_iter) break;
+        ++restarts;
+        BS_VEC_LAUNCH(pcg_residual_kernel, vec, B.nwg, st, T, B.code, B.b, B.x, B.r, B.p, B.partS, 1);
+        pcg_residual_finalize<<<1, 256, 0, st>>>(B.partS, B.nwg, B.ctrl, 1, it0);
+        BS_RETURN_IF_ERR(hipGetLastError());
+        rc = pcg_iterate(T, vec, B, it0, max_iter, st);
+        if (rc != hipSuccess) return rc;
+        it0 = g_poll.final->iter;
+    }
+    if (stats) {
+        stats->iterations = fcg_iters;
+        stats->fallback = fell_back;
+        stats->pcg_iterations = it0 - fcg_iters;
+        stats->restarts = restarts;
         stats->relres_recursive = h.bsq > 0.0 ? sqrt(h.rr / h.bsq) : 0.0;
     }
     return hipSuccess;

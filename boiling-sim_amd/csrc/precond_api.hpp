@@ -1,0 +1,40 @@
+// Internal interfaces between the Krylov driver (pcg.hip), the U-Net
+// (unet.hip) and the multigrid hierarchy (mg.hip).  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace bs {
+
+constexpr unsigned kUNetPartWg = 1024;  // workgroups of the U-Net's reduction partials
+
+// Workspace of one U-Net application on an nx*ny*nz grid (inference only).
+size_t unet_ws_bytes(int nx, int ny, int nz);
+
+// z = N(r) on fluid cells.  rr_part (G partials of r.r) may be null; rz_part
+// (2 x kUNetPartWg partials of r.z and r.z_old) is written when not null.
+// Every kernel returns at once when *done != 0.
+int unet_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const float* params, const void* packed,
+               const float* r, const uint8_t* code, double n_fluid, const double* rr_part, unsigned G,
+               float* z, const float* z_old, double* rz_part, const int* done);
+
+// Multigrid-augmented preconditioner: damped-Jacobi V-cycle on the fine grid
+// and its 2x coarsening, with the 4x-coarsened level solved by the U-Net
+// (coarse_kind 1) or by damped Jacobi sweeps (coarse_kind 0).
+size_t mg_ws_bytes(int nx, int ny, int nz);
+// Builds the coarse classes and codes from the fine mask; returns the fluid
+// count of the coarsest level through *n_fluid_coarse (synchronises once).
+int mg_setup(hipStream_t st, void* ws, int nx, int ny, int nz, const int* mat, int mf, int ma,
+             double* n_fluid_coarse);
+int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* code0, const float* r, float* z,
+             int coarse_kind, const float* params, const void* packed, double n_fluid_coarse, const int* done);
+
+// e *= (r.e)/(e.A e) (0 when not positive): the A-norm-optimal multiple of a
+// correction e for A x = r.  part: step_ws_bytes() of device scratch.
+size_t step_ws_bytes();
+void scale_to_optimal(hipStream_t st, int nx, int ny, int nz, const uint8_t* code, const float* r, float* e,
+                      double* part, const int* done);
+
+}  // namespace bs

@@ -934,3 +934,22 @@ extern "C" int bs_unet_adam(float* params, const float* grads, float* m, float* 
     adam_kernel<<<(n + 255) / 256, 256, 0, stream>>>(params, grads, m, v, n, lr, b1, b2, eps, bc1, bc2, gscale);
     return (int)hipGetLastError();
 }
+
+// ---- internal interface for the Krylov driver (precond_api.hpp) ----------
+#include "precond_api.hpp"
+
+namespace bs {
+
+static_assert(kUNetPartWg == kPartWg, "partial counts must agree");
+
+size_t unet_ws_bytes(int nx, int ny, int nz) { return carve_unet(nullptr, nx, ny, nz, 0).bytes; }
+
+int unet_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const float* params, const void* packed,
+               const float* r, const uint8_t* code, double n_fluid, const double* rr_part, unsigned G,
+               float* z, const float* z_old, double* rz_part, const int* done) {
+    UNetWs W = carve_unet(ws, nx, ny, nz, 0);
+    return unet_forward(st, W, params, (const __bf16*)packed, r, code, n_fluid, rr_part, G, z, z_old, rz_part,
+                        done, 0);
+}
+
+}  // namespace bs

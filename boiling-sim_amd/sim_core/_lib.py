@@ -34,6 +34,27 @@ class SolveStats(Structure):
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
+
+
+class FcgStats(Structure):
+    """bs_fcg_stats (include/boilsim_hip.h)."""
+
+    _fields_ = [
+        ("iterations", c_int),
+        ("converged", c_int),
+        ("fallback", c_int),
+        ("pcg_iterations", c_int),
+        ("restarts", c_int),
+        ("reserved", c_int),
+        ("relres_recursive", c_double),
+        ("relres_true", c_double),
+        ("bnorm", c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 VP = c_void_p  # device pointer
 I = c_int
 F = c_float
@@ -74,6 +95,8 @@ SIGNATURES = {
     "bs_unet_forward": [VP, VP, VP, VP, I, I, I, c_double, VP, VP, c_size_t, I, S],
     "bs_unet_backward": [VP, VP, VP, VP, VP, I, I, I, VP, VP, VP, c_size_t, S],
     "bs_unet_adam": [VP, VP, VP, VP, I, F, F, S],
+    "bs_fcg_workspace_bytes": [I, I, I, I],
+    "bs_fcg_solve": [VP, VP, VP, I, I, I, F, F, F, I, I, c_double, I, I, I, VP, VP, VP, c_size_t, S, c_void_p],
 }
 
 _RESTYPES = {
@@ -82,6 +105,7 @@ _RESTYPES = {
     "bs_unet_num_params": c_size_t,
     "bs_unet_packed_bytes": c_size_t,
     "bs_unet_workspace_bytes": c_size_t,
+    "bs_fcg_workspace_bytes": c_size_t,
 }
 
 

@@ -185,3 +185,28 @@ class SimCore:
             ctypes.byref(st),
         )
         return st.as_dict()
+
+    # -- flexible CG with the learned preconditioner (new) ------------------------
+    @staticmethod
+    def fcg_workspace_bytes(nx, ny, nz, precond) -> int:
+        return int(_lib.load().bs_fcg_workspace_bytes(_usize("nx", nx), _usize("ny", ny), _usize("nz", nz),
+                                                      int(precond)))
+
+    def fcg_solve(self, x_ptr, div_u_ptr, mat_ptr, nx, ny, nz, dx, dt, rho, tol, max_iter, precond,
+                  unet_params_ptr, unet_packed_ptr, workspace_ptr, workspace_bytes, warm_start=False,
+                  mat_fluid=0, mat_air=2, stream=0) -> dict:
+        """Flexible CG with preconditioner kind ``precond`` (1 U-Net, 2 V-cycle +
+        U-Net coarse solve, 3 V-cycle + Jacobi coarse sweeps) to a true relative
+        residual ``tol``; falls back to plain PCG on a stall."""
+        st = _lib.FcgStats()
+        _lib.call(
+            "bs_fcg_solve",
+            _ptr("x_ptr", x_ptr), _ptr("div_u_ptr", div_u_ptr), _ptr("mat_ptr", mat_ptr),
+            _usize("nx", nx), _usize("ny", ny), _usize("nz", nz),
+            float(dx), float(dt), float(rho), _i32("mat_fluid", mat_fluid), _i32("mat_air", mat_air),
+            float(tol), _usize("max_iter", max_iter), 1 if warm_start else 0, int(precond),
+            _ptr("unet_params_ptr", unet_params_ptr), _ptr("unet_packed_ptr", unet_packed_ptr),
+            _ptr("workspace_ptr", workspace_ptr), int(workspace_bytes), _ptr("stream", stream),
+            ctypes.byref(st),
+        )
+        return st.as_dict()

@@ -175,6 +175,32 @@ int bs_unet_forward(const float* params, const void* packed, const float* r, con
 int bs_unet_backward(const float* params, const void* packed, const float* r, const float* e,
                      const unsigned char* code, int nx, int ny, int nz, float* grads,
                      double* loss_dev, void* workspace, size_t workspace_bytes, bs_stream_t stream);
+/* Statistics of bs_fcg_solve. */
+typedef struct {
+    int iterations;     /* flexible-CG iterations */
+    int converged;      /* true residual <= tol at exit */
+    int fallback;       /* the flexible CG stalled and plain PCG finished the solve */
+    int pcg_iterations; /* plain PCG iterations after a stall or residual drift */
+    int restarts;
+    int reserved;
+    double relres_recursive;
+    double relres_true; /* ||b - A x||_2 / ||b||_2, FP64 accumulation */
+    double bnorm;
+} bs_fcg_stats;
+
+/* Workspace bytes of bs_fcg_solve for preconditioner kind `precond`. */
+size_t bs_fcg_workspace_bytes(int nx, int ny, int nz, int precond);
+/* Flexible CG (Polak-Ribiere beta) with a learned preconditioner, to a true
+ * relative residual tol: precond 1 = the U-Net on this grid, 2 = a damped-
+ * Jacobi V-cycle on this grid and its 2x coarsening with the U-Net solving the
+ * 4x-coarsened grid (its correction polished by Jacobi sweeps), 3 = the same
+ * V-cycle with Jacobi sweeps on the coarsest grid (no network), 4 = as 2 with
+ * sweeps before the network, which then corrects their residual.  A stall (no 2 % residual drop in 20 iterations) hands
+ * over to plain Jacobi-PCG from the current x. */
+int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx, int ny, int nz, float dx, float dt,
+                 float rho, int mat_fluid, int mat_air, double tol, int max_iter, int warm_start,
+                 int precond, const float* unet_params, const void* unet_packed, void* workspace,
+                 size_t workspace_bytes, bs_stream_t stream, bs_fcg_stats* stats);
 /* Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on FP32 params; gscale
  * multiplies the gradient (1/batch). */
 int bs_unet_adam(float* params, const float* grads, float* m, float* v, int step, float lr,
