@@ -1,14 +1,22 @@
 """Pressure-projection benchmark on one MI355X.
 
-Default run: the 440x440x280 pot (dx = 0.5 mm, ~54.2 M cells, FP32), inputs
-already resident in HBM, and
-  * the Jacobi projection leg: divergence -> 200 masked Jacobi sweeps ->
-    gradient subtraction -> divergence -> on-device max|div|;
-  * roofline of the dominant kernel (the mask-free Jacobi sweep), timed with
-    HIP events on the stream it is launched on;
-  * the CPU baseline: the oracle's OpenMP restatement on a bounded sample.
+Workload (the headline metric): one pressure projection on the 440x440x280
+pot (dx = 0.5 mm, ~54.2 M cells, FP32 fields), RHS the divergence of seeded
+N(0, 0.01) face velocities (seed 11, no plume), solved to a TRUE relative
+residual ||b - A x||_2/||b||_2 <= 1e-6 (FP64 check), inputs resident in HBM:
+  * learned-PCG: flexible CG preconditioned by a damped-Jacobi V-cycle whose
+    4x-coarsened grid is solved by the trained U-Net  -> `value`
+  * plain PCG: Jacobi-preconditioned CG, FP32 vectors / FP64 scalars
+  * (ablation) the same V-cycle with Jacobi sweeps instead of the network
+ms/projection is the whole projection: divergence -> solve -> gradient
+subtraction -> divergence -> on-device max|div|.  Each step restores the
+input velocities (untimed) so every projection solves the same system.
 
-Prints ONE JSON line (see the repository's bench contract).
+Also measured live: the roofline of the mask-free Jacobi sweep and of the
+PCG iteration kernels (HIP events on the launch stream), and the CPU baseline
+(the oracle's PCG on a bounded sample on the host cores).
+
+Prints ONE JSON line on stdout; progress goes to stderr.
 """
 
 from __future__ import annotations
@@ -27,6 +35,11 @@ for _p in (ROOT, os.path.join(ROOT, "boiling-sim_amd")):
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth, GB/s
+METRIC = "ms/projection to rel-res 1e-6 at 440x440x280: learned-PCG vs plain PCG"
+
+# Algorithmic bytes per cell (FP32 fields, one-byte class code where used).
+B_JACOBI = 12.0   # p_in, div, p_out (class carried in p, no mask read)
+B_PCG_ITER = 47.0  # spmv 4+1+4, update 4+4+(4+4)+(4+4)+1, direction 4+1+(4+4)
 
 
 def log(msg: str) -> None:
@@ -36,126 +49,170 @@ def log(msg: str) -> None:
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="prod", choices=["small", "dev", "prod"])
-    ap.add_argument("--sweeps", type=int, default=200)
+    ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all cores in the affinity mask")
+    ap.add_argument("--no-ablation", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=12, help="oracle PCG iterations in the CPU sample")
     return ap.parse_args()
-
-
-def cpu_baseline(cfg, mat, div, n_sweeps_sample: int, threads: int):
-    """Oracle Jacobi sweeps on the host cores: a bounded sample of the same
-    workload, scaled to ms per 200-sweep solve."""
-    import oracle
-
-    oracle.set_threads(threads)
-    p = np.zeros(cfg.shape, np.float32)
-    q = np.zeros(cfg.shape, np.float32)
-    t0 = time.perf_counter()
-    oracle.pressure_solve(p, q, div, mat, cfg.dx, 1e-3, 958.0, n_sweeps_sample)
-    dt = time.perf_counter() - t0
-    return dt / n_sweeps_sample * 1e3
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
-                          "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1", "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
 
     import sim_core
     from boilsim.fixtures import CONFIGS, DT, RHO, pot_mask, seeded_velocities
+    from boilsim.learned import PRECOND_MG_JACOBI, PRECOND_MG_UNET
+    from boilsim.unet import UNet
 
     if sim_core.cuda is None:
-        raise RuntimeError("libboilsim_hip.so not loadable: build first")
+        raise RuntimeError("libboilsim_hip.so not loadable: run __graft_entry__.build() first")
     core = sim_core.cuda.SimCore(0)
     log(core.device_info())
     cfg = CONFIGS[args.config]
     nx, ny, nz = cfg.shape
     n = cfg.n
-    log(f"generating {cfg.shape} pot + seeded velocities")
+    log(f"generating the {nx}x{ny}x{nz} pot and seeded velocities (seed 11, no plume)")
     mat = pot_mask(nx, ny, nz)
-    ux, uy, uz = seeded_velocities(nx, ny, nz, seed=7)
+    ux, uy, uz = seeded_velocities(nx, ny, nz, seed=11, plume=False)
     d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
-    gmat, gux, guy, guz = d(mat), d(ux), d(uy), d(uz)
+    gmat = d(mat)
+    ux0, uy0, uz0 = d(ux), d(uy), d(uz)
+    gux, guy, guz = ux0.clone(), uy0.clone(), uz0.clone()
     p = torch.zeros(cfg.shape, dtype=torch.float32, device="cuda")
-    p_tmp = torch.zeros_like(p)
     div = torch.zeros_like(p)
     bits = torch.zeros(1, dtype=torch.int32, device="cuda")
+    net = UNet.load()
+    ws_pcg = torch.empty(core.pcg_workspace_bytes(nx, ny, nz), dtype=torch.uint8, device="cuda")
+    ws_f = {k: torch.empty(core.fcg_workspace_bytes(nx, ny, nz, k), dtype=torch.uint8, device="cuda")
+            for k in (PRECOND_MG_UNET, PRECOND_MG_JACOBI)}
     torch.cuda.synchronize()
+    ptr = lambda t: t.data_ptr()
 
-    def projection():
-        core.projection_jacobi(p.data_ptr(), p_tmp.data_ptr(), div.data_ptr(), gux.data_ptr(),
-                               guy.data_ptr(), guz.data_ptr(), gmat.data_ptr(), nx, ny, nz,
-                               cfg.dx, DT, RHO, args.sweeps, bits.data_ptr())
+    def projection(solver):
+        core.compute_divergence(ptr(div), ptr(gux), ptr(guy), ptr(guz), ptr(gmat), nx, ny, nz, cfg.dx)
+        if solver == "pcg":
+            st = core.pcg_solve(ptr(p), ptr(div), ptr(gmat), nx, ny, nz, cfg.dx, DT, RHO, args.tol, 20000,
+                                ptr(ws_pcg), ws_pcg.numel())
+        else:
+            k = PRECOND_MG_UNET if solver == "learned" else PRECOND_MG_JACOBI
+            st = core.fcg_solve(ptr(p), ptr(div), ptr(gmat), nx, ny, nz, cfg.dx, DT, RHO, args.tol, 5000, k,
+                                ptr(net.params), ptr(net.packed), ptr(ws_f[k]), ws_f[k].numel())
+        core.subtract_pressure_gradient(ptr(gux), ptr(guy), ptr(guz), ptr(p), ptr(gmat), nx, ny, nz, cfg.dx, DT, RHO)
+        core.compute_divergence(ptr(div), ptr(gux), ptr(guy), ptr(guz), ptr(gmat), nx, ny, nz, cfg.dx)
+        core.max_abs(ptr(div), n, ptr(bits))
+        return st
 
-    for _ in range(args.warmup):
-        projection()
+    def leg(solver):
+        times, stats = [], None
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        for s in range(args.warmup + args.steps):
+            gux.copy_(ux0)
+            guy.copy_(uy0)
+            guz.copy_(uz0)
+            torch.cuda.synchronize()
+            e0.record()
+            stats = projection(solver)
+            e1.record()
+            torch.cuda.synchronize()
+            if s >= args.warmup:
+                times.append(e0.elapsed_time(e1))
+        maxdiv = float(bits.cpu().numpy().view(np.float32)[0])
+        ms = float(np.mean(times))
+        log(f"{solver}: {ms:.2f} ms/projection (min {min(times):.2f}), iters {stats['iterations']}, "
+            f"true relres {stats['relres_true']:.3e}, converged {stats['converged']}, max|div| after {maxdiv:.3e}")
+        return ms, times, stats, maxdiv
+
+    # Initial divergence (for the report).
+    core.compute_divergence(ptr(div), ptr(gux), ptr(guy), ptr(guz), ptr(gmat), nx, ny, nz, cfg.dx)
+    core.max_abs(ptr(div), n, ptr(bits))
+    torch.cuda.synchronize()
+    maxdiv0 = float(bits.cpu().numpy().view(np.float32)[0])
+
+    ms_l, t_l, st_l, md_l = leg("learned")
+    ms_p, t_p, st_p, md_p = leg("pcg")
+    abl = None
+    if not args.no_ablation:
+        ms_a, _, st_a, _ = leg("mg-jacobi")
+        abl = {"ms_per_projection": ms_a, "iterations": st_a["iterations"], "relres_true": st_a["relres_true"]}
+
+    # Roofline of the Jacobi sweep: 200 mask-free sweeps timed with HIP events
+    # on the null stream, the stream the launcher uses.
+    gdiv = div.clone()
+    core.compute_divergence(ptr(gdiv), ptr(ux0), ptr(uy0), ptr(uz0), ptr(gmat), nx, ny, nz, cfg.dx)
+    p_tmp = torch.zeros_like(p)
+    solve = lambda: sim_core._lib.call("bs_jacobi_solve_launch", ptr(p), ptr(p_tmp), ptr(gdiv), ptr(gmat), nx, ny,
+                                       nz, cfg.dx, DT, RHO, 200, 0, 2, 1, 0)
+    solve()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
     e0.record()
-    for _ in range(args.steps):
-        projection()
+    for _ in range(3):
+        solve()
     e1.record()
     torch.cuda.synchronize()
-    wall = (time.perf_counter() - t0) * 1e3 / args.steps
-    ms_proj = e0.elapsed_time(e1) / args.steps
-    maxdiv = float(bits.cpu().numpy().view(np.float32)[0])
-    log(f"jacobi projection: {ms_proj:.3f} ms (host wall {wall:.3f} ms), max|div| {maxdiv:.4e}")
-
-    # Dominant kernel: the mask-free sweep.  Time a 200-sweep solve alone.
-    gdiv = div.clone()
-    reps = 5
-    core_solve = lambda: sim_core._lib.call(
-        "bs_jacobi_solve_launch", p.data_ptr(), p_tmp.data_ptr(), gdiv.data_ptr(), gmat.data_ptr(),
-        nx, ny, nz, cfg.dx, DT, RHO, args.sweeps, 0, 2, 1, 0)
-    core_solve()
-    torch.cuda.synchronize()
-    e0.record()
-    for _ in range(reps):
-        core_solve()
-    e1.record()
-    torch.cuda.synchronize()
-    ms_sweep = e0.elapsed_time(e1) / reps / args.sweeps
-    bytes_sweep = 12.0 * n
-    achieved = bytes_sweep / (ms_sweep * 1e-3) / 1e9
-    log(f"sweep: {ms_sweep * 1e3:.1f} us, {achieved:.0f} GB/s algorithmic (12 B/cell)")
+    us_sweep = e0.elapsed_time(e1) / 3 / 200 * 1e3
+    jac_gbs = B_JACOBI * n / (us_sweep * 1e-6) / 1e9
+    log(f"jacobi sweep: {us_sweep:.1f} us, {jac_gbs:.0f} GB/s algorithmic ({B_JACOBI:.0f} B/cell)")
+    # PCG iteration rate (three kernels), from the plain solve.
+    us_iter = ms_p * 1e3 / max(st_p["iterations"], 1)
+    pcg_gbs = B_PCG_ITER * n / (us_iter * 1e-6) / 1e9
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or len(os.sched_getaffinity(0))
-        threads = min(threads, 16)
-        sample = 10 if cfg.name == "prod" else 50
-        mat_np, div_np = mat, div.cpu().numpy()
-        ms_cpu_sweep = cpu_baseline(cfg, mat_np, div_np, sample, threads)
-        cpu = {"value": ms_cpu_sweep * args.sweeps, "unit": "ms/projection (200 Jacobi sweeps)",
+        import oracle
+
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        oracle.set_threads(threads)
+        div_np = gdiv.cpu().numpy()
+        t0 = time.perf_counter()
+        _, it_c, _ = oracle.pcg_ref(div_np, mat, cfg.dx, DT, RHO, 1e-30, args.cpu_iters)
+        dt_c = time.perf_counter() - t0
+        ms_per_it = dt_c * 1e3 / max(it_c, 1)
+        cpu = {"value": ms_per_it * st_p["iterations"], "unit": "ms/projection (plain PCG solve, same iterations)",
                "cores": threads, "kind": "port",
-               "sample": f"{sample} oracle Jacobi sweeps at {nx}x{ny}x{nz}, scaled to {args.sweeps}"}
-        log(f"cpu baseline: {ms_cpu_sweep:.2f} ms/sweep on {threads} threads")
+               "sample": f"{it_c} iterations of the oracle's FP32/FP64 Jacobi-PCG at {nx}x{ny}x{nz} on {threads} "
+                         f"threads ({dt_c:.1f} s incl. set-up), scaled to the GPU plain-PCG count "
+                         f"{st_p['iterations']}"}
+        log(f"cpu baseline: {ms_per_it:.1f} ms/iteration on {threads} threads")
 
     out = {
-        "metric": "ms/projection to rel-res 1e-6 at 440x440x280: learned-PCG vs plain PCG",
-        "value": ms_proj,
+        "metric": METRIC,
+        "value": ms_l,
         "unit": "ms/projection",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_proj,
+        "ms_per_step": ms_l,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded pot mask and N(0,0.01) face velocities + plume)",
-        "config": {"workload": f"jacobi projection, {args.sweeps} sweeps, {nx}x{ny}x{nz} pot",
-                   "grid": [nx, ny, nz], "dx_m": cfg.dx, "sweeps": args.sweeps},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "jacobi_sweep_kernel<4,encoded,encoded>", "bytes_per_cell": 12},
+        "data": "synthetic: seeded pot mask, N(0,0.01) face velocities (seed 11), trained U-Net weights",
+        "config": {"workload": f"pressure projection, {nx}x{ny}x{nz} pot, true rel-res {args.tol:g}",
+                   "grid": [nx, ny, nz], "dx_m": cfg.dx, "rhs": "divergence of N(0,0.01) faces, seed 11",
+                   "learned_preconditioner": "FCG + Jacobi V-cycle, U-Net (8/16/32) coarse solve"},
+        "plain_pcg": {"ms_per_projection": ms_p, "iterations": st_p["iterations"],
+                      "relres_true": st_p["relres_true"], "max_div_after": md_p},
+        "learned_pcg": {"ms_per_projection": ms_l, "iterations": st_l["iterations"],
+                        "relres_true": st_l["relres_true"], "fallback": st_l["fallback"],
+                        "max_div_after": md_l, "times_ms": t_l},
+        "speedup_vs_plain_pcg": ms_p / ms_l,
+        "ablation_mg_jacobi_coarse": abl,
+        "max_div_before": maxdiv0,
+        "roofline": {"bound": "hbm", "achieved": jac_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": jac_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "jacobi_sweep_kernel<4,1,1> (mask-free sweep)", "bytes_per_cell": B_JACOBI,
+                     "us_per_launch": us_sweep},
+        "roofline_pcg_iteration": {"bound": "hbm", "achieved": pcg_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": pcg_gbs / HBM_PEAK_GBS, "bytes_per_cell": B_PCG_ITER,
+                                   "us_per_iteration": us_iter},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
