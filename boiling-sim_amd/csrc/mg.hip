@@ -247,6 +247,180 @@ __global__ __launch_bounds__(256) void prolong_kernel(MLvl F, MLvl C, const uint
     }
 }
 
+// ---------------------------------------------------------------------------
+// Vectorised kernels for the two fine levels (VEC consecutive k cells per
+// thread, one 256-thread tile per workgroup, XCD-aware order).
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float winv(uint8_t code) {
+    return code >= kCodeFluid0 ? kOmega * inv_diag(code) : 0.0f;
+}
+
+__device__ __forceinline__ bool one_tile(const Tiles& T, int VEC, int& i, int& j, int& k0, int64_t& c0) {
+    const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t chunk = t % T.chunks;
+    i = (int)(t / T.chunks);
+    const uint32_t q = chunk * 256u + threadIdx.x;
+    if (q >= (uint32_t)T.ny * T.nzv.d) return false;
+    j = (int)fdiv(q, T.nzv);
+    k0 = (int)(q - (uint32_t)j * T.nzv.d) * VEC;
+    c0 = ((int64_t)i * T.ny + j) * T.nz + k0;
+    return true;
+}
+
+// Two damped-Jacobi sweeps from z = 0 in one pass: z1 = wD^-1 r is formed on
+// the fly at the cell and its neighbours, z = z1 + wD^-1 (r - A z1).
+template <int VEC>
+__global__ __launch_bounds__(256) void pre2_kernel(Tiles T, const uint8_t* __restrict__ code,
+                                                   const float* __restrict__ r, float* __restrict__ z,
+                                                   const int* __restrict__ done) {
+    if (done && *done) return;
+    int i, j, k0;
+    int64_t c0;
+    if (!one_tile(T, VEC, i, j, k0, c0)) return;
+    Nbhd<VEC> nb;
+    nb.load(T, code, r, i, j, k0, c0);
+    float rc[VEC], out[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) {
+        rc[e] = nb.vc[e];
+        nb.vc[e] *= winv(nb.cc[e]);
+        nb.vl[e] *= winv(nb.cl[e]);
+        nb.vr[e] *= winv(nb.cr[e]);
+        nb.vd[e] *= winv(nb.cd[e]);
+        nb.vu[e] *= winv(nb.cu[e]);
+    }
+    nb.vb *= winv(nb.cb);
+    nb.vf *= winv(nb.cf);
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) {
+        const float az = nb.apply(e);
+        out[e] = nb.cc[e] >= kCodeFluid0 ? nb.vc[e] + winv(nb.cc[e]) * (rc[e] - az) : 0.0f;
+    }
+    stf<VEC>(z, c0, out);
+}
+
+// zo = zi + wD^-1 (r - A zi).
+template <int VEC>
+__global__ __launch_bounds__(256) void smooth_kernel(Tiles T, const uint8_t* __restrict__ code,
+                                                     const float* __restrict__ r, const float* __restrict__ zi,
+                                                     float* __restrict__ zo, const int* __restrict__ done) {
+    if (done && *done) return;
+    int i, j, k0;
+    int64_t c0;
+    if (!one_tile(T, VEC, i, j, k0, c0)) return;
+    Nbhd<VEC> nb;
+    nb.load(T, code, zi, i, j, k0, c0);
+    float rv[VEC], out[VEC];
+    ldf<VEC>(r, c0, rv);
+#pragma unroll
+    for (int e = 0; e < VEC; ++e)
+        out[e] = nb.cc[e] >= kCodeFluid0 ? nb.vc[e] + winv(nb.cc[e]) * (rv[e] - nb.apply(e)) : 0.0f;
+    stf<VEC>(zo, c0, out);
+}
+
+// Last smoothing sweep of the V-cycle fused with the flexible-CG products:
+// zo = zi + wD^-1 (r - A zi) and partials of r.zo and r.z_old (grid-stride).
+template <int VEC>
+__global__ __launch_bounds__(256) void smooth_dots_kernel(Tiles T, const uint8_t* __restrict__ code,
+                                                          const float* __restrict__ r, const float* __restrict__ zi,
+                                                          float* __restrict__ zo, const float* __restrict__ z_old,
+                                                          double* __restrict__ part, const int* __restrict__ done) {
+    if (done && *done) return;
+    uint32_t t0, t1;
+    tile_span(T.ntiles, t0, t1);
+    double a = 0.0, b = 0.0;
+    const int64_t plane = (int64_t)T.ny * T.nz;
+    for (uint32_t t = t0; t < t1; ++t) {
+        int i, j, k0;
+        if (!tile_cell<VEC>(T, t, i, j, k0)) continue;
+        const int64_t c0 = (int64_t)i * plane + (int64_t)j * T.nz + k0;
+        Nbhd<VEC> nb;
+        nb.load(T, code, zi, i, j, k0, c0);
+        float rv[VEC], out[VEC], zo_[VEC];
+        ldf<VEC>(r, c0, rv);
+        if (z_old) ldf<VEC>(z_old, c0, zo_);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            out[e] = nb.cc[e] >= kCodeFluid0 ? nb.vc[e] + winv(nb.cc[e]) * (rv[e] - nb.apply(e)) : 0.0f;
+            a += (double)rv[e] * (double)out[e];
+            if (z_old) b += (double)rv[e] * (double)zo_[e];
+        }
+        stf<VEC>(zo, c0, out);
+    }
+    double v[2] = {a, b};
+    write_partials<2>(v, part);
+}
+
+// rc[I,J,K] = 0.5 * sum over the 8 children of (r - A z), coarse-centred:
+// a thread owns one coarse cell and reads its children as 2-cell k pairs
+// (fine nz even).
+__global__ __launch_bounds__(256) void restrict2_kernel(Tiles F, Tiles C, const uint8_t* __restrict__ fcode,
+                                                        const float* __restrict__ r, const float* __restrict__ z,
+                                                        float* __restrict__ rc, const int* __restrict__ done) {
+    if (done && *done) return;
+    int I, J, K;
+    int64_t cc;
+    if (!one_tile(C, 1, I, J, K, cc)) return;
+    float s = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int i = 2 * I + a;
+        if (i >= F.nx) continue;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int j = 2 * J + b;
+            if (j >= F.ny) continue;
+            const int k0 = 2 * K;
+            const int64_t f0 = ((int64_t)i * F.ny + j) * F.nz + k0;
+            Nbhd<2> nb;
+            nb.load(F, fcode, z, i, j, k0, f0);
+            float rv[2];
+            ldf<2>(r, f0, rv);
+#pragma unroll
+            for (int e = 0; e < 2; ++e)
+                if (nb.cc[e] >= kCodeFluid0) s += rv[e] - nb.apply(e);
+        }
+    }
+    rc[cc] = 0.5f * s;
+}
+
+// z += trilinear(ec) on fluid cells, VEC fine cells per thread.
+template <int VEC>
+__global__ __launch_bounds__(256) void prolong_vec_kernel(Tiles F, Tiles C, const uint8_t* __restrict__ fcode,
+                                                          const float* __restrict__ ec, float* __restrict__ z,
+                                                          const int* __restrict__ done) {
+    if (done && *done) return;
+    int i, j, k0;
+    int64_t c0;
+    if (!one_tile(F, VEC, i, j, k0, c0)) return;
+    uint8_t cc[VEC];
+    float zv[VEC];
+    ldc<VEC>(fcode, c0, cc);
+    ldf<VEC>(z, c0, zv);
+    const int I = i >> 1, J = j >> 1;
+    int I2 = I + ((i & 1) ? 1 : -1), J2 = J + ((j & 1) ? 1 : -1);
+    I2 = I2 < 0 ? 0 : (I2 >= C.nx ? C.nx - 1 : I2);
+    J2 = J2 < 0 ? 0 : (J2 >= C.ny ? C.ny - 1 : J2);
+    const int64_t cp = (int64_t)C.ny * C.nz;
+    const int64_t rows[4] = {I * cp + (int64_t)J * C.nz, I * cp + (int64_t)J2 * C.nz, I2 * cp + (int64_t)J * C.nz,
+                             I2 * cp + (int64_t)J2 * C.nz};
+    const float wr[4] = {0.5625f, 0.1875f, 0.1875f, 0.0625f};
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) {
+        if (cc[e] < kCodeFluid0) continue;
+        const int k = k0 + e;
+        const int K = k >> 1;
+        int K2 = K + ((k & 1) ? 1 : -1);
+        K2 = K2 < 0 ? 0 : (K2 >= C.nz ? C.nz - 1 : K2);
+        float v = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v += wr[q] * (0.75f * ec[rows[q] + K] + 0.25f * ec[rows[q] + K2]);
+        zv[e] += v;
+    }
+    stf<VEC>(z, c0, zv);
+}
+
 static inline size_t al(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct MgWs {
@@ -336,23 +510,46 @@ int mg_setup(hipStream_t st, void* ws, int nx, int ny, int nz, const int* mat, i
     return hipSuccess;
 }
 
+unsigned mg_dots_groups(int nx, int ny, int nz) {
+    const Tiles T = make_tiles(nx, ny, nz, vec_for(nz, {}));
+    return T.ntiles < 1024 ? T.ntiles : 1024;
+}
+
+#define MG_VEC(KERNEL, VEC, GRID, ST, ...)                                          \
+    do {                                                                           \
+        if ((VEC) == 4) KERNEL<4><<<(GRID), 256, 0, (ST)>>>(__VA_ARGS__);          \
+        else if ((VEC) == 2) KERNEL<2><<<(GRID), 256, 0, (ST)>>>(__VA_ARGS__);     \
+        else KERNEL<1><<<(GRID), 256, 0, (ST)>>>(__VA_ARGS__);                     \
+    } while (0)
+
 int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* code0, const float* r, float* z,
-             int coarse_kind, const float* params, const void* packed, double n_fluid_coarse, const int* done) {
+             const float* z_old, double* dots_part, int coarse_kind, const float* params, const void* packed,
+             double n_fluid_coarse, const int* done) {
     MgWs W = carve_mg(ws, nx, ny, nz);
-    const unsigned g0 = grid_for(W.L0.n), g1 = grid_for(W.L1.n), g2 = grid_for(W.L2.n);
-    // fine pre-smoothing: z = wD^-1 r, then one sweep into zt0 -> z
-    jacobi0_kernel<<<g0, 256, 0, st>>>(W.L0, code0, r, W.zt0, done);
-    jacobi_kernel<<<g0, 256, 0, st>>>(W.L0, code0, r, W.zt0, z, done);
-    restrict_kernel<<<g1, 256, 0, st>>>(W.L0, W.L1, code0, r, z, W.r1, done);
+    const int v0 = vec_for(W.L0.nz, {r, z, z_old, code0, W.zt0});
+    const int v1 = vec_for(W.L1.nz, {W.r1, W.z1, W.zt1, W.code1});
+    const Tiles T0 = make_tiles(W.L0.nx, W.L0.ny, W.L0.nz, v0);
+    const Tiles T1 = make_tiles(W.L1.nx, W.L1.ny, W.L1.nz, v1);
+    const Tiles C1 = make_tiles(W.L1.nx, W.L1.ny, W.L1.nz, 1);
+    const Tiles C2 = make_tiles(W.L2.nx, W.L2.ny, W.L2.nz, 1);
+    const unsigned g1 = grid_for(W.L1.n), g2 = grid_for(W.L2.n);
+    // fine: two pre-smoothing sweeps from zero, restriction of the residual
+    MG_VEC(pre2_kernel, v0, T0.ntiles, st, T0, code0, r, z, done);
+    if ((W.L0.nz & 1) == 0)
+        restrict2_kernel<<<C1.ntiles, 256, 0, st>>>(T0, C1, code0, r, z, W.r1, done);
+    else
+        restrict_kernel<<<g1, 256, 0, st>>>(W.L0, W.L1, code0, r, z, W.r1, done);
     // level 1
-    jacobi0_kernel<<<g1, 256, 0, st>>>(W.L1, W.code1, W.r1, W.zt1, done);
-    jacobi_kernel<<<g1, 256, 0, st>>>(W.L1, W.code1, W.r1, W.zt1, W.z1, done);
-    restrict_kernel<<<g2, 256, 0, st>>>(W.L1, W.L2, W.code1, W.r1, W.z1, W.r2, done);
+    MG_VEC(pre2_kernel, v1, T1.ntiles, st, T1, W.code1, W.r1, W.z1, done);
+    if ((W.L1.nz & 1) == 0)
+        restrict2_kernel<<<C2.ntiles, 256, 0, st>>>(T1, C2, W.code1, W.r1, W.z1, W.r2, done);
+    else
+        restrict_kernel<<<g2, 256, 0, st>>>(W.L1, W.L2, W.code1, W.r1, W.z1, W.r2, done);
     // level 2: coarse solve.  coarse_kind 0: 11 damped-Jacobi sweeps;
-    // 1: the network, then 10 sweeps polishing its correction; 2: 5 sweeps,
-    // the network on their residual, then 5 sweeps.
+    // 1: the network (A-norm-optimal step), then 10 sweeps polishing its
+    // correction; 2: 5 sweeps, the network on their residual, then 4 sweeps.
     auto sweeps = [&](int pairs) {
-        for (int s = 0; s < pairs; ++s) {
+        for (int q = 0; q < pairs; ++q) {
             jacobi_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, W.zt2, done);
             jacobi_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.zt2, W.e2, done);
         }
@@ -379,13 +576,14 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
         sweeps(2);
     }
     // back up: level 1
-    prolong_kernel<<<g1, 256, 0, st>>>(W.L1, W.L2, W.code1, W.e2, W.z1, done);
-    jacobi_kernel<<<g1, 256, 0, st>>>(W.L1, W.code1, W.r1, W.z1, W.zt1, done);
-    jacobi_kernel<<<g1, 256, 0, st>>>(W.L1, W.code1, W.r1, W.zt1, W.z1, done);
-    // fine
-    prolong_kernel<<<g0, 256, 0, st>>>(W.L0, W.L1, code0, W.z1, z, done);
-    jacobi_kernel<<<g0, 256, 0, st>>>(W.L0, code0, r, z, W.zt0, done);
-    jacobi_kernel<<<g0, 256, 0, st>>>(W.L0, code0, r, W.zt0, z, done);
+    MG_VEC(prolong_vec_kernel, v1, T1.ntiles, st, T1, C2, W.code1, W.e2, W.z1, done);
+    MG_VEC(smooth_kernel, v1, T1.ntiles, st, T1, W.code1, W.r1, W.z1, W.zt1, done);
+    MG_VEC(smooth_kernel, v1, T1.ntiles, st, T1, W.code1, W.r1, W.zt1, W.z1, done);
+    // fine: prolongation, one sweep, and the last sweep fused with r.z, r.z_old
+    MG_VEC(prolong_vec_kernel, v0, T0.ntiles, st, T0, C1, code0, W.z1, z, done);
+    MG_VEC(smooth_kernel, v0, T0.ntiles, st, T0, code0, r, z, W.zt0, done);
+    const unsigned gd = mg_dots_groups(nx, ny, nz);  // the partial count the caller sums
+    MG_VEC(smooth_dots_kernel, v0, gd, st, T0, code0, r, W.zt0, z, z_old, dots_part, done);
     return (int)hipGetLastError();
 }
 

@@ -32,6 +32,7 @@
 
 #include <math.h>
 #include <stddef.h>
+#include <stdlib.h>
 #include <string.h>
 
 namespace bs {
@@ -610,8 +611,9 @@ template <int VEC>
 __global__ __launch_bounds__(256) void fcg_update_kernel(Tiles T, const float* __restrict__ p,
                                                          const float* __restrict__ ap, float* __restrict__ x,
                                                          float* __restrict__ r, const double* __restrict__ partA,
-                                                         double* __restrict__ partB, PcgCtrl* ctrl, int it) {
+                                                         double* __restrict__ partB, PcgCtrl* ctrl) {
     if (ctrl->done) return;
+    const int it = ctrl->iter;  // advanced only by fcg_check_kernel
     double pap[1];
     sum_partials<1>(partA, gridDim.x, pap);
     const double rz = ctrl->rz[it & 1];
@@ -647,17 +649,19 @@ __global__ __launch_bounds__(256) void fcg_update_kernel(Tiles T, const float* _
     write_partials<1>(v, partB);
 }
 
-// One workgroup: convergence and stall test after the update of iteration it.
-__global__ __launch_bounds__(256) void fcg_check_kernel(const double* __restrict__ partB, unsigned G, PcgCtrl* ctrl,
-                                                        int it) {
+// One workgroup: convergence and stall test after the update of iteration
+// it = ctrl->iter, which it advances to it + 1 (the only writer of the
+// counter inside an iteration, so the other kernels can read it freely).
+__global__ __launch_bounds__(256) void fcg_check_kernel(const double* __restrict__ partB, unsigned G, PcgCtrl* ctrl) {
     if (ctrl->done) return;
     double v[1];
     sum_partials<1>(partB, G, v);
     if (threadIdx.x != 0) return;
+    const int it = ctrl->iter;
     const double rr = v[0];
     ctrl->rr = rr;
+    ctrl->iter = it + 1;
     if (rr < ctrl->stop) {
-        ctrl->iter = it + 1;
         ctrl->converged = 1;
         ctrl->done = 1;
         return;
@@ -666,7 +670,6 @@ __global__ __launch_bounds__(256) void fcg_check_kernel(const double* __restrict
         ctrl->best_rr = rr;
         ctrl->best_it = it;
     } else if (it - ctrl->best_it >= 20) {
-        ctrl->iter = it + 1;
         ctrl->fallback = 1;
         ctrl->done = 1;
     }
@@ -687,18 +690,19 @@ __global__ __launch_bounds__(256) void fcg_dots_kernel(int64_t n, const float* _
     write_partials<2>(v, partZ);
 }
 
-// beta = (r.z_new - r.z_old) / r.z; p = z_new + beta p.  init: p = z, r.z only.
+// beta = (r.z_new - r.z_old) / r.z; p = z_new + beta p, after the check of
+// iteration it advanced ctrl->iter to it + 1.  init: p = z and r.z only.
 template <int VEC>
 __global__ __launch_bounds__(256) void fcg_direction_kernel(Tiles T, const float* __restrict__ z,
                                                             float* __restrict__ p, const double* __restrict__ partZ,
-                                                            unsigned G, PcgCtrl* ctrl, int it, int init) {
+                                                            unsigned G, PcgCtrl* ctrl, int init) {
     if (ctrl->done) return;
+    const int it1 = ctrl->iter;  // iterations completed
     double v[2];
     sum_partials<2>(partZ, G, v);
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (!(v[0] > 0.0)) {  // preconditioner not positive on this residual
         if (lead) {
-            ctrl->iter = it + (init ? 0 : 1);
             ctrl->fallback = 1;
             ctrl->done = 1;
         }
@@ -706,18 +710,14 @@ __global__ __launch_bounds__(256) void fcg_direction_kernel(Tiles T, const float
     }
     double beta = 0.0;
     if (!init) {
-        const double rz_old = ctrl->rz[it & 1];
+        const double rz_old = ctrl->rz[(it1 - 1) & 1];
         beta = rz_old != 0.0 ? (v[0] - v[1]) / rz_old : 0.0;
     }
-    const int slot = init ? (it & 1) : ((it + 1) & 1);
     if (lead) {
-        ctrl->rz[slot] = v[0];
+        ctrl->rz[it1 & 1] = v[0];
         ctrl->rz_new = v[0];
         ctrl->beta = beta;
-        if (!init) {
-            ctrl->iter = it + 1;
-            if (it + 1 >= ctrl->max_iter) ctrl->done = 1;
-        }
+        if (!init && it1 >= ctrl->max_iter) ctrl->done = 1;
     }
     const float bf = (float)beta;
     uint32_t t0, t1;
@@ -764,6 +764,7 @@ struct FcgBuffers {
     const float* params;
     const void* packed;
     double n_fluid;  // of the level the network runs on
+    unsigned partZ_G;  // workgroups that wrote partZ
     int nx, ny, nz;
 };
 
@@ -782,34 +783,107 @@ static int fcg_precond(hipStream_t st, const FcgBuffers& F, const float* r, floa
         return (int)hipGetLastError();
     }
     const int coarse = F.kind == 2 ? 1 : (F.kind == 4 ? 2 : 0);
-    const int rc = mg_apply(st, F.pws, F.nx, F.ny, F.nz, B.code, r, z, coarse, F.params, F.packed, F.n_fluid, done);
+    (void)n;
+    // The V-cycle's last sweep also writes the r.z and r.z_old partials.
+    return mg_apply(st, F.pws, F.nx, F.ny, F.nz, B.code, r, z, z_old, F.partZ, coarse, F.params, F.packed,
+                    F.n_fluid, done);
+}
+
+// One flexible-CG iteration of parity `par` (z_new = z[1-par], z_old = z[par]).
+static int fcg_enqueue_iteration(const Tiles& T, int vec, const FcgBuffers& F, int par, hipStream_t st) {
+    const PcgBuffers& B = F.B;
+    BS_VEC_LAUNCH(pcg_spmv_kernel, vec, B.nwg, st, T, B.code, B.p, B.ap, B.partA, B.ctrl);
+    BS_VEC_LAUNCH(fcg_update_kernel, vec, B.nwg, st, T, B.p, B.ap, B.x, B.r, B.partA, B.partB, B.ctrl);
+    fcg_check_kernel<<<1, 256, 0, st>>>(B.partB, B.nwg, B.ctrl);
+    const int rc = fcg_precond(st, F, B.r, F.z[par ^ 1], F.z[par]);
     if (rc != hipSuccess) return rc;
-    fcg_dots_kernel<<<kUNetPartWg, 256, 0, st>>>(n, r, z, z_old, F.partZ, B.ctrl);
+    BS_VEC_LAUNCH(fcg_direction_kernel, vec, B.nwg, st, T, F.z[par ^ 1], B.p, F.partZ, F.partZ_G, B.ctrl, 0);
     return (int)hipGetLastError();
 }
 
-static int fcg_iterate(const Tiles& T, int vec, const FcgBuffers& F, int it0, int max_iter, hipStream_t st) {
+// Instantiated graphs of an (even, odd) iteration pair, keyed by everything
+// the captured launches depend on.  The ~60 launches of one iteration (most
+// of them small coarse-level kernels) become one graph launch.
+struct PairGraphKey {
+    const void* ws;
+    const float* x;
+    const float* params;
+    const void* packed;
+    double n_fluid;
+    int nx, ny, nz, kind, vec;
+    unsigned nwg;
+    bool operator==(const PairGraphKey& o) const { return memcmp(this, &o, sizeof(*this)) == 0; }
+};
+struct PairGraphEntry {
+    PairGraphKey key;
+    hipGraphExec_t exec;
+    bool used;
+};
+static PairGraphEntry g_graphs[8];
+static int g_graph_next = 0;
+static hipStream_t g_capture_stream = nullptr;
+
+static int pair_graph(const Tiles& T, int vec, const FcgBuffers& F, hipGraphExec_t* out) {
+    PairGraphKey key;
+    memset(&key, 0, sizeof(key));
+    key.ws = F.B.code;
+    key.x = F.B.x;
+    key.params = F.params;
+    key.packed = F.packed;
+    key.n_fluid = F.n_fluid;
+    key.nx = F.nx; key.ny = F.ny; key.nz = F.nz; key.kind = F.kind; key.vec = vec; key.nwg = F.B.nwg;
+    for (auto& e : g_graphs)
+        if (e.used && e.key == key) { *out = e.exec; return hipSuccess; }
+    if (!g_capture_stream) BS_RETURN_IF_ERR(hipStreamCreateWithFlags(&g_capture_stream, hipStreamNonBlocking));
+    BS_RETURN_IF_ERR(hipStreamBeginCapture(g_capture_stream, hipStreamCaptureModeRelaxed));
+    int rc = fcg_enqueue_iteration(T, vec, F, 0, g_capture_stream);
+    if (rc == hipSuccess) rc = fcg_enqueue_iteration(T, vec, F, 1, g_capture_stream);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(g_capture_stream, &graph);
+    if (rc != hipSuccess) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+    BS_RETURN_IF_ERR(ec);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    BS_RETURN_IF_ERR(ei);
+    PairGraphEntry& slot = g_graphs[g_graph_next];
+    g_graph_next = (g_graph_next + 1) % 8;
+    if (slot.used) (void)hipGraphExecDestroy(slot.exec);
+    slot.key = key;
+    slot.exec = exec;
+    slot.used = true;
+    *out = exec;
+    return hipSuccess;
+}
+
+// Runs iterations from the device counter (even on entry) in graph-launched
+// pairs until done, polling the flag one batch behind.
+static int fcg_iterate(const Tiles& T, int vec, const FcgBuffers& F, int max_iter, hipStream_t st) {
     const PcgBuffers& B = F.B;
     BS_RETURN_IF_ERR((hipError_t)ensure_poll());
+    // BOILSIM_FCG_EAGER=1 launches the kernels directly instead of replaying
+    // the captured graph (same results; used to check the graph path).
+    const char* eager_env = getenv("BOILSIM_FCG_EAGER");
+    const bool eager = eager_env && eager_env[0] == '1';
+    hipGraphExec_t exec = nullptr;
+    int rc = eager ? hipSuccess : pair_graph(T, vec, F, &exec);
+    if (rc != hipSuccess) return rc;
     hipEvent_t ev[2];
     BS_RETURN_IF_ERR(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     BS_RETURN_IF_ERR(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     const size_t done_off = offsetof(PcgCtrl, done);
-    int it = it0, batch = 2, slot = 0;
+    int it = 0, batch = 1, slot = 0;
     bool pending[2] = {false, false};
-    int rc = hipSuccess;
     g_poll.slots[0] = g_poll.slots[2] = 0;
-    while (it < max_iter) {
-        const int nb = (max_iter - it) < batch ? (max_iter - it) : batch;
-        for (int q = 0; q < nb && rc == hipSuccess; ++q, ++it) {
-            BS_VEC_LAUNCH(pcg_spmv_kernel, vec, B.nwg, st, T, B.code, B.p, B.ap, B.partA, B.ctrl);
-            BS_VEC_LAUNCH(fcg_update_kernel, vec, B.nwg, st, T, B.p, B.ap, B.x, B.r, B.partA, B.partB, B.ctrl, it);
-            fcg_check_kernel<<<1, 256, 0, st>>>(B.partB, B.nwg, B.ctrl, it);
-            rc = fcg_precond(st, F, B.r, F.z[(it + 1) & 1], F.z[it & 1]);
-            BS_VEC_LAUNCH(fcg_direction_kernel, vec, B.nwg, st, T, F.z[(it + 1) & 1], B.p, F.partZ, kUNetPartWg,
-                          B.ctrl, it, 0);
+    while (it < max_iter && rc == hipSuccess) {
+        for (int q = 0; q < batch && it < max_iter && rc == hipSuccess; ++q, it += 2) {
+            if (eager) {
+                rc = fcg_enqueue_iteration(T, vec, F, 0, st);
+                if (rc == hipSuccess) rc = fcg_enqueue_iteration(T, vec, F, 1, st);
+            } else {
+                rc = hipGraphLaunch(exec, st);
+            }
         }
-        if (rc == hipSuccess) rc = hipGetLastError();
         if (rc != hipSuccess) break;
         rc = hipMemcpyAsync(g_poll.slots + 2 * slot, (const char*)B.ctrl + done_off, 2 * sizeof(int),
                             hipMemcpyDeviceToHost, st);
@@ -825,7 +899,7 @@ static int fcg_iterate(const Tiles& T, int vec, const FcgBuffers& F, int it0, in
             if (g_poll.slots[2 * prev] != 0) break;
         }
         slot ^= 1;
-        if (batch < 16) batch *= 2;
+        if (batch < 8) batch *= 2;
     }
     if (rc == hipSuccess) {
         rc = hipMemcpyAsync(g_poll.final, B.ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, st);
@@ -879,6 +953,7 @@ extern "C" int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx
     F.params = unet_params;
     F.packed = unet_packed;
     F.nx = nx; F.ny = ny; F.nz = nz;
+    F.partZ_G = precond == 1 ? kUNetPartWg : mg_dots_groups(nx, ny, nz);
     B.x = x;
     const int vec = vec_for(nz, {x, div_u, mat, B.b, B.r, B.p, B.ap, B.code, F.z[0], F.z[1]});
     const Tiles T = make_tiles(nx, ny, nz, vec);
@@ -909,9 +984,9 @@ extern "C" int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx
     fcg_prepare_kernel<<<1, 64, 0, st>>>(B.ctrl, 0);
     int rc = fcg_precond(st, F, B.r, F.z[0], nullptr);
     if (rc != hipSuccess) return rc;
-    BS_VEC_LAUNCH(fcg_direction_kernel, vec, B.nwg, st, T, F.z[0], B.p, F.partZ, kUNetPartWg, B.ctrl, 0, 1);
+    BS_VEC_LAUNCH(fcg_direction_kernel, vec, B.nwg, st, T, F.z[0], B.p, F.partZ, F.partZ_G, B.ctrl, 1);
     BS_RETURN_IF_ERR(hipGetLastError());
-    rc = fcg_iterate(T, vec, F, 0, max_iter, st);
+    rc = fcg_iterate(T, vec, F, max_iter, st);
     if (rc != hipSuccess) return rc;
     PcgCtrl h = *g_poll.final;
     const int fcg_iters = h.iter;

@@ -28,8 +28,12 @@ size_t mg_ws_bytes(int nx, int ny, int nz);
 // count of the coarsest level through *n_fluid_coarse (synchronises once).
 int mg_setup(hipStream_t st, void* ws, int nx, int ny, int nz, const int* mat, int mf, int ma,
              double* n_fluid_coarse);
+// z = V-cycle(r); also writes partials of r.z and r.z_old (z_old may be null)
+// into dots_part, mg_dots_groups(nx,ny,nz) workgroups of 2 values.
 int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* code0, const float* r, float* z,
-             int coarse_kind, const float* params, const void* packed, double n_fluid_coarse, const int* done);
+             const float* z_old, double* dots_part, int coarse_kind, const float* params, const void* packed,
+             double n_fluid_coarse, const int* done);
+unsigned mg_dots_groups(int nx, int ny, int nz);
 
 // e *= (r.e)/(e.A e) (0 when not positive): the A-norm-optimal multiple of a
 // correction e for A x = r.  part: step_ws_bytes() of device scratch.

@@ -150,3 +150,29 @@ def test_projection_driver(simcore, torch_gpu, monkeypatch, solver):
         b = oracle.rhs_f64(oracle.divergence(ux, uy, uz, mat, cfg.dx), mat, cfg.dx, DT, RHO)
         rel2, _ = oracle.residual_f64(host(grid.p), b, mat)
         assert rel2 < (1.05e-6 if solver != "cg-rust" else 1e-5)
+
+
+def test_graph_replay_matches_eager(simcore, torch_gpu, monkeypatch):
+    """The flexible CG replays a captured HIP graph of two iterations; running
+    the same solve with eager launches, and the graph path again after eager
+    work and synchronisations in between, gives bit-identical solutions and
+    statistics (the reductions are deterministic)."""
+    torch = torch_gpu
+    from boilsim.unet import UNet
+
+    net = UNet.load()
+    cfg, mat, div, dt, rho = _pot("small")
+    mat_d, div_d = dev(torch, mat), dev(torch, div)
+    results = []
+    for mode in ("graph", "eager", "graph", "graph"):
+        if mode == "eager":
+            monkeypatch.setenv("BOILSIM_FCG_EAGER", "1")
+        else:
+            monkeypatch.delenv("BOILSIM_FCG_EAGER", raising=False)
+        x, st = _fcg(simcore, torch, net, cfg, mat_d, div_d, 2)
+        junk = torch.randn(1 << 20, device="cuda").sum().item()  # eager work between replays
+        results.append((x, st, junk))
+    x0, st0, _ = results[0]
+    for x, st, _ in results[1:]:
+        assert np.array_equal(x.view(np.uint32), x0.view(np.uint32))
+        assert st == st0
