@@ -252,8 +252,12 @@ __global__ __launch_bounds__(256) void prolong_kernel(MLvl F, MLvl C, const uint
 // thread, one 256-thread tile per workgroup, XCD-aware order).
 // ---------------------------------------------------------------------------
 
+// w / diag for the smoother; v_rcp_f32 (1 ulp) instead of an IEEE divide —
+// the smoother needs no exact rounding and this runs ~26 times per thread.
 __device__ __forceinline__ float winv(uint8_t code) {
-    return code >= kCodeFluid0 ? kOmega * inv_diag(code) : 0.0f;
+    const int dg = 8 - (int)code;
+    const float inv = dg > 0 ? __builtin_amdgcn_rcpf((float)dg) : 1.0f;
+    return code >= kCodeFluid0 ? kOmega * inv : 0.0f;
 }
 
 __device__ __forceinline__ bool one_tile(const Tiles& T, int VEC, int& i, int& j, int& k0, int64_t& c0) {
@@ -320,21 +324,17 @@ __global__ __launch_bounds__(256) void smooth_kernel(Tiles T, const uint8_t* __r
 }
 
 // Last smoothing sweep of the V-cycle fused with the flexible-CG products:
-// zo = zi + wD^-1 (r - A zi) and partials of r.zo and r.z_old (grid-stride).
+// zo = zi + wD^-1 (r - A zi) and one partial of r.zo and of r.z_old per tile.
 template <int VEC>
 __global__ __launch_bounds__(256) void smooth_dots_kernel(Tiles T, const uint8_t* __restrict__ code,
                                                           const float* __restrict__ r, const float* __restrict__ zi,
                                                           float* __restrict__ zo, const float* __restrict__ z_old,
                                                           double* __restrict__ part, const int* __restrict__ done) {
     if (done && *done) return;
-    uint32_t t0, t1;
-    tile_span(T.ntiles, t0, t1);
     double a = 0.0, b = 0.0;
-    const int64_t plane = (int64_t)T.ny * T.nz;
-    for (uint32_t t = t0; t < t1; ++t) {
-        int i, j, k0;
-        if (!tile_cell<VEC>(T, t, i, j, k0)) continue;
-        const int64_t c0 = (int64_t)i * plane + (int64_t)j * T.nz + k0;
+    int i, j, k0;
+    int64_t c0;
+    if (one_tile(T, VEC, i, j, k0, c0)) {
         Nbhd<VEC> nb;
         nb.load(T, code, zi, i, j, k0, c0);
         float rv[VEC], out[VEC], zo_[VEC];
@@ -406,6 +406,21 @@ __global__ __launch_bounds__(256) void prolong_vec_kernel(Tiles F, Tiles C, cons
     const int64_t rows[4] = {I * cp + (int64_t)J * C.nz, I * cp + (int64_t)J2 * C.nz, I2 * cp + (int64_t)J * C.nz,
                              I2 * cp + (int64_t)J2 * C.nz};
     const float wr[4] = {0.5625f, 0.1875f, 0.1875f, 0.0625f};
+    // The VEC fine cells k0..k0+VEC-1 use coarse k in [k0/2 - 1, k0/2 + W - 2]
+    // (k0 is even for VEC >= 2; a single cell needs 3 slots for odd k):
+    // gather that window once per row (clamped).
+    constexpr int W = VEC == 1 ? 3 : VEC / 2 + 2;
+    const int kb = (k0 >> 1) - 1;
+    float cw[W];  // y-x weighted coarse values over the k window
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        int kk = kb + w;
+        kk = kk < 0 ? 0 : (kk >= C.nz ? C.nz - 1 : kk);
+        float acc = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc += wr[q] * ec[rows[q] + kk];
+        cw[w] = acc;
+    }
 #pragma unroll
     for (int e = 0; e < VEC; ++e) {
         if (cc[e] < kCodeFluid0) continue;
@@ -413,10 +428,10 @@ __global__ __launch_bounds__(256) void prolong_vec_kernel(Tiles F, Tiles C, cons
         const int K = k >> 1;
         int K2 = K + ((k & 1) ? 1 : -1);
         K2 = K2 < 0 ? 0 : (K2 >= C.nz ? C.nz - 1 : K2);
-        float v = 0.0f;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) v += wr[q] * (0.75f * ec[rows[q] + K] + 0.25f * ec[rows[q] + K2]);
-        zv[e] += v;
+        // window slots of K and K2 (clamping inside the window matches the
+        // clamped index because both clamp to the same boundary value)
+        const int sK = K - kb, sK2 = (K2 - kb) < 0 ? 0 : ((K2 - kb) >= W ? W - 1 : (K2 - kb));
+        zv[e] += 0.75f * cw[sK] + 0.25f * cw[sK2];
     }
     stf<VEC>(z, c0, zv);
 }
@@ -510,10 +525,10 @@ int mg_setup(hipStream_t st, void* ws, int nx, int ny, int nz, const int* mat, i
     return hipSuccess;
 }
 
-unsigned mg_dots_groups(int nx, int ny, int nz) {
-    const Tiles T = make_tiles(nx, ny, nz, vec_for(nz, {}));
-    return T.ntiles < 1024 ? T.ntiles : 1024;
-}
+// Partial count of the last sweep: one per fine tile (the V-cycle's fine
+// buffers all come from 256-byte aligned workspace, so the vector width
+// depends on nz only).
+unsigned mg_dots_groups(int nx, int ny, int nz) { return make_tiles(nx, ny, nz, vec_for(nz, {})).ntiles; }
 
 #define MG_VEC(KERNEL, VEC, GRID, ST, ...)                                          \
     do {                                                                           \
@@ -582,8 +597,8 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
     // fine: prolongation, one sweep, and the last sweep fused with r.z, r.z_old
     MG_VEC(prolong_vec_kernel, v0, T0.ntiles, st, T0, C1, code0, W.z1, z, done);
     MG_VEC(smooth_kernel, v0, T0.ntiles, st, T0, code0, r, z, W.zt0, done);
-    const unsigned gd = mg_dots_groups(nx, ny, nz);  // the partial count the caller sums
-    MG_VEC(smooth_dots_kernel, v0, gd, st, T0, code0, r, W.zt0, z, z_old, dots_part, done);
+    if (T0.ntiles != mg_dots_groups(nx, ny, nz)) return (int)hipErrorInvalidValue;  // misaligned caller buffers
+    MG_VEC(smooth_dots_kernel, v0, T0.ntiles, st, T0, code0, r, W.zt0, z, z_old, dots_part, done);
     return (int)hipGetLastError();
 }
 

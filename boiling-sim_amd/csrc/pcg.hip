@@ -57,6 +57,7 @@ struct PcgCtrl {
     int fallback;    // flexible CG: stalled or lost positivity
     int pad;
     double best_rr;
+    double dots[2];  // flexible CG: r.z_new, r.z_old (reduced)
 };
 static_assert(sizeof(PcgCtrl) <= 256, "control block must fit 256 bytes");
 
@@ -66,6 +67,7 @@ struct PcgBuffers {
     double* partA;  // [G]     p.Ap
     double* partB;  // [2G]    r.r, r.z after the update
     double* partS;  // [2G]    set-up / residual pass
+    double* partT;  // [ntiles] p.Ap per tile (one-tile SpMV), or null
     PcgCtrl* ctrl;
     unsigned nwg;
 };
@@ -225,6 +227,78 @@ __global__ __launch_bounds__(256) void pcg_spmv_kernel(Tiles T, const uint8_t* _
     write_partials<1>(v, partA);
 }
 
+// Ap = A p with one 256-thread tile per workgroup (better latency hiding for
+// a stencil than a grid-stride loop); one p.Ap partial per tile.
+template <int VEC>
+__global__ __launch_bounds__(256) void pcg_spmv1_kernel(Tiles T, const uint8_t* __restrict__ code,
+                                                        const float* __restrict__ p, float* __restrict__ ap,
+                                                        double* __restrict__ partA, const PcgCtrl* ctrl) {
+    if (ctrl->done) return;
+    const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
+    double pap = 0.0;
+    int i, j, k0;
+    if (tile_cell<VEC>(T, t, i, j, k0)) {
+        const int64_t c0 = ((int64_t)i * T.ny + j) * T.nz + k0;
+        Nbhd<VEC> nb;
+        nb.load(T, code, p, i, j, k0, c0);
+        float out[VEC];
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            out[e] = nb.apply(e);
+            pap += (double)nb.vc[e] * (double)out[e];
+        }
+        stf<VEC>(ap, c0, out);
+    }
+    double v[1] = {pap};
+    write_partials<1>(v, partA);
+}
+
+// One 1024-thread workgroup: deterministic FP64 total of G partials per value.
+template <int NV>
+__device__ __forceinline__ void reduce1024(const double* __restrict__ part, unsigned G, double (&out)[NV]) {
+    __shared__ double red[NV][16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double acc = 0.0;
+        for (unsigned w = threadIdx.x; w < G; w += 1024) acc += part[(size_t)k * G + w];
+        acc = wave_sum64(acc);
+        if (lane == 0) red[k][wave] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double s = red[k][0];
+        for (int w = 1; w < 16; ++w) s += red[k][w];
+        out[k] = s;
+    }
+}
+
+// p.Ap total and alpha = r.z / p.Ap for iteration `it` (it < 0: read the
+// device counter).
+__global__ __launch_bounds__(1024) void pap_reduce_kernel(const double* __restrict__ partA, unsigned G,
+                                                          PcgCtrl* ctrl, int it) {
+    if (ctrl->done) return;
+    double v[1];
+    reduce1024<1>(partA, G, v);
+    if (threadIdx.x != 0) return;
+    const int itv = it >= 0 ? it : ctrl->iter;
+    const double rz = ctrl->rz[itv & 1];
+    ctrl->pAp = v[0];
+    ctrl->alpha = v[0] != 0.0 ? rz / v[0] : 0.0;
+}
+
+// r.z_new, r.z_old totals for the flexible-CG direction kernel.
+__global__ __launch_bounds__(1024) void dots_reduce_kernel(const double* __restrict__ partZ, unsigned G,
+                                                           PcgCtrl* ctrl) {
+    if (ctrl->done) return;
+    double v[2];
+    reduce1024<2>(partZ, G, v);
+    if (threadIdx.x != 0) return;
+    ctrl->dots[0] = v[0];
+    ctrl->dots[1] = v[1];
+}
+
 // alpha = r.z / p.Ap (0 if p.Ap == 0, axpy_device.cu:82-91); x += alpha p,
 // r -= alpha Ap (pressure_solve_pcg.cu:192-202); partials of r.r and r.z.
 template <int VEC>
@@ -235,14 +309,9 @@ __global__ __launch_bounds__(256) void pcg_update_kernel(Tiles T, const uint8_t*
                                                          const double* __restrict__ partA,
                                                          double* __restrict__ partB, PcgCtrl* ctrl, int it) {
     if (ctrl->done) return;
-    double pap[1];
-    sum_partials<1>(partA, gridDim.x, pap);
-    const double rz = ctrl->rz[it & 1];
-    const double alpha = pap[0] != 0.0 ? rz / pap[0] : 0.0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctrl->alpha = alpha;
-        ctrl->pAp = pap[0];
-    }
+    (void)partA;
+    (void)it;
+    const double alpha = ctrl->alpha;  // formed by pap_reduce_kernel
     const float af = (float)alpha;
     uint32_t t0, t1;
     tile_span(T.ntiles, t0, t1);
@@ -378,6 +447,18 @@ static int ensure_poll() {
         else KERNEL<1><<<(GRID), 256, 0, (ST)>>>(__VA_ARGS__);                     \
     } while (0)
 
+// Ap = A p and alpha = r.z / p.Ap into the control block (it < 0: the
+// iteration number is the device counter).
+static void enqueue_spmv(const Tiles& T, int vec, const PcgBuffers& B, int it, hipStream_t st) {
+    if (B.partT) {
+        BS_VEC_LAUNCH(pcg_spmv1_kernel, vec, T.ntiles, st, T, B.code, B.p, B.ap, B.partT, B.ctrl);
+        pap_reduce_kernel<<<1, 1024, 0, st>>>(B.partT, T.ntiles, B.ctrl, it);
+    } else {
+        BS_VEC_LAUNCH(pcg_spmv_kernel, vec, B.nwg, st, T, B.code, B.p, B.ap, B.partA, B.ctrl);
+        pap_reduce_kernel<<<1, 1024, 0, st>>>(B.partA, B.nwg, B.ctrl, it);
+    }
+}
+
 // Queues PCG iterations it0, it0+1, ... (< max_iter) in batches until the
 // device reports done, polling the flag one batch behind; leaves the final
 // control block in g_poll.final.
@@ -394,7 +475,7 @@ static int pcg_iterate(const Tiles& T, int vec, const PcgBuffers& B, int it0, in
     while (it < max_iter) {
         const int nb = (max_iter - it) < batch ? (max_iter - it) : batch;
         for (int q = 0; q < nb; ++q, ++it) {
-            BS_VEC_LAUNCH(pcg_spmv_kernel, vec, B.nwg, st, T, B.code, B.p, B.ap, B.partA, B.ctrl);
+            enqueue_spmv(T, vec, B, it, st);
             BS_VEC_LAUNCH(pcg_update_kernel, vec, B.nwg, st, T, B.code, B.p, B.ap, B.x, B.r, B.partA, B.partB,
                           B.ctrl, it);
             BS_VEC_LAUNCH(pcg_direction_kernel, vec, B.nwg, st, T, B.code, B.r, B.p, B.partB, B.ctrl, it);
@@ -482,6 +563,10 @@ extern "C" int bs_pressure_solve_pcg_launch(
     B.partA = part;
     B.partB = part + B.nwg;
     B.partS = part + 3 * B.nwg;
+    // Per-tile p.Ap partials after those when ws_z has room for them too.
+    B.partT = nullptr;
+    if (cap >= 64 && zbytes - code_bytes >= (5 * (uint64_t)B.nwg + T.ntiles) * sizeof(double))
+        B.partT = part + 5 * B.nwg;
     const double tol2 = (double)pressure_tol * (double)pressure_tol;
     pcg_ctrl_init_kernel<<<1, 64, 0, st>>>(B.ctrl, tol2, max_iter, 0);
     const float bscale = -rho * dx * dx / dt;  // pressure_solve_pcg.cu:137
@@ -508,10 +593,17 @@ extern "C" int bs_pressure_solve_pcg_launch(
 // iteration convergence test, any stream.
 // ---------------------------------------------------------------------------
 
+namespace bs {
+// Tile count of the widest vector width a grid can use (an upper bound for
+// the per-tile partial arrays).
+static uint32_t max_tiles(int nx, int ny, int nz) { return make_tiles(nx, ny, nz, 1).ntiles; }
+}  // namespace bs
+
 extern "C" size_t bs_pcg_workspace_bytes(int nx, int ny, int nz) {
     if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
     const size_t n = (size_t)nx * ny * nz;
-    return align256(n) + 4 * align256(n * 4) + align256(5 * kMaxWg * sizeof(double)) + 256;
+    return align256(n) + 4 * align256(n * 4) + align256(5 * kMaxWg * sizeof(double)) + 256 +
+           align256((size_t)max_tiles(nx, ny, nz) * sizeof(double));
 }
 
 static void carve(void* ws, size_t n, PcgBuffers& B) {
@@ -525,7 +617,8 @@ static void carve(void* ws, size_t n, PcgBuffers& B) {
     B.partA = part;
     B.partB = part + kMaxWg;
     B.partS = part + 3 * kMaxWg;
-    B.ctrl = (PcgCtrl*)c;
+    B.ctrl = (PcgCtrl*)c; c += 256;
+    B.partT = (double*)c;
 }
 
 // Solves A x = b (b from div_u as above) to ||b - A x||_2 <= tol ||b||_2 on
@@ -613,15 +706,8 @@ __global__ __launch_bounds__(256) void fcg_update_kernel(Tiles T, const float* _
                                                          float* __restrict__ r, const double* __restrict__ partA,
                                                          double* __restrict__ partB, PcgCtrl* ctrl) {
     if (ctrl->done) return;
-    const int it = ctrl->iter;  // advanced only by fcg_check_kernel
-    double pap[1];
-    sum_partials<1>(partA, gridDim.x, pap);
-    const double rz = ctrl->rz[it & 1];
-    const double alpha = pap[0] != 0.0 ? rz / pap[0] : 0.0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctrl->alpha = alpha;
-        ctrl->pAp = pap[0];
-    }
+    (void)partA;
+    const double alpha = ctrl->alpha;  // formed by pap_reduce_kernel
     const float af = (float)alpha;
     uint32_t t0, t1;
     tile_span(T.ntiles, t0, t1);
@@ -698,8 +784,9 @@ __global__ __launch_bounds__(256) void fcg_direction_kernel(Tiles T, const float
                                                             unsigned G, PcgCtrl* ctrl, int init) {
     if (ctrl->done) return;
     const int it1 = ctrl->iter;  // iterations completed
-    double v[2];
-    sum_partials<2>(partZ, G, v);
+    (void)partZ;
+    (void)G;
+    const double v[2] = {ctrl->dots[0], ctrl->dots[1]};
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (!(v[0] > 0.0)) {  // preconditioner not positive on this residual
         if (lead) {
@@ -792,11 +879,12 @@ static int fcg_precond(hipStream_t st, const FcgBuffers& F, const float* r, floa
 // One flexible-CG iteration of parity `par` (z_new = z[1-par], z_old = z[par]).
 static int fcg_enqueue_iteration(const Tiles& T, int vec, const FcgBuffers& F, int par, hipStream_t st) {
     const PcgBuffers& B = F.B;
-    BS_VEC_LAUNCH(pcg_spmv_kernel, vec, B.nwg, st, T, B.code, B.p, B.ap, B.partA, B.ctrl);
+    enqueue_spmv(T, vec, B, -1, st);
     BS_VEC_LAUNCH(fcg_update_kernel, vec, B.nwg, st, T, B.p, B.ap, B.x, B.r, B.partA, B.partB, B.ctrl);
     fcg_check_kernel<<<1, 256, 0, st>>>(B.partB, B.nwg, B.ctrl);
     const int rc = fcg_precond(st, F, B.r, F.z[par ^ 1], F.z[par]);
     if (rc != hipSuccess) return rc;
+    dots_reduce_kernel<<<1, 1024, 0, st>>>(F.partZ, F.partZ_G, B.ctrl);
     BS_VEC_LAUNCH(fcg_direction_kernel, vec, B.nwg, st, T, F.z[par ^ 1], B.p, F.partZ, F.partZ_G, B.ctrl, 0);
     return (int)hipGetLastError();
 }
@@ -910,6 +998,12 @@ static int fcg_iterate(const Tiles& T, int vec, const FcgBuffers& F, int max_ite
     return rc;
 }
 
+// Two values (r.z_new, r.z_old) per preconditioner workgroup.
+static size_t partz_slots(int nx, int ny, int nz) {
+    const size_t g = mg_dots_groups(nx, ny, nz);
+    return 2 * (g > kUNetPartWg ? g : (size_t)kUNetPartWg);
+}
+
 static size_t precond_ws_bytes(int nx, int ny, int nz, int kind) {
     if (kind == 1) return unet_ws_bytes(nx, ny, nz);
     if (kind >= 2) return mg_ws_bytes(nx, ny, nz);
@@ -921,7 +1015,7 @@ static size_t precond_ws_bytes(int nx, int ny, int nz, int kind) {
 extern "C" size_t bs_fcg_workspace_bytes(int nx, int ny, int nz, int precond) {
     if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
     const size_t n = (size_t)nx * ny * nz;
-    return bs_pcg_workspace_bytes(nx, ny, nz) + 2 * align256(n * 4) + align256(2 * kUNetPartWg * sizeof(double)) +
+    return bs_pcg_workspace_bytes(nx, ny, nz) + 2 * align256(n * 4) + align256(partz_slots(nx, ny, nz) * sizeof(double)) +
            align256(precond_ws_bytes(nx, ny, nz, precond));
 }
 
@@ -947,7 +1041,7 @@ extern "C" int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx
     char* c = (char*)workspace + bs_pcg_workspace_bytes(nx, ny, nz);
     F.z[0] = (float*)c; c += align256(n * 4);
     F.z[1] = (float*)c; c += align256(n * 4);
-    F.partZ = (double*)c; c += align256(2 * kUNetPartWg * sizeof(double));
+    F.partZ = (double*)c; c += align256(partz_slots(nx, ny, nz) * sizeof(double));
     F.pws = c;
     F.kind = precond;
     F.params = unet_params;
@@ -984,6 +1078,7 @@ extern "C" int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx
     fcg_prepare_kernel<<<1, 64, 0, st>>>(B.ctrl, 0);
     int rc = fcg_precond(st, F, B.r, F.z[0], nullptr);
     if (rc != hipSuccess) return rc;
+    dots_reduce_kernel<<<1, 1024, 0, st>>>(F.partZ, F.partZ_G, B.ctrl);
     BS_VEC_LAUNCH(fcg_direction_kernel, vec, B.nwg, st, T, F.z[0], B.p, F.partZ, F.partZ_G, B.ctrl, 1);
     BS_RETURN_IF_ERR(hipGetLastError());
     rc = fcg_iterate(T, vec, F, max_iter, st);
