@@ -54,9 +54,29 @@ __device__ __forceinline__ bool tile_cell(const Tiles& T, uint32_t t, int& i, in
     return true;
 }
 
+// One tile per workgroup, tiles ordered chunk-fastest (a plane's chunks are
+// consecutive blocks) behind the XCD remap; false when this thread is past
+// the plane.  c0 is the linear index of the thread's first cell.
+__device__ __forceinline__ bool one_tile(const Tiles& T, int VEC, int& i, int& j, int& k0, int64_t& c0) {
+    const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t chunk = t % T.chunks;
+    i = (int)(t / T.chunks);
+    const uint32_t q = chunk * 256u + threadIdx.x;
+    if (q >= (uint32_t)T.ny * T.nzv.d) return false;
+    j = (int)fdiv(q, T.nzv);
+    k0 = (int)(q - (uint32_t)j * T.nzv.d) * VEC;
+    c0 = ((int64_t)i * T.ny + j) * T.nz + k0;
+    return true;
+}
+
 template <int VEC>
 __device__ __forceinline__ void ldf(const float* __restrict__ p, int64_t off, float (&v)[VEC]) {
-    if constexpr (VEC == 4) {
+    if constexpr (VEC == 8) {
+        const float4 a = *reinterpret_cast<const float4*>(p + off);
+        const float4 b = *reinterpret_cast<const float4*>(p + off + 4);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else if constexpr (VEC == 4) {
         const float4 t = *reinterpret_cast<const float4*>(p + off);
         v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
     } else if constexpr (VEC == 2) {
@@ -69,7 +89,10 @@ __device__ __forceinline__ void ldf(const float* __restrict__ p, int64_t off, fl
 
 template <int VEC>
 __device__ __forceinline__ void stf(float* __restrict__ p, int64_t off, const float (&v)[VEC]) {
-    if constexpr (VEC == 4) {
+    if constexpr (VEC == 8) {
+        *reinterpret_cast<float4*>(p + off) = make_float4(v[0], v[1], v[2], v[3]);
+        *reinterpret_cast<float4*>(p + off + 4) = make_float4(v[4], v[5], v[6], v[7]);
+    } else if constexpr (VEC == 4) {
         *reinterpret_cast<float4*>(p + off) = make_float4(v[0], v[1], v[2], v[3]);
     } else if constexpr (VEC == 2) {
         *reinterpret_cast<float2*>(p + off) = make_float2(v[0], v[1]);
@@ -80,7 +103,14 @@ __device__ __forceinline__ void stf(float* __restrict__ p, int64_t off, const fl
 
 template <int VEC>
 __device__ __forceinline__ void ldc(const uint8_t* __restrict__ p, int64_t off, uint8_t (&v)[VEC]) {
-    if constexpr (VEC == 4) {
+    if constexpr (VEC == 8) {
+        const uint2 t = *reinterpret_cast<const uint2*>(p + off);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            v[e] = (t.x >> (8 * e)) & 0xFF;
+            v[4 + e] = (t.y >> (8 * e)) & 0xFF;
+        }
+    } else if constexpr (VEC == 4) {
         const uint32_t t = *reinterpret_cast<const uint32_t*>(p + off);
         v[0] = t & 0xFF; v[1] = (t >> 8) & 0xFF; v[2] = (t >> 16) & 0xFF; v[3] = t >> 24;
     } else if constexpr (VEC == 2) {
@@ -93,7 +123,12 @@ __device__ __forceinline__ void ldc(const uint8_t* __restrict__ p, int64_t off, 
 
 template <int VEC>
 __device__ __forceinline__ void stc(uint8_t* __restrict__ p, int64_t off, const uint8_t (&v)[VEC]) {
-    if constexpr (VEC == 4) {
+    if constexpr (VEC == 8) {
+        uint2 t;
+        t.x = (uint32_t)v[0] | ((uint32_t)v[1] << 8) | ((uint32_t)v[2] << 16) | ((uint32_t)v[3] << 24);
+        t.y = (uint32_t)v[4] | ((uint32_t)v[5] << 8) | ((uint32_t)v[6] << 16) | ((uint32_t)v[7] << 24);
+        *reinterpret_cast<uint2*>(p + off) = t;
+    } else if constexpr (VEC == 4) {
         *reinterpret_cast<uint32_t*>(p + off) =
             (uint32_t)v[0] | ((uint32_t)v[1] << 8) | ((uint32_t)v[2] << 16) | ((uint32_t)v[3] << 24);
     } else if constexpr (VEC == 2) {
@@ -102,6 +137,52 @@ __device__ __forceinline__ void stc(uint8_t* __restrict__ p, int64_t off, const 
         p[off] = v[0];
     }
 }
+
+// Vectors stored in 16-bit floats (the inner solve of the mixed-precision
+// refinement): VEC elements of storage type S are loaded as one 2*VEC-byte
+// access and widened to float; stores round to nearest even.
+template <int BYTES> struct RawBytes;
+template <> struct RawBytes<2> { using T = uint16_t; };
+template <> struct RawBytes<4> { using T = uint32_t; };
+template <> struct RawBytes<8> { using T = uint2; };
+template <> struct RawBytes<16> { using T = uint4; };
+
+template <int VEC, typename S>
+__device__ __forceinline__ void ldv(const S* __restrict__ p, int64_t off, float (&v)[VEC]) {
+    if constexpr (sizeof(S) == 4) {
+        ldf<VEC>(reinterpret_cast<const float*>(p), off, v);
+    } else {
+        using R = typename RawBytes<2 * VEC>::T;
+        const R raw = *reinterpret_cast<const R*>(p + off);
+        S s[VEC];
+        __builtin_memcpy(s, &raw, sizeof(raw));
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) v[e] = (float)s[e];
+    }
+}
+
+template <int VEC, typename S>
+__device__ __forceinline__ void stv(S* __restrict__ p, int64_t off, const float (&v)[VEC]) {
+    if constexpr (sizeof(S) == 4) {
+        stf<VEC>(reinterpret_cast<float*>(p), off, v);
+    } else {
+        using R = typename RawBytes<2 * VEC>::T;
+        S s[VEC];
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) s[e] = (S)v[e];
+        R raw;
+        __builtin_memcpy(&raw, s, sizeof(raw));
+        *reinterpret_cast<R*>(p + off) = raw;
+    }
+}
+
+// Value of one stored element as float.
+template <typename S>
+__device__ __forceinline__ float ld1(const S* __restrict__ p, int64_t off) { return (float)p[off]; }
+
+// A float rounded through storage type S (identity for float).
+template <typename S>
+__device__ __forceinline__ float through(float v) { return (float)(S)v; }
 
 // z = M^-1 r for one cell from its code (diag_inverse_apply.cu:63-89).
 __device__ __forceinline__ float precond(float r, uint8_t code) {
@@ -122,24 +203,25 @@ struct Nbhd {
     uint8_t cc[VEC], cl[VEC], cr[VEC], cd[VEC], cu[VEC], cb, cf;
     float vc[VEC], vl[VEC], vr[VEC], vd[VEC], vu[VEC], vb, vf;
 
+    template <typename S>
     __device__ __forceinline__ void load(const Tiles& T, const uint8_t* __restrict__ code,
-                                         const float* __restrict__ v, int i, int j, int k0, int64_t c0) {
+                                         const S* __restrict__ v, int i, int j, int k0, int64_t c0) {
         const int64_t plane = (int64_t)T.ny * T.nz;
         ldc<VEC>(code, c0, cc);
-        ldf<VEC>(v, c0, vc);
+        ldv<VEC>(v, c0, vc);
 #pragma unroll
         for (int e = 0; e < VEC; ++e) {
             cl[e] = cr[e] = cd[e] = cu[e] = kCodeAir;
             vl[e] = vr[e] = vd[e] = vu[e] = 0.0f;
         }
-        if (i > 0) { ldc<VEC>(code, c0 - plane, cl); ldf<VEC>(v, c0 - plane, vl); }
-        if (i < T.nx - 1) { ldc<VEC>(code, c0 + plane, cr); ldf<VEC>(v, c0 + plane, vr); }
-        if (j > 0) { ldc<VEC>(code, c0 - T.nz, cd); ldf<VEC>(v, c0 - T.nz, vd); }
-        if (j < T.ny - 1) { ldc<VEC>(code, c0 + T.nz, cu); ldf<VEC>(v, c0 + T.nz, vu); }
+        if (i > 0) { ldc<VEC>(code, c0 - plane, cl); ldv<VEC>(v, c0 - plane, vl); }
+        if (i < T.nx - 1) { ldc<VEC>(code, c0 + plane, cr); ldv<VEC>(v, c0 + plane, vr); }
+        if (j > 0) { ldc<VEC>(code, c0 - T.nz, cd); ldv<VEC>(v, c0 - T.nz, vd); }
+        if (j < T.ny - 1) { ldc<VEC>(code, c0 + T.nz, cu); ldv<VEC>(v, c0 + T.nz, vu); }
         cb = cf = kCodeAir;
         vb = vf = 0.0f;
-        if (k0 > 0) { cb = code[c0 - 1]; vb = v[c0 - 1]; }
-        if (k0 + VEC < T.nz) { cf = code[c0 + VEC]; vf = v[c0 + VEC]; }
+        if (k0 > 0) { cb = code[c0 - 1]; vb = ld1(v, c0 - 1); }
+        if (k0 + VEC < T.nz) { cf = code[c0 + VEC]; vf = ld1(v, c0 + VEC); }
     }
 
     // 6*v - S(v) at element e in the reference's float order

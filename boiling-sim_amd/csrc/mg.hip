@@ -260,18 +260,6 @@ __device__ __forceinline__ float winv(uint8_t code) {
     return code >= kCodeFluid0 ? kOmega * inv : 0.0f;
 }
 
-__device__ __forceinline__ bool one_tile(const Tiles& T, int VEC, int& i, int& j, int& k0, int64_t& c0) {
-    const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t chunk = t % T.chunks;
-    i = (int)(t / T.chunks);
-    const uint32_t q = chunk * 256u + threadIdx.x;
-    if (q >= (uint32_t)T.ny * T.nzv.d) return false;
-    j = (int)fdiv(q, T.nzv);
-    k0 = (int)(q - (uint32_t)j * T.nzv.d) * VEC;
-    c0 = ((int64_t)i * T.ny + j) * T.nz + k0;
-    return true;
-}
-
 // Two damped-Jacobi sweeps from z = 0 in one pass: z1 = wD^-1 r is formed on
 // the fly at the cell and its neighbours, z = z1 + wD^-1 (r - A z1).
 template <int VEC>

@@ -229,25 +229,24 @@ __global__ __launch_bounds__(256) void pcg_spmv_kernel(Tiles T, const uint8_t* _
 
 // Ap = A p with one 256-thread tile per workgroup (better latency hiding for
 // a stencil than a grid-stride loop); one p.Ap partial per tile.
-template <int VEC>
+template <int VEC, typename S = float>
 __global__ __launch_bounds__(256) void pcg_spmv1_kernel(Tiles T, const uint8_t* __restrict__ code,
-                                                        const float* __restrict__ p, float* __restrict__ ap,
+                                                        const S* __restrict__ p, S* __restrict__ ap,
                                                         double* __restrict__ partA, const PcgCtrl* ctrl) {
     if (ctrl->done) return;
-    const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
     double pap = 0.0;
     int i, j, k0;
-    if (tile_cell<VEC>(T, t, i, j, k0)) {
-        const int64_t c0 = ((int64_t)i * T.ny + j) * T.nz + k0;
+    int64_t c0;
+    if (one_tile(T, VEC, i, j, k0, c0)) {
         Nbhd<VEC> nb;
         nb.load(T, code, p, i, j, k0, c0);
         float out[VEC];
 #pragma unroll
         for (int e = 0; e < VEC; ++e) {
-            out[e] = nb.apply(e);
+            out[e] = through<S>(nb.apply(e));
             pap += (double)nb.vc[e] * (double)out[e];
         }
-        stf<VEC>(ap, c0, out);
+        stv<VEC>(ap, c0, out);
     }
     double v[1] = {pap};
     write_partials<1>(v, partA);
@@ -301,11 +300,11 @@ __global__ __launch_bounds__(1024) void dots_reduce_kernel(const double* __restr
 
 // alpha = r.z / p.Ap (0 if p.Ap == 0, axpy_device.cu:82-91); x += alpha p,
 // r -= alpha Ap (pressure_solve_pcg.cu:192-202); partials of r.r and r.z.
-template <int VEC>
+template <int VEC, typename S = float>
 __global__ __launch_bounds__(256) void pcg_update_kernel(Tiles T, const uint8_t* __restrict__ code,
-                                                         const float* __restrict__ p,
-                                                         const float* __restrict__ ap,
-                                                         float* __restrict__ x, float* __restrict__ r,
+                                                         const S* __restrict__ p,
+                                                         const S* __restrict__ ap,
+                                                         float* __restrict__ x, S* __restrict__ r,
                                                          const double* __restrict__ partA,
                                                          double* __restrict__ partB, PcgCtrl* ctrl, int it) {
     if (ctrl->done) return;
@@ -324,20 +323,20 @@ __global__ __launch_bounds__(256) void pcg_update_kernel(Tiles T, const uint8_t*
         uint8_t cc[VEC];
         float pv[VEC], av[VEC], xv[VEC], rv[VEC];
         ldc<VEC>(code, c0, cc);
-        ldf<VEC>(p, c0, pv);
-        ldf<VEC>(ap, c0, av);
+        ldv<VEC>(p, c0, pv);
+        ldv<VEC>(ap, c0, av);
         ldf<VEC>(x, c0, xv);
-        ldf<VEC>(r, c0, rv);
+        ldv<VEC>(r, c0, rv);
 #pragma unroll
         for (int e = 0; e < VEC; ++e) {
             xv[e] = xv[e] + af * pv[e];
-            rv[e] = rv[e] - af * av[e];
+            rv[e] = through<S>(rv[e] - af * av[e]);
             const float z = precond(rv[e], cc[e]);
             rr += (double)rv[e] * (double)rv[e];
             rzn += (double)rv[e] * (double)z;
         }
         stf<VEC>(x, c0, xv);
-        stf<VEC>(r, c0, rv);
+        stv<VEC>(r, c0, rv);
     }
     double v[2] = {rr, rzn};
     write_partials<2>(v, partB);
@@ -346,10 +345,10 @@ __global__ __launch_bounds__(256) void pcg_update_kernel(Tiles T, const uint8_t*
 // Convergence test on the recursive r.r (mode 0: only when it >= 8 &&
 // it % 5 == 0, reporting it + 1, pressure_solve_pcg.cu:205-216), then
 // beta = r.z_new / r.z and p = M^-1 r + beta p.
-template <int VEC>
+template <int VEC, typename S = float>
 __global__ __launch_bounds__(256) void pcg_direction_kernel(Tiles T, const uint8_t* __restrict__ code,
-                                                            const float* __restrict__ r,
-                                                            float* __restrict__ p,
+                                                            const S* __restrict__ r,
+                                                            S* __restrict__ p,
                                                             const double* __restrict__ partB,
                                                             PcgCtrl* ctrl, int it) {
     if (ctrl->done) return;
@@ -357,11 +356,15 @@ __global__ __launch_bounds__(256) void pcg_direction_kernel(Tiles T, const uint8
     sum_partials<2>(partB, gridDim.x, v);
     const bool check = ctrl->mode != 0 || (it >= 8 && (it % 5) == 0);
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    if (check && v[0] < ctrl->stop) {
+    // mode 2 (inner solve of a refinement) also ends when the recursive
+    // residual norm has grown 4x over its start: low-precision storage has
+    // lost the process, and the caller replaces the residual.
+    const bool diverged = ctrl->mode == 2 && v[0] > 16.0 * ctrl->bsq;
+    if ((check && v[0] < ctrl->stop) || diverged) {
         if (lead) {
             ctrl->rr = v[0];
             ctrl->iter = it + 1;
-            ctrl->converged = 1;
+            ctrl->converged = diverged ? 0 : 1;
             ctrl->done = 1;
         }
         return;
@@ -387,11 +390,11 @@ __global__ __launch_bounds__(256) void pcg_direction_kernel(Tiles T, const uint8
         uint8_t cc[VEC];
         float rv[VEC], pv[VEC];
         ldc<VEC>(code, c0, cc);
-        ldf<VEC>(r, c0, rv);
-        ldf<VEC>(p, c0, pv);
+        ldv<VEC>(r, c0, rv);
+        ldv<VEC>(p, c0, pv);
 #pragma unroll
         for (int e = 0; e < VEC; ++e) pv[e] = precond(rv[e], cc[e]) + bf * pv[e];
-        stf<VEC>(p, c0, pv);
+        stv<VEC>(p, c0, pv);
     }
 }
 
@@ -459,10 +462,39 @@ static void enqueue_spmv(const Tiles& T, int vec, const PcgBuffers& B, int it, h
     }
 }
 
+#define BS_VECS_LAUNCH(KERNEL, S, VEC, GRID, ST, ...)                                   \
+    do {                                                                               \
+        if ((VEC) == 8) KERNEL<8, S><<<(GRID), 256, 0, (ST)>>>(__VA_ARGS__);           \
+        else if ((VEC) == 4) KERNEL<4, S><<<(GRID), 256, 0, (ST)>>>(__VA_ARGS__);      \
+        else if ((VEC) == 2) KERNEL<2, S><<<(GRID), 256, 0, (ST)>>>(__VA_ARGS__);      \
+        else KERNEL<1, S><<<(GRID), 256, 0, (ST)>>>(__VA_ARGS__);                      \
+    } while (0)
+
+// One PCG iteration whose vectors r, p, Ap are stored as S (x stays float).
+template <typename S>
+static void enqueue_pcg_iteration(const Tiles& T, int vec, const PcgBuffers& B, int it, hipStream_t st) {
+    if constexpr (sizeof(S) == 4) {
+        enqueue_spmv(T, vec, B, it, st);
+        BS_VEC_LAUNCH(pcg_update_kernel, vec, B.nwg, st, T, B.code, B.p, B.ap, B.x, B.r, B.partA, B.partB,
+                      B.ctrl, it);
+        BS_VEC_LAUNCH(pcg_direction_kernel, vec, B.nwg, st, T, B.code, B.r, B.p, B.partB, B.ctrl, it);
+    } else {
+        S* p = reinterpret_cast<S*>(B.p);
+        S* ap = reinterpret_cast<S*>(B.ap);
+        S* r = reinterpret_cast<S*>(B.r);
+        BS_VECS_LAUNCH(pcg_spmv1_kernel, S, vec, T.ntiles, st, T, B.code, p, ap, B.partT, B.ctrl);
+        pap_reduce_kernel<<<1, 1024, 0, st>>>(B.partT, T.ntiles, B.ctrl, it);
+        BS_VECS_LAUNCH(pcg_update_kernel, S, vec, B.nwg, st, T, B.code, p, ap, B.x, r, B.partA, B.partB, B.ctrl,
+                       it);
+        BS_VECS_LAUNCH(pcg_direction_kernel, S, vec, B.nwg, st, T, B.code, r, p, B.partB, B.ctrl, it);
+    }
+}
+
 // Queues PCG iterations it0, it0+1, ... (< max_iter) in batches until the
 // device reports done, polling the flag one batch behind; leaves the final
 // control block in g_poll.final.
-static int pcg_iterate(const Tiles& T, int vec, const PcgBuffers& B, int it0, int max_iter, hipStream_t st) {
+template <typename S>
+static int pcg_iterate_t(const Tiles& T, int vec, const PcgBuffers& B, int it0, int max_iter, hipStream_t st) {
     BS_RETURN_IF_ERR((hipError_t)ensure_poll());
     hipEvent_t ev[2];
     BS_RETURN_IF_ERR(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
@@ -474,12 +506,7 @@ static int pcg_iterate(const Tiles& T, int vec, const PcgBuffers& B, int it0, in
     g_poll.slots[0] = g_poll.slots[2] = 0;
     while (it < max_iter) {
         const int nb = (max_iter - it) < batch ? (max_iter - it) : batch;
-        for (int q = 0; q < nb; ++q, ++it) {
-            enqueue_spmv(T, vec, B, it, st);
-            BS_VEC_LAUNCH(pcg_update_kernel, vec, B.nwg, st, T, B.code, B.p, B.ap, B.x, B.r, B.partA, B.partB,
-                          B.ctrl, it);
-            BS_VEC_LAUNCH(pcg_direction_kernel, vec, B.nwg, st, T, B.code, B.r, B.p, B.partB, B.ctrl, it);
-        }
+        for (int q = 0; q < nb; ++q, ++it) enqueue_pcg_iteration<S>(T, vec, B, it, st);
         rc = hipGetLastError();
         if (rc != hipSuccess) break;
         rc = hipMemcpyAsync(g_poll.slots + 2 * slot, (const char*)B.ctrl + done_off, 2 * sizeof(int),
@@ -497,7 +524,7 @@ static int pcg_iterate(const Tiles& T, int vec, const PcgBuffers& B, int it0, in
             if (g_poll.slots[2 * prev] != 0) break;
         }
         slot ^= 1;
-        if (batch < 64) batch *= 2;
+        if (batch < 16) batch *= 2;
     }
     if (rc == hipSuccess) {
         rc = hipMemcpyAsync(g_poll.final, B.ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, st);
@@ -506,6 +533,10 @@ static int pcg_iterate(const Tiles& T, int vec, const PcgBuffers& B, int it0, in
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
     return (int)rc;
+}
+
+static int pcg_iterate(const Tiles& T, int vec, const PcgBuffers& B, int it0, int max_iter, hipStream_t st) {
+    return pcg_iterate_t<float>(T, vec, B, it0, max_iter, st);
 }
 
 static unsigned pick_nwg(const Tiles& T, unsigned cap) {
@@ -960,7 +991,10 @@ static int fcg_iterate(const Tiles& T, int vec, const FcgBuffers& F, int max_ite
     BS_RETURN_IF_ERR(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     BS_RETURN_IF_ERR(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     const size_t done_off = offsetof(PcgCtrl, done);
-    int it = 0, batch = 1, slot = 0;
+    // One pair per poll: a pair is milliseconds of GPU work, so the host stays
+    // ahead, and at most two queued pairs run (as no-ops) after convergence.
+    const int batch = 1;
+    int it = 0, slot = 0;
     bool pending[2] = {false, false};
     g_poll.slots[0] = g_poll.slots[2] = 0;
     while (it < max_iter && rc == hipSuccess) {
@@ -987,7 +1021,6 @@ static int fcg_iterate(const Tiles& T, int vec, const FcgBuffers& F, int max_ite
             if (g_poll.slots[2 * prev] != 0) break;
         }
         slot ^= 1;
-        if (batch < 8) batch *= 2;
     }
     if (rc == hipSuccess) {
         rc = hipMemcpyAsync(g_poll.final, B.ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, st);
@@ -1119,6 +1152,505 @@ extern "C" int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx
         stats->pcg_iterations = it0 - fcg_iters;
         stats->restarts = restarts;
         stats->relres_recursive = h.bsq > 0.0 ? sqrt(h.rr / h.bsq) : 0.0;
+    }
+    return hipSuccess;
+}
+
+// ===========================================================================
+// Mixed-precision iterative refinement
+//   x (FP64) <- x + A^-1 (b - A x),  the residual formed in FP64 from the FP64
+//   iterate and A^-1 applied approximately by a cold-started Jacobi-PCG whose
+//   vectors r, p, Ap are stored in FP32, FP16 or BF16 (arithmetic in FP32,
+//   dots in FP64), on the residual scaled to unit RMS over the fluid cells.
+// The FP64 iterate is what removes the FP32 floor on the true residual: for a
+// smooth right-hand side |x| >> |b|, and rounding x to FP32 alone leaves a
+// relative residual of order 1e-5.
+// ===========================================================================
+namespace bs {
+
+struct RefCtrl {
+    double bb;          // b.b of the outer problem
+    double rr;          // |b - A x|^2 at the current FP64 iterate
+    double scale;       // inner residual = scale * (b - A x), unit RMS over fluid
+    double scale_prev;  // the scale before the last residual pass
+    unsigned n_fluid;   // fluid cells
+    int pad;
+};
+
+template <int VEC>
+__device__ __forceinline__ void ldd(const double* __restrict__ p, int64_t off, double (&v)[VEC]) {
+    if constexpr (VEC == 2) {
+        const double2 t = *reinterpret_cast<const double2*>(p + off);
+        v[0] = t.x; v[1] = t.y;
+    } else {
+        v[0] = p[off];
+    }
+}
+
+__device__ __forceinline__ double nbr64(uint8_t c, double v, double xs) {
+    return c >= kCodeFluid0 ? v : (c == kCodeSolid ? xs : 0.0);
+}
+
+// r = b - A x with x, the stencil and the subtraction in FP64 (stored as
+// FP32); one partial of r.r per tile.
+template <int VEC>
+__global__ __launch_bounds__(256) void refine_residual_kernel(Tiles T, const uint8_t* __restrict__ code,
+                                                              const float* __restrict__ b,
+                                                              const double* __restrict__ x, float* __restrict__ r,
+                                                              double* __restrict__ part) {
+    double rr = 0.0;
+    int i, j, k0;
+    int64_t c0;
+    if (one_tile(T, VEC, i, j, k0, c0)) {
+        const int64_t plane = (int64_t)T.ny * T.nz;
+        uint8_t cc[VEC], cl[VEC], cr[VEC], cd[VEC], cu[VEC];
+        double xc[VEC], xl[VEC], xr[VEC], xd[VEC], xu[VEC];
+        ldc<VEC>(code, c0, cc);
+        ldd<VEC>(x, c0, xc);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            cl[e] = cr[e] = cd[e] = cu[e] = kCodeAir;
+            xl[e] = xr[e] = xd[e] = xu[e] = 0.0;
+        }
+        if (i > 0) { ldc<VEC>(code, c0 - plane, cl); ldd<VEC>(x, c0 - plane, xl); }
+        if (i < T.nx - 1) { ldc<VEC>(code, c0 + plane, cr); ldd<VEC>(x, c0 + plane, xr); }
+        if (j > 0) { ldc<VEC>(code, c0 - T.nz, cd); ldd<VEC>(x, c0 - T.nz, xd); }
+        if (j < T.ny - 1) { ldc<VEC>(code, c0 + T.nz, cu); ldd<VEC>(x, c0 + T.nz, xu); }
+        uint8_t cb = kCodeAir, cf = kCodeAir;
+        double xb = 0.0, xf = 0.0;
+        if (k0 > 0) { cb = code[c0 - 1]; xb = x[c0 - 1]; }
+        if (k0 + VEC < T.nz) { cf = code[c0 + VEC]; xf = x[c0 + VEC]; }
+        float bv[VEC], rv[VEC];
+        ldf<VEC>(b, c0, bv);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            if (cc[e] < kCodeFluid0) { rv[e] = 0.0f; continue; }
+            const double xs = xc[e];
+            const uint8_t c_b = e > 0 ? cc[e - 1] : cb;
+            const double v_b = e > 0 ? xc[e - 1] : xb;
+            const uint8_t c_f = e < VEC - 1 ? cc[e + 1] : cf;
+            const double v_f = e < VEC - 1 ? xc[e + 1] : xf;
+            double s = nbr64(cl[e], xl[e], xs) + nbr64(cr[e], xr[e], xs);
+            s += nbr64(cd[e], xd[e], xs);
+            s += nbr64(cu[e], xu[e], xs);
+            s += nbr64(c_b, v_b, xs);
+            s += nbr64(c_f, v_f, xs);
+            const double r64 = (double)bv[e] - (6.0 * xs - s);
+            rv[e] = (float)r64;
+            rr += r64 * r64;
+        }
+        stf<VEC>(r, c0, rv);
+    }
+    double v[1] = {rr};
+    write_partials<1>(v, part);
+}
+
+// what 0: b.b; 1: r.r and the inner scale sqrt(n_fluid / r.r).
+__global__ __launch_bounds__(1024) void refine_reduce_kernel(const double* __restrict__ part, unsigned G,
+                                                             RefCtrl* rc, int what) {
+    double v[1];
+    reduce1024<1>(part, G, v);
+    if (threadIdx.x != 0) return;
+    if (what == 0) {
+        rc->bb = v[0];
+    } else {
+        rc->rr = v[0];
+        rc->scale_prev = rc->scale;
+        rc->scale = v[0] > 0.0 ? sqrt((double)rc->n_fluid / v[0]) : 1.0;
+    }
+}
+
+// Residual replacement (reliable update): r = scale * rs with rs the
+// FP64-formed true residual, p rescaled by scale / scale_prev so it stays the
+// same direction in the new units, e = 0; partials of r.r and r.z laid out as
+// pcg_update_kernel writes them, for the direction step that follows.
+template <int VEC, typename S>
+__global__ __launch_bounds__(256) void refine_replace_kernel(Tiles T, const uint8_t* __restrict__ code,
+                                                             float* __restrict__ rs_e, const RefCtrl* rc,
+                                                             S* __restrict__ r, S* __restrict__ p,
+                                                             double* __restrict__ partB) {
+    const float sc = (float)rc->scale;
+    const float ratio = (float)(rc->scale / rc->scale_prev);
+    uint32_t t0, t1;
+    tile_span(T.ntiles, t0, t1);
+    double rr = 0.0, rz = 0.0;
+    const int64_t plane = (int64_t)T.ny * T.nz;
+    for (uint32_t t = t0; t < t1; ++t) {
+        int i, j, k0;
+        if (!tile_cell<VEC>(T, t, i, j, k0)) continue;
+        const int64_t c0 = (int64_t)i * plane + (int64_t)j * T.nz + k0;
+        uint8_t cc[VEC];
+        float rv[VEC], pv[VEC], zero[VEC];
+        ldc<VEC>(code, c0, cc);
+        ldf<VEC>(rs_e, c0, rv);
+        ldv<VEC>(p, c0, pv);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            zero[e] = 0.0f;
+            rv[e] = cc[e] >= kCodeFluid0 ? through<S>(sc * rv[e]) : 0.0f;
+            pv[e] = through<S>(ratio * pv[e]);
+            const float z = precond(rv[e], cc[e]);
+            rr += (double)rv[e] * (double)rv[e];
+            rz += (double)rv[e] * (double)z;
+        }
+        stf<VEC>(rs_e, c0, zero);
+        stv<VEC>(r, c0, rv);
+        stv<VEC>(p, c0, pv);
+    }
+    double v[2] = {rr, rz};
+    write_partials<2>(v, partB);
+}
+
+// Resumes the inner PCG after a replacement at iteration it: r.z of the last
+// direction in the new units, a fresh stop level (the replaced residual has
+// r.r = n_fluid by construction of the scale).
+__global__ void refine_resume_kernel(PcgCtrl* ctrl, const RefCtrl* rc, int it, double inner_tol2) {
+    if (threadIdx.x != 0) return;
+    const double ratio = rc->scale / rc->scale_prev;
+    ctrl->rz[it & 1] *= ratio * ratio;
+    ctrl->done = 0;
+    ctrl->converged = 0;
+    ctrl->bsq = (double)rc->n_fluid;
+    ctrl->stop = inner_tol2 * (double)rc->n_fluid;
+}
+
+// Reduction the next inner segment aims at: inner_tol, or less when that
+// would overshoot the outer tolerance (0.7 leaves room for the gap between
+// the recursive and the true residual).
+static double segment_target(double inner_tol, double tol, double rel) {
+    const double need = rel > 0.0 ? 0.7 * tol / rel : inner_tol;
+    return need > inner_tol ? (need < 1.0 ? need : 1.0) : inner_tol;
+}
+
+// Inner cold start from the stored residual rs (FP32): b_in = scale * rs
+// rounded to S, e = 0 (in place of rs), r = b_in, p = M^-1 b_in; partials of
+// b_in.b_in and r.z.
+template <int VEC, typename S>
+__global__ __launch_bounds__(256) void refine_inner_start_kernel(Tiles T, const uint8_t* __restrict__ code,
+                                                                 float* __restrict__ rs_e, const RefCtrl* rc,
+                                                                 S* __restrict__ r, S* __restrict__ p,
+                                                                 double* __restrict__ partS) {
+    const float sc = (float)rc->scale;
+    uint32_t t0, t1;
+    tile_span(T.ntiles, t0, t1);
+    double bb = 0.0, rz = 0.0;
+    const int64_t plane = (int64_t)T.ny * T.nz;
+    for (uint32_t t = t0; t < t1; ++t) {
+        int i, j, k0;
+        if (!tile_cell<VEC>(T, t, i, j, k0)) continue;
+        const int64_t c0 = (int64_t)i * plane + (int64_t)j * T.nz + k0;
+        uint8_t cc[VEC];
+        float rv[VEC], zv[VEC], zero[VEC];
+        ldc<VEC>(code, c0, cc);
+        ldf<VEC>(rs_e, c0, rv);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            zero[e] = 0.0f;
+            rv[e] = cc[e] >= kCodeFluid0 ? through<S>(sc * rv[e]) : 0.0f;
+            zv[e] = through<S>(precond(rv[e], cc[e]));
+            bb += (double)rv[e] * (double)rv[e];
+            rz += (double)rv[e] * (double)zv[e];
+        }
+        stf<VEC>(rs_e, c0, zero);
+        stv<VEC>(r, c0, rv);
+        stv<VEC>(p, c0, zv);
+    }
+    double v[2] = {bb, rz};
+    write_partials<2>(v, partS);
+}
+
+// x += e / scale (FP64).
+__global__ __launch_bounds__(256) void refine_accumulate_kernel(int64_t n, const float* __restrict__ e,
+                                                                const RefCtrl* rc, double* __restrict__ x) {
+    const double inv = 1.0 / rc->scale;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += stride) x[t] += (double)e[t] * inv;
+}
+
+__global__ __launch_bounds__(256) void refine_zero_kernel(int64_t n, double* __restrict__ x) {
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += stride) x[t] = 0.0;
+}
+
+__global__ __launch_bounds__(256) void refine_round_kernel(int64_t n, const double* __restrict__ x,
+                                                           float* __restrict__ out) {
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += stride) out[t] = (float)x[t];
+}
+
+__global__ void refine_ctrl_init_kernel(RefCtrl* rc) {
+    if (threadIdx.x != 0) return;
+    RefCtrl c;
+    memset(&c, 0, sizeof(c));
+    c.scale = 1.0;
+    c.scale_prev = 1.0;
+    *rc = c;
+}
+
+static int refine_vec(int nz, int storage) {
+    if (storage != 0 && nz % 8 == 0) return 8;  // 16-byte accesses of 16-bit values
+    return nz % 4 == 0 ? 4 : (nz % 2 == 0 ? 2 : 1);
+}
+
+static unsigned stream_grid(int64_t n) {
+    const int64_t g = (n + 255) / 256;
+    return (unsigned)(g < 4096 ? (g < 1 ? 1 : g) : 4096);
+}
+
+// Outer state of a refinement: the FP32 residual (then the inner solution),
+// the residual-pass partials and the control block, carved after the inner
+// solver's own workspace.
+struct RefineBufs {
+    float* rs;
+    double* rpart;
+    RefCtrl* rc;
+    Tiles T2;
+    int v2;
+};
+
+static size_t refine_tail_bytes(int nx, int ny, int nz) {
+    const size_t n = (size_t)nx * ny * nz;
+    return align256(n * 4) + align256((size_t)make_tiles(nx, ny, nz, 1).ntiles * sizeof(double)) + 256;
+}
+
+// Codes and b (FP32, as the reference's compute_b) into B, b.b and the fluid
+// count into the control block, x = 0 unless warm.
+static int refine_begin(PcgBuffers& B, RefineBufs& R, char* tail, double* x, const float* div_u, const int* mat,
+                        int nx, int ny, int nz, float dx, float dt, float rho, int mat_fluid, int mat_air,
+                        int warm_start, hipStream_t st) {
+    const int64_t n = (int64_t)nx * ny * nz;
+    R.rs = (float*)tail; tail += align256((size_t)n * 4);
+    R.rpart = (double*)tail; tail += align256((size_t)make_tiles(nx, ny, nz, 1).ntiles * sizeof(double));
+    R.rc = (RefCtrl*)tail;
+    R.v2 = (nz % 2 == 0 && ((uintptr_t)x % 16) == 0) ? 2 : 1;
+    R.T2 = make_tiles(nx, ny, nz, R.v2);
+    const int v4 = vec_for(nz, {div_u, mat, B.b, B.code});
+    const Tiles T4 = make_tiles(nx, ny, nz, v4);
+    const unsigned nwg4 = pick_nwg(T4, kMaxWg);
+    const float bscale = -rho * dx * dx / dt;
+    refine_ctrl_init_kernel<<<1, 64, 0, st>>>(R.rc);
+    BS_VEC_LAUNCH(pcg_setup_kernel, v4, nwg4, st, T4, mat, div_u, bscale, mat_fluid, mat_air, 0, B.code, B.b,
+                  nullptr, nullptr, nullptr, B.partS);
+    refine_reduce_kernel<<<1, 1024, 0, st>>>(B.partS, nwg4, R.rc, 0);
+    count_fluid_kernel<<<256, 256, 0, st>>>(n, B.code, (int*)&R.rc->n_fluid);
+    if (!warm_start) refine_zero_kernel<<<stream_grid(n), 256, 0, st>>>(n, x);
+    return (int)hipGetLastError();
+}
+
+// rs = b - A x (FP64-formed), its norm and the next inner scale; the control
+// block is copied to h.
+static int refine_residual(const PcgBuffers& B, const RefineBufs& R, const double* x, hipStream_t st, RefCtrl* h) {
+    if (R.v2 == 2) refine_residual_kernel<2><<<R.T2.ntiles, 256, 0, st>>>(R.T2, B.code, B.b, x, R.rs, R.rpart);
+    else refine_residual_kernel<1><<<R.T2.ntiles, 256, 0, st>>>(R.T2, B.code, B.b, x, R.rs, R.rpart);
+    refine_reduce_kernel<<<1, 1024, 0, st>>>(R.rpart, R.T2.ntiles, R.rc, 1);
+    BS_RETURN_IF_ERR(hipGetLastError());
+    BS_RETURN_IF_ERR(hipMemcpyAsync(h, R.rc, sizeof(*h), hipMemcpyDeviceToHost, st));
+    return (int)hipStreamSynchronize(st);
+}
+
+}  // namespace bs
+
+// Launch of KERNEL<vi, S> for the storage type of the call (SR casts a float
+// buffer to that type).
+#define REFINE_S_LAUNCH(KERNEL, GRID, ...)                                                  \
+    do {                                                                                   \
+        if (storage == 0) {                                                                \
+            using SType = float;                                                           \
+            BS_VECS_LAUNCH(KERNEL, SType, vi, GRID, st, __VA_ARGS__);                      \
+        } else if (storage == 1) {                                                         \
+            using SType = _Float16;                                                        \
+            BS_VECS_LAUNCH(KERNEL, SType, vi, GRID, st, __VA_ARGS__);                      \
+        } else {                                                                           \
+            using SType = __bf16;                                                          \
+            BS_VECS_LAUNCH(KERNEL, SType, vi, GRID, st, __VA_ARGS__);                      \
+        }                                                                                  \
+    } while (0)
+#define SR(buf) reinterpret_cast<SType*>(buf)
+
+extern "C" size_t bs_refine_workspace_bytes(int nx, int ny, int nz) {
+    if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
+    const size_t n = (size_t)nx * ny * nz;
+    (void)n;
+    return bs_pcg_workspace_bytes(nx, ny, nz) + refine_tail_bytes(nx, ny, nz);
+}
+
+// Solves A x = b (b from div_u as bs_pcg_solve) for an FP64 x to a true
+// relative residual tol (FP64 residual of the FP64 iterate) by iterative
+// refinement.  storage: 0 FP32, 1 FP16, 2 BF16 vectors in the inner PCG, which
+// runs to a recursive relative residual inner_tol (at most max_inner
+// iterations) per refinement step; at most max_outer steps.  warm_start uses
+// x's contents; x32 (nullable) receives x rounded to FP32.
+extern "C" int bs_refine_solve(double* x, float* x32, const float* div_u, const int* mat, int nx, int ny, int nz,
+                               float dx, float dt, float rho, int mat_fluid, int mat_air, double tol, int max_outer,
+                               int storage, double inner_tol, int max_inner, int warm_start, void* workspace,
+                               size_t workspace_bytes, hipStream_t st, bs_refine_stats* stats) {
+    if (stats) memset(stats, 0, sizeof(*stats));
+    if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
+    if (storage < 0 || storage > 2 || max_outer < 0) return (int)hipErrorInvalidValue;
+    if (workspace_bytes < bs_refine_workspace_bytes(nx, ny, nz)) return (int)hipErrorInvalidValue;
+    BS_RETURN_IF_ERR((hipError_t)ensure_poll());
+    const int64_t n = (int64_t)nx * ny * nz;
+    PcgBuffers B;
+    carve(workspace, (size_t)n, B);
+    RefineBufs R;
+    int rcode = refine_begin(B, R, (char*)workspace + bs_pcg_workspace_bytes(nx, ny, nz), x, div_u, mat, nx, ny,
+                             nz, dx, dt, rho, mat_fluid, mat_air, warm_start, st);
+    if (rcode != hipSuccess) return rcode;
+    float* rs = R.rs;
+    RefCtrl* rc = R.rc;
+    B.x = rs;
+    const int vi = refine_vec(nz, storage);
+    const Tiles Ti = make_tiles(nx, ny, nz, vi);
+    B.nwg = pick_nwg(Ti, kMaxWg);
+    B.partB = B.partA + B.nwg;
+    B.partS = B.partA + 3 * B.nwg;
+    int outer = 0, it = 0, conv = 0;
+    double rel = 0.0, bb = 0.0, inner_rel = 0.0;
+    for (;;) {
+        // True residual of the FP64 iterate (and the scale of the next inner step).
+        RefCtrl h;
+        rcode = refine_residual(B, R, x, st, &h);
+        if (rcode != hipSuccess) return rcode;
+        bb = h.bb;
+        rel = bb > 0.0 ? sqrt(h.rr / bb) : 0.0;
+        if (bb == 0.0 || rel <= tol) { conv = 1; break; }
+        if (outer >= max_outer || it >= max_inner) break;
+        const double f = segment_target(inner_tol, tol, rel);
+        if (outer == 0) {
+            // Cold start of the inner PCG on the scaled residual.
+            pcg_ctrl_init_kernel<<<1, 64, 0, st>>>(B.ctrl, f * f, max_inner, 2);
+            REFINE_S_LAUNCH(refine_inner_start_kernel, B.nwg, Ti, B.code, rs, rc, SR(B.r), SR(B.p), B.partS);
+            pcg_setup_finalize<<<1, 256, 0, st>>>(B.partS, B.nwg, B.ctrl, 1);
+        } else {
+            // Replace the recursive residual by the true one and keep the
+            // search direction: the Krylov process continues instead of
+            // restarting, so the refinement costs no extra iterations.
+            REFINE_S_LAUNCH(refine_replace_kernel, B.nwg, Ti, B.code, rs, rc, SR(B.r), SR(B.p), B.partB);
+            refine_resume_kernel<<<1, 64, 0, st>>>(B.ctrl, rc, it - 1, f * f);
+            REFINE_S_LAUNCH(pcg_direction_kernel, B.nwg, Ti, B.code, SR(B.r), SR(B.p), B.partB, B.ctrl, it - 1);
+        }
+        BS_RETURN_IF_ERR(hipGetLastError());
+        ++outer;
+        if (storage == 0) rcode = pcg_iterate_t<float>(Ti, vi, B, it, max_inner, st);
+        else if (storage == 1) rcode = pcg_iterate_t<_Float16>(Ti, vi, B, it, max_inner, st);
+        else rcode = pcg_iterate_t<__bf16>(Ti, vi, B, it, max_inner, st);
+        if (rcode != hipSuccess) return rcode;
+        it = g_poll.final->iter;
+        inner_rel = g_poll.final->rr > 0.0 && h.n_fluid ? sqrt(g_poll.final->rr / (double)h.n_fluid) : 0.0;
+        refine_accumulate_kernel<<<stream_grid(n), 256, 0, st>>>(n, rs, rc, x);
+    }
+    if (x32) refine_round_kernel<<<stream_grid(n), 256, 0, st>>>(n, x, x32);
+    BS_RETURN_IF_ERR(hipGetLastError());
+    BS_RETURN_IF_ERR(hipStreamSynchronize(st));
+    if (stats) {
+        stats->outer = outer;
+        stats->inner_iterations = it;
+        stats->converged = conv;
+        stats->storage = storage;
+        stats->relres_true = rel;
+        stats->bnorm = sqrt(bb);
+        stats->relres_inner_last = inner_rel;
+    }
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// The same FP64 refinement with the learned flexible CG as the inner solver
+// (cold-started on each scaled residual; FP32 vectors).
+// ---------------------------------------------------------------------------
+
+extern "C" size_t bs_fcg_refine_workspace_bytes(int nx, int ny, int nz, int precond) {
+    if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
+    return bs_fcg_workspace_bytes(nx, ny, nz, precond) + refine_tail_bytes(nx, ny, nz);
+}
+
+// x (FP64) solves A x = b to a true relative residual tol; each refinement
+// step runs bs_fcg_solve's flexible CG (preconditioner `precond`, 1-4) on the
+// residual scaled to unit RMS until its recursive residual drops by inner_tol
+// (at most max_inner iterations per step).  stats.inner_iterations counts the
+// flexible-CG iterations of all steps.
+extern "C" int bs_fcg_refine_solve(double* x, float* x32, const float* div_u, const int* mat, int nx, int ny,
+                                   int nz, float dx, float dt, float rho, int mat_fluid, int mat_air, double tol,
+                                   int max_outer, double inner_tol, int max_inner, int warm_start, int precond,
+                                   const float* unet_params, const void* unet_packed, void* workspace,
+                                   size_t workspace_bytes, hipStream_t st, bs_refine_stats* stats) {
+    if (stats) memset(stats, 0, sizeof(*stats));
+    if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
+    if (precond < 1 || precond > 4 || max_outer < 0) return (int)hipErrorInvalidValue;
+    if (precond != 3 && (!unet_params || !unet_packed)) return (int)hipErrorInvalidValue;
+    if (workspace_bytes < bs_fcg_refine_workspace_bytes(nx, ny, nz, precond)) return (int)hipErrorInvalidValue;
+    BS_RETURN_IF_ERR((hipError_t)ensure_poll());
+    const size_t n = (size_t)nx * ny * nz;
+    FcgBuffers F;
+    carve(workspace, n, F.B);
+    PcgBuffers& B = F.B;
+    char* c = (char*)workspace + bs_pcg_workspace_bytes(nx, ny, nz);
+    F.z[0] = (float*)c; c += align256(n * 4);
+    F.z[1] = (float*)c; c += align256(n * 4);
+    F.partZ = (double*)c; c += align256(partz_slots(nx, ny, nz) * sizeof(double));
+    F.pws = c;
+    F.kind = precond;
+    F.params = unet_params;
+    F.packed = unet_packed;
+    F.nx = nx; F.ny = ny; F.nz = nz;
+    F.partZ_G = precond == 1 ? kUNetPartWg : mg_dots_groups(nx, ny, nz);
+    RefineBufs R;
+    int rc = refine_begin(B, R, (char*)workspace + bs_fcg_workspace_bytes(nx, ny, nz, precond), x, div_u, mat, nx,
+                          ny, nz, dx, dt, rho, mat_fluid, mat_air, warm_start, st);
+    if (rc != hipSuccess) return rc;
+    B.x = R.rs;  // the inner solution e
+    const int vec = vec_for(nz, {B.x, B.b, B.r, B.p, B.ap, B.code, F.z[0], F.z[1]});
+    const Tiles T = make_tiles(nx, ny, nz, vec);
+    B.nwg = pick_nwg(T, kMaxWg);
+    B.partB = B.partA + B.nwg;
+    B.partS = B.partA + 3 * B.nwg;
+    if (precond == 1) {
+        RefCtrl h;
+        BS_RETURN_IF_ERR(hipMemcpyAsync(&h, R.rc, sizeof(h), hipMemcpyDeviceToHost, st));
+        BS_RETURN_IF_ERR(hipStreamSynchronize(st));
+        F.n_fluid = (double)h.n_fluid;
+    } else {
+        rc = mg_setup(st, F.pws, nx, ny, nz, mat, mat_fluid, mat_air, &F.n_fluid);
+        if (rc != hipSuccess) return rc;
+    }
+    int outer = 0, its = 0, conv = 0;
+    double rel = 0.0, bb = 0.0, inner_rel = 0.0;
+    for (;;) {
+        RefCtrl h;
+        rc = refine_residual(B, R, x, st, &h);
+        if (rc != hipSuccess) return rc;
+        bb = h.bb;
+        rel = bb > 0.0 ? sqrt(h.rr / bb) : 0.0;
+        if (bb == 0.0 || rel <= tol) { conv = 1; break; }
+        if (outer >= max_outer) break;
+        ++outer;
+        const double f = segment_target(inner_tol, tol, rel);
+        pcg_ctrl_init_kernel<<<1, 64, 0, st>>>(B.ctrl, f * f, max_inner, 1);
+        BS_VECS_LAUNCH(refine_inner_start_kernel, float, vec, B.nwg, st, T, B.code, R.rs, R.rc, B.r, B.p, B.partS);
+        pcg_setup_finalize<<<1, 256, 0, st>>>(B.partS, B.nwg, B.ctrl, 1);
+        fcg_prepare_kernel<<<1, 64, 0, st>>>(B.ctrl, 0);
+        rc = fcg_precond(st, F, B.r, F.z[0], nullptr);
+        if (rc != hipSuccess) return rc;
+        dots_reduce_kernel<<<1, 1024, 0, st>>>(F.partZ, F.partZ_G, B.ctrl);
+        BS_VEC_LAUNCH(fcg_direction_kernel, vec, B.nwg, st, T, F.z[0], B.p, F.partZ, F.partZ_G, B.ctrl, 1);
+        BS_RETURN_IF_ERR(hipGetLastError());
+        rc = fcg_iterate(T, vec, F, max_inner, st);
+        if (rc != hipSuccess) return rc;
+        its += g_poll.final->iter;
+        inner_rel = g_poll.final->bsq > 0.0 ? sqrt(g_poll.final->rr / g_poll.final->bsq) : 0.0;
+        refine_accumulate_kernel<<<stream_grid((int64_t)n), 256, 0, st>>>((int64_t)n, R.rs, R.rc, x);
+    }
+    if (x32) refine_round_kernel<<<stream_grid((int64_t)n), 256, 0, st>>>((int64_t)n, x, x32);
+    BS_RETURN_IF_ERR(hipGetLastError());
+    BS_RETURN_IF_ERR(hipStreamSynchronize(st));
+    if (stats) {
+        stats->outer = outer;
+        stats->inner_iterations = its;
+        stats->converged = conv;
+        stats->storage = 0;
+        stats->relres_true = rel;
+        stats->bnorm = sqrt(bb);
+        stats->relres_inner_last = inner_rel;
     }
     return hipSuccess;
 }

@@ -55,6 +55,23 @@ class FcgStats(Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class RefineStats(Structure):
+    """bs_refine_stats (include/boilsim_hip.h)."""
+
+    _fields_ = [
+        ("outer", c_int),
+        ("inner_iterations", c_int),
+        ("converged", c_int),
+        ("storage", c_int),
+        ("relres_true", c_double),
+        ("bnorm", c_double),
+        ("relres_inner_last", c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 VP = c_void_p  # device pointer
 I = c_int
 F = c_float
@@ -97,6 +114,12 @@ SIGNATURES = {
     "bs_unet_adam": [VP, VP, VP, VP, I, F, F, S],
     "bs_fcg_workspace_bytes": [I, I, I, I],
     "bs_fcg_solve": [VP, VP, VP, I, I, I, F, F, F, I, I, c_double, I, I, I, VP, VP, VP, c_size_t, S, c_void_p],
+    "bs_refine_workspace_bytes": [I, I, I],
+    "bs_fcg_refine_workspace_bytes": [I, I, I, I],
+    "bs_fcg_refine_solve": [VP, VP, VP, VP, I, I, I, F, F, F, I, I, c_double, I, c_double, I, I, I, VP, VP, VP,
+                            c_size_t, S, c_void_p],
+    "bs_refine_solve": [VP, VP, VP, VP, I, I, I, F, F, F, I, I, c_double, I, I, c_double, I, I, VP, c_size_t, S,
+                        c_void_p],
 }
 
 _RESTYPES = {
@@ -106,6 +129,8 @@ _RESTYPES = {
     "bs_unet_packed_bytes": c_size_t,
     "bs_unet_workspace_bytes": c_size_t,
     "bs_fcg_workspace_bytes": c_size_t,
+    "bs_refine_workspace_bytes": c_size_t,
+    "bs_fcg_refine_workspace_bytes": c_size_t,
 }
 
 

@@ -210,3 +210,63 @@ class SimCore:
             ctypes.byref(st),
         )
         return st.as_dict()
+
+    # ---- mixed-precision iterative refinement -------------------------------
+    STORAGE = {"fp32": 0, "fp16": 1, "bf16": 2}
+    # Residual reduction between replacements by storage type: 16-bit vectors
+    # keep the recursive residual close to the true one only over about a
+    # decade.
+    INNER_TOL = {"fp32": 1e-2, "fp16": 1e-1, "bf16": 1e-1}
+
+    @staticmethod
+    def refine_workspace_bytes(nx, ny, nz) -> int:
+        return int(_lib.load().bs_refine_workspace_bytes(_usize("nx", nx), _usize("ny", ny), _usize("nz", nz)))
+
+    def refine_solve(self, x64_ptr, div_u_ptr, mat_ptr, nx, ny, nz, dx, dt, rho, tol, workspace_ptr,
+                     workspace_bytes, storage="fp32", inner_tol=None, max_outer=40, max_inner=20000,
+                     warm_start=False, x32_ptr=0, mat_fluid=0, mat_air=2, stream=0) -> dict:
+        """FP64 x (``x64_ptr``, nx*ny*nz doubles) solving A x = b to a true
+        relative residual ``tol`` by iterative refinement around a Jacobi-PCG
+        whose vectors are stored as ``storage`` ("fp32", "fp16" or "bf16");
+        the same A and b as ``pressure_solve_pcg``
+        (crates/cuda-kernels/src/pressure_solve_pcg.cu:89-243)."""
+        if storage not in self.STORAGE:
+            raise ValueError(f"storage must be one of {sorted(self.STORAGE)}, got {storage!r}")
+        if inner_tol is None:
+            inner_tol = self.INNER_TOL[storage]
+        st = _lib.RefineStats()
+        _lib.call(
+            "bs_refine_solve",
+            _ptr("x64_ptr", x64_ptr), _ptr("x32_ptr", x32_ptr), _ptr("div_u_ptr", div_u_ptr), _ptr("mat_ptr", mat_ptr),
+            _usize("nx", nx), _usize("ny", ny), _usize("nz", nz),
+            float(dx), float(dt), float(rho), _i32("mat_fluid", mat_fluid), _i32("mat_air", mat_air),
+            float(tol), _usize("max_outer", max_outer), self.STORAGE[storage], float(inner_tol),
+            _usize("max_inner", max_inner), 1 if warm_start else 0,
+            _ptr("workspace_ptr", workspace_ptr), int(workspace_bytes), _ptr("stream", stream), ctypes.byref(st),
+        )
+        return st.as_dict()
+
+    @staticmethod
+    def fcg_refine_workspace_bytes(nx, ny, nz, precond) -> int:
+        return int(_lib.load().bs_fcg_refine_workspace_bytes(_usize("nx", nx), _usize("ny", ny), _usize("nz", nz),
+                                                             int(precond)))
+
+    def fcg_refine_solve(self, x64_ptr, div_u_ptr, mat_ptr, nx, ny, nz, dx, dt, rho, tol, precond,
+                         unet_params_ptr, unet_packed_ptr, workspace_ptr, workspace_bytes, inner_tol=1e-3,
+                         max_outer=20, max_inner=2000, warm_start=False, x32_ptr=0, mat_fluid=0, mat_air=2,
+                         stream=0) -> dict:
+        """FP64 x solving A x = b to a true relative residual ``tol`` by
+        iterative refinement around the learned flexible CG (``fcg_solve``'s
+        preconditioner kinds)."""
+        st = _lib.RefineStats()
+        _lib.call(
+            "bs_fcg_refine_solve",
+            _ptr("x64_ptr", x64_ptr), _ptr("x32_ptr", x32_ptr), _ptr("div_u_ptr", div_u_ptr), _ptr("mat_ptr", mat_ptr),
+            _usize("nx", nx), _usize("ny", ny), _usize("nz", nz),
+            float(dx), float(dt), float(rho), _i32("mat_fluid", mat_fluid), _i32("mat_air", mat_air),
+            float(tol), _usize("max_outer", max_outer), float(inner_tol), _usize("max_inner", max_inner),
+            1 if warm_start else 0, int(precond), _ptr("unet_params_ptr", unet_params_ptr),
+            _ptr("unet_packed_ptr", unet_packed_ptr), _ptr("workspace_ptr", workspace_ptr), int(workspace_bytes),
+            _ptr("stream", stream), ctypes.byref(st),
+        )
+        return st.as_dict()

@@ -201,6 +201,43 @@ int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx, int ny, i
                  float rho, int mat_fluid, int mat_air, double tol, int max_iter, int warm_start,
                  int precond, const float* unet_params, const void* unet_packed, void* workspace,
                  size_t workspace_bytes, bs_stream_t stream, bs_fcg_stats* stats);
+/* ---- Mixed-precision iterative refinement ------------------------------
+ * No single counterpart in the reference: it wraps the Jacobi-PCG of
+ * crates/cuda-kernels/src/pressure_solve_pcg.cu:89-243 (same A, M and b) in
+ * an FP64 outer loop, x_{k+1} = x_k + PCG(b - A x_k), with the inner PCG's
+ * vectors stored in FP32, FP16 or BF16. */
+typedef struct {
+    int outer;              /* refinement steps taken */
+    int inner_iterations;   /* inner PCG iterations, all steps */
+    int converged;          /* FP64 true relative residual <= tol */
+    int storage;            /* 0 FP32, 1 FP16, 2 BF16 */
+    double relres_true;     /* ||b - A x||_2 / ||b||_2 of the FP64 iterate */
+    double bnorm;           /* ||b||_2 */
+    double relres_inner_last; /* recursive relative residual of the last inner solve */
+} bs_refine_stats;
+
+size_t bs_refine_workspace_bytes(int nx, int ny, int nz);
+/* x (FP64, nx*ny*nz) solves A x = b to a true relative residual tol, the
+ * residual formed in FP64; each step runs the inner PCG to a recursive
+ * relative residual inner_tol (at most max_inner iterations).  warm_start uses
+ * x's contents as the initial guess; x32 (nullable) receives x rounded to
+ * FP32. */
+int bs_refine_solve(double* x, float* x32, const float* div_u, const int* mat, int nx, int ny, int nz,
+                    float dx, float dt, float rho, int mat_fluid, int mat_air, double tol, int max_outer,
+                    int storage, double inner_tol, int max_inner, int warm_start, void* workspace,
+                    size_t workspace_bytes, bs_stream_t stream, bs_refine_stats* stats);
+
+size_t bs_fcg_refine_workspace_bytes(int nx, int ny, int nz, int precond);
+/* The same refinement with bs_fcg_solve's flexible CG (preconditioner
+ * `precond`, FP32 vectors) as the inner solver, cold-started on each scaled
+ * residual and run until its recursive residual drops by inner_tol (at most
+ * max_inner iterations per step). */
+int bs_fcg_refine_solve(double* x, float* x32, const float* div_u, const int* mat, int nx, int ny, int nz,
+                        float dx, float dt, float rho, int mat_fluid, int mat_air, double tol, int max_outer,
+                        double inner_tol, int max_inner, int warm_start, int precond, const float* unet_params,
+                        const void* unet_packed, void* workspace, size_t workspace_bytes, bs_stream_t stream,
+                        bs_refine_stats* stats);
+
 /* Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on FP32 params; gscale
  * multiplies the gradient (1/batch). */
 int bs_unet_adam(float* params, const float* grads, float* m, float* v, int step, float lr,
