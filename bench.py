@@ -13,8 +13,11 @@ subtraction -> divergence -> on-device max|div|.  Each step restores the
 input velocities (untimed) so every projection solves the same system.
 
 Also measured live: the roofline of the mask-free Jacobi sweep and of the
-PCG iteration kernels (HIP events on the launch stream), and the CPU baseline
-(the oracle's PCG on a bounded sample on the host cores).
+PCG iteration kernels (HIP events on the launch stream), the first frames of
+the warm-started sequence of correlated smooth right-hand sides (AR(1)
+Gaussian random fields plus a drifting plume, FP64 refinement around the
+learned flexible CG vs around Jacobi-PCG), and the CPU baseline (the
+oracle's PCG on a bounded sample on the host cores).
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -56,6 +59,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-ablation", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=12, help="oracle PCG iterations in the CPU sample")
+    ap.add_argument("--seq-frames", type=int, default=6,
+                    help="frames of the warm-started AR(1) sequence leg (0: skip; the full run is "
+                         "python -m boilsim.sequence --frames 100)")
     return ap.parse_args()
 
 
@@ -164,6 +170,21 @@ def main():
     us_iter = ms_p * 1e3 / max(st_p["iterations"], 1)
     pcg_gbs = B_PCG_ITER * n / (us_iter * 1e-6) / 1e9
 
+    seq = None
+    if args.seq_frames > 0:
+        from boilsim.sequence import run_sequence, summarize
+
+        log(f"AR(1) sequence: {args.seq_frames} frames, learned vs Jacobi-PCG (FP64 refinement, warm starts)")
+        res = run_sequence(core, torch, gmat, args.seq_frames, methods=("learned", "pcg-fp32", "pcg-fp16"),
+                           net=net, tol=args.tol)
+        seq = summarize(res)
+        for m, v in seq.items():
+            log(f"  {m}: median {v['ms_per_solve_median']:.1f} ms/solve, {v['iterations_mean']:.1f} its, "
+                f"max relres {v['max_relres_true']:.2e}, err vs FP64 {v['max_err_vs_fp64']:.2e}")
+        seq["speedup_learned_vs_pcg_fp32"] = (seq["pcg-fp32"]["ms_per_solve_median"] /
+                                              seq["learned"]["ms_per_solve_median"])
+        seq["rhs"] = "AR(1) GRF (step corr 0.95, ell 12 cells, unit variance) + drifting plume; warm starts"
+
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -213,6 +234,7 @@ def main():
         "roofline_pcg_iteration": {"bound": "hbm", "achieved": pcg_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                    "frac": pcg_gbs / HBM_PEAK_GBS, "bytes_per_cell": B_PCG_ITER,
                                    "us_per_iteration": us_iter},
+        "sequence_ar1": seq,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
