@@ -43,6 +43,13 @@ METRIC = "ms/projection to rel-res 1e-6 at 440x440x280: learned-PCG vs plain PCG
 # Algorithmic bytes per cell (FP32 fields, one-byte class code where used).
 B_JACOBI = 12.0   # p_in, div, p_out (class carried in p, no mask read)
 B_PCG_ITER = 47.0  # spmv 4+1+4, update 4+4+(4+4)+(4+4)+1, direction 4+1+(4+4)
+# HBM bytes per cell one mask-free sweep actually moves at 440x440x280,
+# measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (one counter per
+# run) and calibrated by kernels of known traffic in the same runs (a one-
+# field streaming read reads 0.5 of its bytes on FETCH_SIZE; a one-field fill
+# writes exactly its bytes on WRITE_SIZE): 431,321 KiB read + 211,750 KiB
+# written per launch.
+TRAFFIC_JACOBI = 12.15
 
 
 def log(msg: str) -> None:
@@ -59,10 +66,36 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-ablation", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=12, help="oracle PCG iterations in the CPU sample")
+    ap.add_argument("--legs", default="learned,pcg,ablation,jacobi,mixed,sequence,cpu",
+                    help="comma list of the legs to run (the JSON fields of skipped legs are null)")
     ap.add_argument("--seq-frames", type=int, default=6,
                     help="frames of the warm-started AR(1) sequence leg (0: skip; the full run is "
                          "python -m boilsim.sequence --frames 100)")
     return ap.parse_args()
+
+
+def jacobi_roofline(core, sim_core, torch, p, gdiv, ux0, uy0, uz0, gmat, cfg, nx, ny, nz, n):
+    """Average duration of one mask-free sweep over 3 x 200 sweeps of the
+    fused driver, HIP events on the null stream (the launcher's stream)."""
+    from boilsim.fixtures import DT, RHO
+
+    ptr = lambda t: t.data_ptr()
+    core.compute_divergence(ptr(gdiv), ptr(ux0), ptr(uy0), ptr(uz0), ptr(gmat), nx, ny, nz, cfg.dx)
+    p_tmp = torch.zeros_like(p)
+    solve = lambda: sim_core._lib.call("bs_jacobi_solve_launch", ptr(p), ptr(p_tmp), ptr(gdiv), ptr(gmat), nx, ny,
+                                       nz, cfg.dx, DT, RHO, 200, 0, 2, 1, 0)
+    solve()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(3):
+        solve()
+    e1.record()
+    torch.cuda.synchronize()
+    us_sweep = e0.elapsed_time(e1) / 3 / 200 * 1e3
+    jac_gbs = B_JACOBI * n / (us_sweep * 1e-6) / 1e9
+    log(f"jacobi sweep: {us_sweep:.1f} us, {jac_gbs:.0f} GB/s algorithmic ({B_JACOBI:.0f} B/cell)")
+    return us_sweep, jac_gbs
 
 
 def main():
@@ -141,37 +174,58 @@ def main():
     torch.cuda.synchronize()
     maxdiv0 = float(bits.cpu().numpy().view(np.float32)[0])
 
-    ms_l, t_l, st_l, md_l = leg("learned")
-    ms_p, t_p, st_p, md_p = leg("pcg")
+    legs = set(args.legs.split(","))
+    if args.no_cpu:
+        legs.discard("cpu")
+    if args.no_ablation:
+        legs.discard("ablation")
+    if args.seq_frames <= 0:
+        legs.discard("sequence")
+    nan = float("nan")
+    ms_l, t_l, st_l, md_l = leg("learned") if "learned" in legs else (nan, [], None, None)
+    ms_p, t_p, st_p, md_p = leg("pcg") if "pcg" in legs else (nan, [], None, None)
     abl = None
-    if not args.no_ablation:
+    if "ablation" in legs:
         ms_a, _, st_a, _ = leg("mg-jacobi")
         abl = {"ms_per_projection": ms_a, "iterations": st_a["iterations"], "relres_true": st_a["relres_true"]}
 
     # Roofline of the Jacobi sweep: 200 mask-free sweeps timed with HIP events
-    # on the null stream, the stream the launcher uses.
-    gdiv = div.clone()
+    # on the null stream, the stream the launcher uses.  A plain device copy of
+    # one field (n floats) runs first: its FETCH_SIZE / WRITE_SIZE counters
+    # calibrate the traffic measured for the sweep kernel.
+    us_sweep = jac_gbs = nan
+    gdiv = torch.empty_like(p)  # the initial divergence (the solves' RHS)
     core.compute_divergence(ptr(gdiv), ptr(ux0), ptr(uy0), ptr(uz0), ptr(gmat), nx, ny, nz, cfg.dx)
-    p_tmp = torch.zeros_like(p)
-    solve = lambda: sim_core._lib.call("bs_jacobi_solve_launch", ptr(p), ptr(p_tmp), ptr(gdiv), ptr(gmat), nx, ny,
-                                       nz, cfg.dx, DT, RHO, 200, 0, 2, 1, 0)
-    solve()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(3):
-        solve()
-    e1.record()
-    torch.cuda.synchronize()
-    us_sweep = e0.elapsed_time(e1) / 3 / 200 * 1e3
-    jac_gbs = B_JACOBI * n / (us_sweep * 1e-6) / 1e9
-    log(f"jacobi sweep: {us_sweep:.1f} us, {jac_gbs:.0f} GB/s algorithmic ({B_JACOBI:.0f} B/cell)")
-    # PCG iteration rate (three kernels), from the plain solve.
-    us_iter = ms_p * 1e3 / max(st_p["iterations"], 1)
-    pcg_gbs = B_PCG_ITER * n / (us_iter * 1e-6) / 1e9
+    if "jacobi" in legs:
+        cal = torch.empty_like(p)
+        cal.copy_(gdiv)  # n floats read, n floats written
+        torch.cuda.synchronize()
+        us_sweep, jac_gbs = jacobi_roofline(core, sim_core, torch, p, gdiv, ux0, uy0, uz0, gmat, cfg, nx, ny, nz, n)
+    us_iter = ms_p * 1e3 / max(st_p["iterations"], 1) if st_p else nan
+    pcg_gbs = B_PCG_ITER * n / (us_iter * 1e-6) / 1e9 if st_p else nan
+    mixed = None
+    if "mixed" in legs:
+        # Config 3's mixed-precision variant: FP64 refinement around Jacobi-PCG
+        # with FP32 / FP16 / BF16 inner vectors, same RHS, true rel-res tol.
+        ws_r = torch.empty(core.refine_workspace_bytes(nx, ny, nz), dtype=torch.uint8, device="cuda")
+        x64 = torch.zeros(cfg.shape, dtype=torch.float64, device="cuda")
+        mixed = {}
+        for storage in ("fp32", "fp16", "bf16"):
+            for rep in range(2):  # the first call also builds the partial-sum layout caches
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                st = core.refine_solve(ptr(x64), ptr(gdiv), ptr(gmat), nx, ny, nz, cfg.dx, DT, RHO, args.tol,
+                                       ptr(ws_r), ws_r.numel(), storage=storage)
+                torch.cuda.synchronize()
+                ms = (time.perf_counter() - t0) * 1e3
+            mixed[storage] = {"ms_per_solve": ms, "iterations": st["inner_iterations"], "steps": st["outer"],
+                              "relres_true": st["relres_true"], "converged": st["converged"]}
+            log(f"refinement {storage}: {ms:.1f} ms, {st['inner_iterations']} its, {st['outer']} steps, "
+                f"true relres {st['relres_true']:.3e}")
+        del ws_r, x64
 
     seq = None
-    if args.seq_frames > 0:
+    if "sequence" in legs:
         from boilsim.sequence import run_sequence, summarize
 
         log(f"AR(1) sequence: {args.seq_frames} frames, learned vs Jacobi-PCG (FP64 refinement, warm starts)")
@@ -186,7 +240,7 @@ def main():
         seq["rhs"] = "AR(1) GRF (step corr 0.95, ell 12 cells, unit variance) + drifting plume; warm starts"
 
     cpu = None
-    if not args.no_cpu:
+    if "cpu" in legs and st_p:
         import oracle
 
         threads = min(len(os.sched_getaffinity(0)), 16)
@@ -203,14 +257,15 @@ def main():
                          f"{st_p['iterations']}"}
         log(f"cpu baseline: {ms_per_it:.1f} ms/iteration on {threads} threads")
 
+    fin = lambda v: v if v == v else None  # NaN (leg skipped) -> null
     out = {
         "metric": METRIC,
-        "value": ms_l,
+        "value": fin(ms_l),
         "unit": "ms/projection",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_l,
+        "ms_per_step": fin(ms_l),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
@@ -219,21 +274,24 @@ def main():
         "config": {"workload": f"pressure projection, {nx}x{ny}x{nz} pot, true rel-res {args.tol:g}",
                    "grid": [nx, ny, nz], "dx_m": cfg.dx, "rhs": "divergence of N(0,0.01) faces, seed 11",
                    "learned_preconditioner": "FCG + Jacobi V-cycle, U-Net (8/16/32) coarse solve"},
-        "plain_pcg": {"ms_per_projection": ms_p, "iterations": st_p["iterations"],
-                      "relres_true": st_p["relres_true"], "max_div_after": md_p},
-        "learned_pcg": {"ms_per_projection": ms_l, "iterations": st_l["iterations"],
-                        "relres_true": st_l["relres_true"], "fallback": st_l["fallback"],
-                        "max_div_after": md_l, "times_ms": t_l},
-        "speedup_vs_plain_pcg": ms_p / ms_l,
+        "plain_pcg": None if not st_p else {
+            "ms_per_projection": ms_p, "iterations": st_p["iterations"], "relres_true": st_p["relres_true"],
+            "max_div_after": md_p},
+        "learned_pcg": None if not st_l else {
+            "ms_per_projection": ms_l, "iterations": st_l["iterations"], "relres_true": st_l["relres_true"],
+            "fallback": st_l["fallback"], "max_div_after": md_l, "times_ms": t_l},
+        "speedup_vs_plain_pcg": fin(ms_p / ms_l),
         "ablation_mg_jacobi_coarse": abl,
         "max_div_before": maxdiv0,
-        "roofline": {"bound": "hbm", "achieved": jac_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": jac_gbs / HBM_PEAK_GBS, "traffic": None,
+        "roofline": {"bound": "hbm", "achieved": fin(jac_gbs), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": fin(jac_gbs / HBM_PEAK_GBS), "traffic": TRAFFIC_JACOBI * n if cfg.name == "prod" else None,
+                     "traffic_unit": "bytes/launch (PMC, calibrated)",
                      "kernel": "jacobi_sweep_kernel<4,1,1> (mask-free sweep)", "bytes_per_cell": B_JACOBI,
-                     "us_per_launch": us_sweep},
-        "roofline_pcg_iteration": {"bound": "hbm", "achieved": pcg_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                   "frac": pcg_gbs / HBM_PEAK_GBS, "bytes_per_cell": B_PCG_ITER,
-                                   "us_per_iteration": us_iter},
+                     "us_per_launch": fin(us_sweep)},
+        "roofline_pcg_iteration": {"bound": "hbm", "achieved": fin(pcg_gbs), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": fin(pcg_gbs / HBM_PEAK_GBS), "bytes_per_cell": B_PCG_ITER,
+                                   "us_per_iteration": fin(us_iter)},
+        "mixed_precision_refinement": mixed,
         "sequence_ar1": seq,
         "cpu_baseline": cpu,
     }

@@ -259,8 +259,20 @@ __device__ __forceinline__ void reduce1024(const double* __restrict__ part, unsi
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        double acc = 0.0;
-        for (unsigned w = threadIdx.x; w < G; w += 1024) acc += part[(size_t)k * G + w];
+        // Eight loads in flight per thread (a single running sum would wait
+        // for each load in turn); fixed order, so the total is deterministic.
+        const double* pk = part + (size_t)k * G;
+        double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        unsigned w = threadIdx.x;
+        for (; w + 7 * 1024 < G; w += 8 * 1024) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = pk[w + u * 1024];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) a[u] += v[u];
+        }
+        for (; w < G; w += 1024) a[0] += pk[w];
+        double acc = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
         acc = wave_sum64(acc);
         if (lane == 0) red[k][wave] = acc;
     }
