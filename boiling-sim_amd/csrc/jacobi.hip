@@ -192,6 +192,170 @@ __global__ __launch_bounds__(256) void jacobi_sweep_kernel(
     store_vec<VEC>(p_new, c0, out);
 }
 
+// ---------------------------------------------------------------------------
+// Two sweeps per pass (temporal blocking).  A workgroup owns a (j,k) tile of
+// TJ rows x 4*KT cells and walks down a range of i planes; each thread owns one
+// 4-cell chunk of the tile plus its one-cell halo ring ((TJ+2) x (KT+2) chunks
+// = 256 threads).  Per plane q it computes the first sweep at q over the tile
+// and halo, and the second sweep at q-1 over the tile: i-neighbours come from
+// the thread's own registers (planes rolled down i), j/k neighbours from two
+// double-buffered LDS planes (p and first-sweep values), so one barrier per
+// plane suffices.  HBM sees p_in and div once and p_out once for two sweeps.
+// Both sweeps use exactly the single sweep's float operations: the neighbour
+// sum in the same order (a plain sum when no neighbour carries a sentinel — a
+// NaN pattern would make it NaN — the sentinel-aware form otherwise), rhs =
+// (c2*div)/dt, and /6 as fma(x, zh, x*zl), which equals IEEE x/6 for every
+// finite |x| >= 2^-96 (checked over all floats on the CPU and on gfx950); other
+// x take the IEEE divide.  Requires nz % 4 == 0 and 16-byte aligned arrays.
+// ---------------------------------------------------------------------------
+constexpr int kPairKT = 14, kPairTJ = 14;
+
+__device__ __forceinline__ bool is_sent(float v) {
+    const uint32_t b = __float_as_uint(v);
+    return b == kSentAir || b == kSentSolid;
+}
+
+__device__ __forceinline__ float div6(float x) {
+    const float zh = 0x1.555556p-3f, zl = -0x1.555556p-28f;  // RN(1/6), RN(1/6 - zh)
+    const float ax = fabsf(x);
+    if (ax >= 0x1p-96f && ax <= 0x1.fffffep127f) return fmaf(x, zh, x * zl);
+    return x / 6.0f;
+}
+
+__device__ __forceinline__ float sweep_enc(float ps, float L, float R, float D, float U, float B, float F,
+                                           float rhs) {
+    const float s = ((((L + R) + D) + U) + B) + F;
+    if (!__builtin_isnan(s) && !__builtin_isnan(ps)) return div6(s - rhs);
+    if (is_sent(ps)) return ps;
+    float t = contrib_enc(L, ps) + contrib_enc(R, ps);
+    t = t + contrib_enc(D, ps);
+    t = t + contrib_enc(U, ps);
+    t = t + contrib_enc(B, ps);
+    t = t + contrib_enc(F, ps);
+    return div6(t - rhs);
+}
+
+__global__ __launch_bounds__(256) void jacobi_pair_kernel(float* __restrict__ p_new, const float* __restrict__ p_old,
+                                                          const float* __restrict__ div, int nx, int ny, int nz,
+                                                          int ni_seg, int jtiles, int ktiles, float c2, float dt) {
+    constexpr int KT = kPairKT, TJ = kPairTJ;
+    constexpr int CH = KT + 2, SR = TJ + 2, PR = TJ + 4, W = 4 * CH, TK = 4 * KT;
+    static_assert(CH * SR <= 256, "tile and halo must fit one workgroup");
+    // +4 floats before and after each plane: the k-edge reads of the halo
+    // chunks at the row ends land there (their results are never used).
+    __shared__ __align__(16) float sP[2][PR * W + 8];
+    __shared__ __align__(16) float sS[2][SR * W + 8];
+    const int tid = threadIdx.x;
+    const bool act = tid < CH * SR;
+    const int r = tid / CH, c = tid - (tid / CH) * CH;
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const int cols = jtiles * ktiles;
+    const int seg = (int)(b / cols), col = (int)(b % cols);
+    const int jt = col / ktiles, kt = col - (col / ktiles) * ktiles;
+    const int j0 = jt * TJ, k0 = kt * TK;
+    const int i0 = seg * ni_seg, i1 = min(nx, i0 + ni_seg);
+    const int j = j0 - 1 + r, k = k0 - 4 + 4 * c;
+    const bool jk_in = act && j >= 0 && j < ny && k >= 0 && k < nz;
+    // The p plane also needs rows j0-2 and j0+TJ+1: the first and last thread
+    // rows load them.
+    const int jx = (r == 0) ? j0 - 2 : j0 + TJ + 1;
+    const bool xrow = act && (r == 0 || r == SR - 1);
+    const bool jx_in = xrow && jx >= 0 && jx < ny && k >= 0 && k < nz;
+    const bool interior = act && r >= 1 && r <= TJ && c >= 1 && c <= KT && j < ny && k < nz;
+    const int64_t pl = (int64_t)ny * nz;
+    const int64_t off = (int64_t)j * nz + k, offx = (int64_t)jx * nz + k;
+    const float air = __uint_as_float(kSentAir);
+    // Out-of-domain positions read as air (value and class).
+    auto ldp = [=](int q) {
+        float4 v = make_float4(air, air, air, air);
+        if (jk_in && q >= 0 && q < nx) v = *reinterpret_cast<const float4*>(p_old + (int64_t)q * pl + off);
+        return v;
+    };
+    auto ldx = [=](int q) {
+        float4 v = make_float4(air, air, air, air);
+        if (jx_in && q >= 0 && q < nx) v = *reinterpret_cast<const float4*>(p_old + (int64_t)q * pl + offx);
+        return v;
+    };
+    auto ldrhs = [=](int q) {
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (jk_in && q >= 0 && q < nx) {
+            v = *reinterpret_cast<const float4*>(div + (int64_t)q * pl + off);
+            v = make_float4((c2 * v.x) / dt, (c2 * v.y) / dt, (c2 * v.z) / dt, (c2 * v.w) / dt);
+        }
+        return v;
+    };
+    float4 Pm = ldp(i0 - 2), Pc = ldp(i0 - 1), Pn = ldp(i0), Xr = ldx(i0 - 1);
+    float4 Hc = ldrhs(i0 - 1), Hm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 AIR4 = make_float4(air, air, air, air);
+    float4 S1m = AIR4, S1c = AIR4;
+    for (int q = i0 - 1; q <= i1; ++q) {
+        float* P = sP[q & 1] + 4;
+        float* S = sS[(q - 1) & 1] + 4;
+        if (act) {
+            *reinterpret_cast<float4*>(P + (r + 1) * W + 4 * c) = Pc;
+            if (xrow) *reinterpret_cast<float4*>(P + (r == 0 ? 0 : PR - 1) * W + 4 * c) = Xr;
+            *reinterpret_cast<float4*>(S + r * W + 4 * c) = S1c;
+        }
+        // Next plane's loads are in flight during this plane's arithmetic.
+        const float4 Pf = ldp(q + 2), Xf = ldx(q + 1);
+        const float4 Hf = ldrhs(q + 1);
+        __syncthreads();
+        // First sweep at plane q (tile and halo).
+        float4 S1n = AIR4;
+        if (jk_in && q >= 0 && q < nx) {
+            const float4 d4 = *reinterpret_cast<const float4*>(P + r * W + 4 * c);
+            const float4 u4 = *reinterpret_cast<const float4*>(P + (r + 2) * W + 4 * c);
+            const float bk = P[(r + 1) * W + 4 * c - 1], fk = P[(r + 1) * W + 4 * c + 4];
+            const float cc[4] = {Pc.x, Pc.y, Pc.z, Pc.w}, ll[4] = {Pm.x, Pm.y, Pm.z, Pm.w};
+            const float rr[4] = {Pn.x, Pn.y, Pn.z, Pn.w}, dn[4] = {d4.x, d4.y, d4.z, d4.w};
+            const float up[4] = {u4.x, u4.y, u4.z, u4.w}, hh[4] = {Hc.x, Hc.y, Hc.z, Hc.w};
+            float o[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                o[e] = sweep_enc(cc[e], ll[e], rr[e], dn[e], up[e], e > 0 ? cc[e - 1] : bk, e < 3 ? cc[e + 1] : fk,
+                                 hh[e]);
+            S1n = make_float4(o[0], o[1], o[2], o[3]);
+        }
+        // Second sweep at plane q-1 (tile only).
+        if (q - 1 >= i0 && interior) {
+            const float4 d4 = *reinterpret_cast<const float4*>(S + (r - 1) * W + 4 * c);
+            const float4 u4 = *reinterpret_cast<const float4*>(S + (r + 1) * W + 4 * c);
+            const float bk = S[r * W + 4 * c - 1], fk = S[r * W + 4 * c + 4];
+            const float cc[4] = {S1c.x, S1c.y, S1c.z, S1c.w}, ll[4] = {S1m.x, S1m.y, S1m.z, S1m.w};
+            const float rr[4] = {S1n.x, S1n.y, S1n.z, S1n.w}, dn[4] = {d4.x, d4.y, d4.z, d4.w};
+            const float up[4] = {u4.x, u4.y, u4.z, u4.w}, hh[4] = {Hm.x, Hm.y, Hm.z, Hm.w};
+            float o[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                o[e] = sweep_enc(cc[e], ll[e], rr[e], dn[e], up[e], e > 0 ? cc[e - 1] : bk, e < 3 ? cc[e + 1] : fk,
+                                 hh[e]);
+            *reinterpret_cast<float4*>(p_new + (int64_t)(q - 1) * pl + off) = make_float4(o[0], o[1], o[2], o[3]);
+        }
+        Pm = Pc; Pc = Pn; Pn = Pf; Xr = Xf;
+        S1m = S1c; S1c = S1n;
+        Hm = Hc; Hc = Hf;
+    }
+}
+
+struct PairShape {
+    int jtiles, ktiles, ni;
+    unsigned grid;
+};
+
+// About 4096 workgroups (16 per CU): tiles over (j,k), i split into segments.
+static PairShape pair_shape(int nx, int ny, int nz) {
+    PairShape s;
+    s.jtiles = (ny + kPairTJ - 1) / kPairTJ;
+    s.ktiles = (nz + 4 * kPairKT - 1) / (4 * kPairKT);
+    const int cols = s.jtiles * s.ktiles;
+    int nseg = (4096 + cols / 2) / cols;
+    nseg = nseg < 1 ? 1 : (nseg > nx ? nx : nseg);
+    s.ni = (nx + nseg - 1) / nseg;
+    nseg = (nx + s.ni - 1) / s.ni;
+    s.grid = (unsigned)(nseg * cols);
+    return s;
+}
+
 // Sentinel -> 0 in place (restores the reference's zero at non-fluid cells).
 template <int VEC>
 __global__ __launch_bounds__(256) void decode_kernel(float* __restrict__ p, int64_t nvec) {
@@ -315,25 +479,48 @@ extern "C" int bs_jacobi_solve_launch(float* p, float* p_tmp, const float* div_u
             launch_sweep_t<kInMask, kOutDecoded>(sh, st, p_tmp, p, div_u, mat, nx, ny, nz, c2, dt,
                                                  mat_fluid, mat_air);
     } else {
-        // Iteration `it` reads p when even and p_tmp when odd (fluid.py:688-700).
+        // The reference's ping-pong (fluid.py:688-700): sweep `it` reads p when
+        // even and p_tmp when odd.  Here `cur` holds the latest result.
+        float* cur = p;
+        float* nxt = p_tmp;
+        auto flip = [&] { float* t = cur; cur = nxt; nxt = t; };
         if (zero_init)
-            launch_sweep_t<kInZero, kOutEncoded>(sh, st, p_tmp, p, div_u, mat, nx, ny, nz, c2, dt,
+            launch_sweep_t<kInZero, kOutEncoded>(sh, st, nxt, cur, div_u, mat, nx, ny, nz, c2, dt,
                                                  mat_fluid, mat_air);
         else
-            launch_sweep_t<kInMask, kOutEncoded>(sh, st, p_tmp, p, div_u, mat, nx, ny, nz, c2, dt,
+            launch_sweep_t<kInMask, kOutEncoded>(sh, st, nxt, cur, div_u, mat, nx, ny, nz, c2, dt,
                                                  mat_fluid, mat_air);
-        for (int it = 1; it < n_iter - 1; ++it) {
-            float* dst = (it & 1) ? p : p_tmp;
-            const float* src = (it & 1) ? p_tmp : p;
-            launch_sweep_t<kInEncoded, kOutEncoded>(sh, st, dst, src, div_u, mat, nx, ny, nz, c2,
-                                                    dt, mat_fluid, mat_air);
+        flip();
+        // Sweeps 1 .. n-3 in pairs where the pair kernel applies.  A pair flips
+        // the buffers once where the reference flips twice, so the number of
+        // pairs is even and the buffer holding each later sweep's result is
+        // the reference's.  The last two sweeps always run singly, so both
+        // buffers end exactly as the reference leaves them.
+        const int middle = n_iter >= 3 ? n_iter - 3 : 0;
+        const int pairs = vec == 4 ? (middle / 2) & ~1 : 0;
+        if (pairs > 0) {
+            const PairShape ps = pair_shape(nx, ny, nz);
+            for (int q = 0; q < pairs; ++q) {
+                jacobi_pair_kernel<<<ps.grid, 256, 0, st>>>(nxt, cur, div_u, nx, ny, nz, ps.ni, ps.jtiles,
+                                                            ps.ktiles, c2, dt);
+                flip();
+            }
         }
-        const int it = n_iter - 1;
-        float* dst = (it & 1) ? p : p_tmp;
-        const float* src = (it & 1) ? p_tmp : p;
-        launch_sweep_t<kInEncoded, kOutDecoded>(sh, st, dst, src, div_u, mat, nx, ny, nz, c2, dt,
+        for (int q = 0; q < middle - 2 * pairs; ++q) {
+            launch_sweep_t<kInEncoded, kOutEncoded>(sh, st, nxt, cur, div_u, mat, nx, ny, nz, c2, dt,
+                                                    mat_fluid, mat_air);
+            flip();
+        }
+        if (n_iter >= 3) {
+            launch_sweep_t<kInEncoded, kOutEncoded>(sh, st, nxt, cur, div_u, mat, nx, ny, nz, c2, dt,
+                                                    mat_fluid, mat_air);
+            flip();
+        }
+        launch_sweep_t<kInEncoded, kOutDecoded>(sh, st, nxt, cur, div_u, mat, nx, ny, nz, c2, dt,
                                                 mat_fluid, mat_air);
-        if ((n_iter & 1) == 0) launch_decode(st, p_tmp, n, vec);
+        // nxt holds the last result; cur the previous sweep's, encoded.  With
+        // an odd count cur is p, which the copy below overwrites.
+        if ((n_iter & 1) == 0) launch_decode(st, cur, n, vec);
     }
     BS_RETURN_IF_ERR(hipGetLastError());
     if (n_iter & 1) {

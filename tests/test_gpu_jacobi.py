@@ -134,3 +134,30 @@ def test_200_sweeps_pot(simcore, torch_gpu, name):
     err = np.linalg.norm((got - x64)[fluid]) / np.linalg.norm(x64[fluid])
     print(f"[{name}] 200 Jacobi sweeps: rel L2 error vs FP64 solution = {err:.4e} "
           f"(200 sweeps are far from converged; this is the method's error, equal for both)")
+
+
+@pytest.mark.parametrize("shape", [(97, 83, 68), (40, 30, 56), (33, 29, 116), (9, 7, 4), (64, 440, 280)])
+@pytest.mark.parametrize("n_iter", [5, 9, 30])
+@pytest.mark.parametrize("edge", [False, True])
+def test_two_sweep_pass_bit_exact(simcore, torch_gpu, shape, n_iter, edge):
+    """The fused driver runs its middle sweeps two per pass (nz % 4 == 0):
+    p and p_tmp still end bit-identical to the reference's one-sweep ping-pong,
+    on grids with partial (j, k) tiles, several i segments, and fluid on the
+    domain edges (out-of-bounds neighbours are air)."""
+    torch = torch_gpu
+    if edge:
+        mat, p0, div_u = fluid_everywhere_edge_case(5, shape)
+        dx, dt, rho = 0.002, 0.001, 997.0
+    else:
+        mat, p0, div_u, dx, dt, rho = parity_case(23, shape)
+    p_ref = p0.copy()
+    tmp_ref = np.full(shape, -1.5, np.float32)
+    oracle.pressure_solve(p_ref, tmp_ref, div_u, mat, dx, dt, rho, n_iter)
+    p = dev(torch, p0)
+    tmp = dev(torch, np.full(shape, -1.5, np.float32))
+    div_d, mat_d = dev(torch, div_u), dev(torch, mat)
+    simcore.pressure_solve(p.data_ptr(), tmp.data_ptr(), div_d.data_ptr(), mat_d.data_ptr(),
+                           *shape, dx, dt, rho, n_iter)
+    torch.cuda.synchronize()
+    assert same_bits(host(p), p_ref), bit_diff_report(host(p), p_ref)
+    assert same_bits(host(tmp), tmp_ref), bit_diff_report(host(tmp), tmp_ref)
