@@ -43,13 +43,13 @@ METRIC = "ms/projection to rel-res 1e-6 at 440x440x280: learned-PCG vs plain PCG
 # Algorithmic bytes per cell (FP32 fields, one-byte class code where used).
 B_JACOBI = 12.0   # p_in, div, p_out (class carried in p, no mask read)
 B_PCG_ITER = 47.0  # spmv 4+1+4, update 4+4+(4+4)+(4+4)+1, direction 4+1+(4+4)
-# HBM bytes per cell one mask-free sweep actually moves at 440x440x280,
-# measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (one counter per
-# run) and calibrated by kernels of known traffic in the same runs (a one-
-# field streaming read reads 0.5 of its bytes on FETCH_SIZE; a one-field fill
-# writes exactly its bytes on WRITE_SIZE): 431,321 KiB read + 211,750 KiB
-# written per launch.
-TRAFFIC_JACOBI = 12.15
+# HBM bytes per cell one two-sweep launch of jacobi_pair_kernel actually
+# moves at 440x440x280, measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (one counter per run) and calibrated by kernels of known traffic
+# in the same runs (a one-field streaming read counts 0.5 of its bytes on
+# FETCH_SIZE; a one-field fill writes exactly its bytes on WRITE_SIZE).
+# 266,018.7 KiB x 1.99976 read + 220,094.4 KiB written per launch.
+TRAFFIC_JACOBI_PAIR = 14.207
 
 
 def log(msg: str) -> None:
@@ -75,27 +75,38 @@ def parse():
 
 
 def jacobi_roofline(core, sim_core, torch, p, gdiv, ux0, uy0, uz0, gmat, cfg, nx, ny, nz, n):
-    """Average duration of one mask-free sweep over 3 x 200 sweeps of the
-    fused driver, HIP events on the null stream (the launcher's stream)."""
+    """HIP-event timings on the null stream (the launcher's stream) of the
+    fused Jacobi driver: 200 sweeps from p = 0 run the first sweep (mask read),
+    98 two-sweep passes of `jacobi_pair_kernel`, three single mask-free sweeps
+    and a decode; 4 sweeps run the same single sweeps without the pairs, so
+    (T200 - T4) / 98 is the average pair launch.  Returns (us per pair launch,
+    us per sweep over the whole 200-sweep driver)."""
     from boilsim.fixtures import DT, RHO
 
     ptr = lambda t: t.data_ptr()
     core.compute_divergence(ptr(gdiv), ptr(ux0), ptr(uy0), ptr(uz0), ptr(gmat), nx, ny, nz, cfg.dx)
     p_tmp = torch.zeros_like(p)
-    solve = lambda: sim_core._lib.call("bs_jacobi_solve_launch", ptr(p), ptr(p_tmp), ptr(gdiv), ptr(gmat), nx, ny,
-                                       nz, cfg.dx, DT, RHO, 200, 0, 2, 1, 0)
-    solve()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(3):
-        solve()
-    e1.record()
-    torch.cuda.synchronize()
-    us_sweep = e0.elapsed_time(e1) / 3 / 200 * 1e3
-    jac_gbs = B_JACOBI * n / (us_sweep * 1e-6) / 1e9
-    log(f"jacobi sweep: {us_sweep:.1f} us, {jac_gbs:.0f} GB/s algorithmic ({B_JACOBI:.0f} B/cell)")
-    return us_sweep, jac_gbs
+    solve = lambda k: sim_core._lib.call("bs_jacobi_solve_launch", ptr(p), ptr(p_tmp), ptr(gdiv), ptr(gmat), nx, ny,
+                                         nz, cfg.dx, DT, RHO, k, 0, 2, 1, 0)
+
+    def timed(k, reps):
+        solve(k)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            solve(k)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e3  # us per call
+
+    t200 = timed(200, 3)
+    t4 = timed(4, 10)
+    us_pair = (t200 - t4) / 98
+    us_sweep = t200 / 200
+    log(f"jacobi: {us_pair:.1f} us per two-sweep launch ({2 * B_JACOBI * n / (us_pair * 1e-6) / 1e9:.0f} GB/s "
+        f"algorithmic), {us_sweep:.1f} us per sweep over the 200-sweep driver")
+    return us_pair, us_sweep
 
 
 def main():
@@ -193,14 +204,15 @@ def main():
     # on the null stream, the stream the launcher uses.  A plain device copy of
     # one field (n floats) runs first: its FETCH_SIZE / WRITE_SIZE counters
     # calibrate the traffic measured for the sweep kernel.
-    us_sweep = jac_gbs = nan
+    us_sweep = us_pair = jac_gbs = nan
     gdiv = torch.empty_like(p)  # the initial divergence (the solves' RHS)
     core.compute_divergence(ptr(gdiv), ptr(ux0), ptr(uy0), ptr(uz0), ptr(gmat), nx, ny, nz, cfg.dx)
     if "jacobi" in legs:
         cal = torch.empty_like(p)
         cal.copy_(gdiv)  # n floats read, n floats written
         torch.cuda.synchronize()
-        us_sweep, jac_gbs = jacobi_roofline(core, sim_core, torch, p, gdiv, ux0, uy0, uz0, gmat, cfg, nx, ny, nz, n)
+        us_pair, us_sweep = jacobi_roofline(core, sim_core, torch, p, gdiv, ux0, uy0, uz0, gmat, cfg, nx, ny, nz, n)
+        jac_gbs = 2 * B_JACOBI * n / (us_pair * 1e-6) / 1e9
     us_iter = ms_p * 1e3 / max(st_p["iterations"], 1) if st_p else nan
     pcg_gbs = B_PCG_ITER * n / (us_iter * 1e-6) / 1e9 if st_p else nan
     mixed = None
@@ -284,10 +296,12 @@ def main():
         "ablation_mg_jacobi_coarse": abl,
         "max_div_before": maxdiv0,
         "roofline": {"bound": "hbm", "achieved": fin(jac_gbs), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": fin(jac_gbs / HBM_PEAK_GBS), "traffic": TRAFFIC_JACOBI * n if cfg.name == "prod" else None,
+                     "frac": fin(jac_gbs / HBM_PEAK_GBS),
+                     "traffic": TRAFFIC_JACOBI_PAIR * n if (cfg.name == "prod" and TRAFFIC_JACOBI_PAIR) else None,
                      "traffic_unit": "bytes/launch (PMC, calibrated)",
-                     "kernel": "jacobi_sweep_kernel<4,1,1> (mask-free sweep)", "bytes_per_cell": B_JACOBI,
-                     "us_per_launch": fin(us_sweep)},
+                     "kernel": "jacobi_pair_kernel (two mask-free sweeps per launch)",
+                     "bytes_per_cell": 2 * B_JACOBI, "us_per_launch": fin(us_pair),
+                     "us_per_sweep_200_sweep_driver": fin(us_sweep)},
         "roofline_pcg_iteration": {"bound": "hbm", "achieved": fin(pcg_gbs), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                    "frac": fin(pcg_gbs / HBM_PEAK_GBS), "bytes_per_cell": B_PCG_ITER,
                                    "us_per_iteration": fin(us_iter)},
