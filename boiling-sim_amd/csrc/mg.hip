@@ -373,6 +373,65 @@ __global__ __launch_bounds__(256) void restrict2_kernel(Tiles F, Tiles C, const 
     rc[cc] = 0.5f * s;
 }
 
+// rc = 0.5 * sum over children of (r - A z) for two consecutive coarse cells
+// per thread (fine nz % 4 == 0): the thread's 2x2 fine columns of 4 cells and
+// the 8 columns around them (a 4x4 patch without corners) are loaded once as
+// float4 values and 4-byte code words and shared by the four residuals.
+__global__ __launch_bounds__(256) void restrict4_kernel(Tiles F, Tiles C, const uint8_t* __restrict__ fcode,
+                                                        const float* __restrict__ r, const float* __restrict__ z,
+                                                        float* __restrict__ rc, const int* __restrict__ done) {
+    if (done && *done) return;
+    int I, J, K0;
+    int64_t cc0;
+    if (!one_tile(C, 2, I, J, K0, cc0)) return;
+    const int i0 = 2 * I, j0 = 2 * J, k0 = 2 * K0;
+    const int64_t plane = (int64_t)F.ny * F.nz;
+    float zv[4][4][4];
+    uint8_t cv[4][4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            if ((a == 0 || a == 3) && (b == 0 || b == 3)) continue;  // corners are not neighbours
+            const int i = i0 + a - 1, j = j0 + b - 1;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) { zv[a][b][e] = 0.0f; cv[a][b][e] = kCodeAir; }
+            if (i >= 0 && i < F.nx && j >= 0 && j < F.ny) {
+                const int64_t c = (int64_t)i * plane + (int64_t)j * F.nz + k0;
+                ldf<4>(z, c, zv[a][b]);
+                ldc<4>(fcode, c, cv[a][b]);
+            }
+        }
+    float acc[2] = {0.0f, 0.0f};
+#pragma unroll
+    for (int a = 1; a < 3; ++a)
+#pragma unroll
+        for (int b = 1; b < 3; ++b) {
+            const int i = i0 + a - 1, j = j0 + b - 1;
+            if (i >= F.nx || j >= F.ny) continue;
+            const int64_t c = (int64_t)i * plane + (int64_t)j * F.nz + k0;
+            float rv[4];
+            ldf<4>(r, c, rv);
+            float zb = 0.0f, zf = 0.0f;
+            uint8_t cb = kCodeAir, cf = kCodeAir;
+            if (k0 > 0) { cb = fcode[c - 1]; zb = z[c - 1]; }
+            if (k0 + 4 < F.nz) { cf = fcode[c + 4]; zf = z[c + 4]; }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (cv[a][b][e] < kCodeFluid0) continue;
+                const float ps = zv[a][b][e];
+                float s = nbr_term(cv[a - 1][b][e], zv[a - 1][b][e], ps) + nbr_term(cv[a + 1][b][e], zv[a + 1][b][e], ps);
+                s += nbr_term(cv[a][b - 1][e], zv[a][b - 1][e], ps);
+                s += nbr_term(cv[a][b + 1][e], zv[a][b + 1][e], ps);
+                s += e > 0 ? nbr_term(cv[a][b][e - 1], zv[a][b][e - 1], ps) : nbr_term(cb, zb, ps);
+                s += e < 3 ? nbr_term(cv[a][b][e + 1], zv[a][b][e + 1], ps) : nbr_term(cf, zf, ps);
+                acc[e >> 1] += rv[e] - (6.0f * ps - s);
+            }
+        }
+    const float out[2] = {0.5f * acc[0], 0.5f * acc[1]};
+    stf<2>(rc, cc0, out);
+}
+
 // z += trilinear(ec) on fluid cells, VEC fine cells per thread.
 template <int VEC>
 __global__ __launch_bounds__(256) void prolong_vec_kernel(Tiles F, Tiles C, const uint8_t* __restrict__ fcode,
@@ -423,6 +482,80 @@ __global__ __launch_bounds__(256) void prolong_vec_kernel(Tiles F, Tiles C, cons
     }
     stf<VEC>(z, c0, zv);
 }
+
+// z += trilinear(ec) on fluid cells for the 2x2 fine columns of one coarse
+// column, 4 fine k cells each (fine nz % 4 == 0): the 3x3 coarse rows
+// (clamped at the edges) over the 4-value k window are loaded once and
+// shared; per fine cell the arithmetic is that of prolong_vec_kernel.
+__global__ __launch_bounds__(256) void prolong4_kernel(Tiles F, Tiles CV, const uint8_t* __restrict__ fcode,
+                                                       const float* __restrict__ ec, float* __restrict__ z,
+                                                       const int* __restrict__ done) {
+    if (done && *done) return;
+    // CV tiles the (coarse nx, coarse ny, fine nz) index space, 4 k per thread.
+    int I, J, k0;
+    int64_t unused;
+    if (!one_tile(CV, 4, I, J, k0, unused)) return;
+    (void)unused;
+    const int cnx = CV.nx, cny = CV.ny, cnz = (F.nz + 1) / 2;
+    const int64_t cp = (int64_t)cny * cnz;
+    const int kb = (k0 >> 1) - 1;
+    float E[3][3][4];
+#pragma unroll
+    for (int da = 0; da < 3; ++da)
+#pragma unroll
+        for (int db = 0; db < 3; ++db) {
+            int Ir = I + da - 1, Jr = J + db - 1;
+            Ir = Ir < 0 ? 0 : (Ir >= cnx ? cnx - 1 : Ir);
+            Jr = Jr < 0 ? 0 : (Jr >= cny ? cny - 1 : Jr);
+            const int64_t row = (int64_t)Ir * cp + (int64_t)Jr * cnz;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                int kk = kb + w;
+                kk = kk < 0 ? 0 : (kk >= cnz ? cnz - 1 : kk);
+                E[da][db][w] = ec[row + kk];
+            }
+        }
+    const int64_t plane = (int64_t)F.ny * F.nz;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int i = 2 * I + a, j = 2 * J + b;
+            if (i >= F.nx || j >= F.ny) continue;
+            const int64_t c0 = (int64_t)i * plane + (int64_t)j * F.nz + k0;
+            uint8_t cc[4];
+            float zv[4];
+            ldc<4>(fcode, c0, cc);
+            ldf<4>(z, c0, zv);
+            // rows (I,J), (I,J2), (I2,J), (I2,J2) with I2 = I -/+ 1 for even/odd i
+            const int a2 = a ? 2 : 0, b2 = b ? 2 : 0;
+            float cw[4];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                float acc = 0.0f;
+                acc += 0.5625f * E[1][1][w];
+                acc += 0.1875f * E[1][b2][w];
+                acc += 0.1875f * E[a2][1][w];
+                acc += 0.0625f * E[a2][b2][w];
+                cw[w] = acc;
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (cc[e] < kCodeFluid0) continue;
+                const int k = k0 + e;
+                const int K = k >> 1;
+                int K2 = K + ((k & 1) ? 1 : -1);
+                K2 = K2 < 0 ? 0 : (K2 >= cnz ? cnz - 1 : K2);
+                const int sK = K - kb, sK2 = (K2 - kb) < 0 ? 0 : ((K2 - kb) >= 4 ? 3 : (K2 - kb));
+                zv[e] += 0.75f * cw[sK] + 0.25f * cw[sK2];
+            }
+            stf<4>(z, c0, zv);
+        }
+}
+
+// z += P ec from level LC to its refinement LF (tiles TF, vector width vf).
+static void prolong_level(hipStream_t st, const Tiles& TF, int vf, const MLvl& LF, const MLvl& LC,
+                          const Tiles& TC, const uint8_t* fcode, const float* ec, float* z, const int* done);
 
 static inline size_t al(size_t v) { return (v + 255) & ~(size_t)255; }
 
@@ -525,6 +658,32 @@ unsigned mg_dots_groups(int nx, int ny, int nz) { return make_tiles(nx, ny, nz, 
         else KERNEL<1><<<(GRID), 256, 0, (ST)>>>(__VA_ARGS__);                     \
     } while (0)
 
+static void prolong_level(hipStream_t st, const Tiles& TF, int vf, const MLvl& LF, const MLvl& LC,
+                          const Tiles& TC, const uint8_t* fcode, const float* ec, float* z, const int* done) {
+    if (vf == 4) {
+        const Tiles CV = make_tiles(LC.nx, LC.ny, LF.nz, 4);
+        prolong4_kernel<<<CV.ntiles, 256, 0, st>>>(TF, CV, fcode, ec, z, done);
+    } else {
+        MG_VEC(prolong_vec_kernel, vf, TF.ntiles, st, TF, TC, fcode, ec, z, done);
+    }
+}
+
+// Restriction of the residual r - A z from level F to its coarsening: the
+// 2x2x4-block kernel when the fine vectors allow 4-wide access, the
+// coarse-centred pair kernel for even nz, the one-cell kernel otherwise.
+static void restrict_level(hipStream_t st, const Tiles& TF, int vf, const MLvl& LF, const MLvl& LC,
+                           const Tiles& TC1, const uint8_t* fcode, const float* r, const float* z, float* rc,
+                           const int* done) {
+    if (vf == 4 && (LC.nz & 1) == 0) {
+        const Tiles TC2 = make_tiles(LC.nx, LC.ny, LC.nz, 2);
+        restrict4_kernel<<<TC2.ntiles, 256, 0, st>>>(TF, TC2, fcode, r, z, rc, done);
+    } else if ((LF.nz & 1) == 0) {
+        restrict2_kernel<<<TC1.ntiles, 256, 0, st>>>(TF, TC1, fcode, r, z, rc, done);
+    } else {
+        restrict_kernel<<<grid_for(LC.n), 256, 0, st>>>(LF, LC, fcode, r, z, rc, done);
+    }
+}
+
 int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* code0, const float* r, float* z,
              const float* z_old, double* dots_part, int coarse_kind, const float* params, const void* packed,
              double n_fluid_coarse, const int* done) {
@@ -535,19 +694,13 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
     const Tiles T1 = make_tiles(W.L1.nx, W.L1.ny, W.L1.nz, v1);
     const Tiles C1 = make_tiles(W.L1.nx, W.L1.ny, W.L1.nz, 1);
     const Tiles C2 = make_tiles(W.L2.nx, W.L2.ny, W.L2.nz, 1);
-    const unsigned g1 = grid_for(W.L1.n), g2 = grid_for(W.L2.n);
+    const unsigned g2 = grid_for(W.L2.n);
     // fine: two pre-smoothing sweeps from zero, restriction of the residual
     MG_VEC(pre2_kernel, v0, T0.ntiles, st, T0, code0, r, z, done);
-    if ((W.L0.nz & 1) == 0)
-        restrict2_kernel<<<C1.ntiles, 256, 0, st>>>(T0, C1, code0, r, z, W.r1, done);
-    else
-        restrict_kernel<<<g1, 256, 0, st>>>(W.L0, W.L1, code0, r, z, W.r1, done);
+    restrict_level(st, T0, v0, W.L0, W.L1, C1, code0, r, z, W.r1, done);
     // level 1
     MG_VEC(pre2_kernel, v1, T1.ntiles, st, T1, W.code1, W.r1, W.z1, done);
-    if ((W.L1.nz & 1) == 0)
-        restrict2_kernel<<<C2.ntiles, 256, 0, st>>>(T1, C2, W.code1, W.r1, W.z1, W.r2, done);
-    else
-        restrict_kernel<<<g2, 256, 0, st>>>(W.L1, W.L2, W.code1, W.r1, W.z1, W.r2, done);
+    restrict_level(st, T1, v1, W.L1, W.L2, C2, W.code1, W.r1, W.z1, W.r2, done);
     // level 2: coarse solve.  coarse_kind 0: 11 damped-Jacobi sweeps;
     // 1: the network (A-norm-optimal step), then 10 sweeps polishing its
     // correction; 2: 5 sweeps, the network on their residual, then 4 sweeps.
@@ -579,11 +732,11 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
         sweeps(2);
     }
     // back up: level 1
-    MG_VEC(prolong_vec_kernel, v1, T1.ntiles, st, T1, C2, W.code1, W.e2, W.z1, done);
+    prolong_level(st, T1, v1, W.L1, W.L2, C2, W.code1, W.e2, W.z1, done);
     MG_VEC(smooth_kernel, v1, T1.ntiles, st, T1, W.code1, W.r1, W.z1, W.zt1, done);
     MG_VEC(smooth_kernel, v1, T1.ntiles, st, T1, W.code1, W.r1, W.zt1, W.z1, done);
     // fine: prolongation, one sweep, and the last sweep fused with r.z, r.z_old
-    MG_VEC(prolong_vec_kernel, v0, T0.ntiles, st, T0, C1, code0, W.z1, z, done);
+    prolong_level(st, T0, v0, W.L0, W.L1, C1, code0, W.z1, z, done);
     MG_VEC(smooth_kernel, v0, T0.ntiles, st, T0, code0, r, z, W.zt0, done);
     if (T0.ntiles != mg_dots_groups(nx, ny, nz)) return (int)hipErrorInvalidValue;  // misaligned caller buffers
     MG_VEC(smooth_dots_kernel, v0, T0.ntiles, st, T0, code0, r, W.zt0, z, z_old, dots_part, done);
