@@ -66,7 +66,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-ablation", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=12, help="oracle PCG iterations in the CPU sample")
-    ap.add_argument("--legs", default="learned,pcg,ablation,jacobi,mixed,sequence,cpu",
+    ap.add_argument("--legs", default="learned,pcg,ablation,jacobi,mixed,thermal,sequence,cpu",
                     help="comma list of the legs to run (the JSON fields of skipped legs are null)")
     ap.add_argument("--seq-frames", type=int, default=6,
                     help="frames of the warm-started AR(1) sequence leg (0: skip; the full run is "
@@ -107,6 +107,49 @@ def jacobi_roofline(core, sim_core, torch, p, gdiv, ux0, uy0, uz0, gmat, cfg, nx
     log(f"jacobi: {us_pair:.1f} us per two-sweep launch ({2 * B_JACOBI * n / (us_pair * 1e-6) / 1e9:.0f} GB/s "
         f"algorithmic), {us_sweep:.1f} us per sweep over the 200-sweep driver")
     return us_pair, us_sweep
+
+
+def conduction_leg(core, torch, gmat, mat, cfg, nx, ny, nz, n):
+    """Backward-Euler Jacobi conduction (thermal.py:259-388, 631-665) on the
+    pot: one step of the reference's default 15 sweeps (dt 0.05 s, hot pot
+    base), and the sweep kernel alone, 20 launches timed with HIP events on
+    the null stream.  Algorithmic bytes per sweep: T_iter, T_old, mat read,
+    T_dst written = 16 B/cell."""
+    from boilsim.thermal import MaterialProps, allocate_thermal_workspace, conduct_implicit
+
+    props = MaterialProps.from_pot("steel_304")
+    k, rho, cp = props.device(torch)
+    t0 = np.full(cfg.shape, 295.15, np.float32)
+    base = (mat == 1) & (np.arange(nz)[None, None, :] < nz // 6)
+    t0[base] = 500.0
+    T = torch.from_numpy(t0).cuda()
+    ws = allocate_thermal_workspace(torch, cfg.shape)
+    dt = 0.05
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    conduct_implicit(core, T, gmat, (k, rho, cp), ws, cfg.dx, dt, 15)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(3):
+        conduct_implicit(core, T, gmat, (k, rho, cp), ws, cfg.dx, dt, 15)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_step = e0.elapsed_time(e1) / 3
+    src, dst = T, ws.T_iter_alt
+    ptr = lambda t: t.data_ptr()
+    sweep = lambda: core.conduction_sweep(ptr(dst), ptr(src), ptr(ws.T_old_snapshot), ptr(gmat), ptr(k), ptr(rho),
+                                          ptr(cp), k.numel(), nx, ny, nz, cfg.dx, dt, 10.0, 295.15)
+    sweep()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(20):
+        sweep()
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) / 20 * 1e3
+    gbs = 16.0 * n / (us * 1e-6) / 1e9
+    log(f"conduction: {us:.1f} us/sweep ({gbs:.0f} GB/s algorithmic, 16 B/cell), {ms_step:.2f} ms per 15-sweep step")
+    return {"us_per_sweep": us, "achieved_gbs": gbs, "frac_of_peak": gbs / HBM_PEAK_GBS, "bytes_per_cell": 16.0,
+            "ms_per_step_15_sweeps": ms_step, "kernel": "conduction_sweep_kernel<4>"}
 
 
 def main():
@@ -215,6 +258,10 @@ def main():
         jac_gbs = 2 * B_JACOBI * n / (us_pair * 1e-6) / 1e9
     us_iter = ms_p * 1e3 / max(st_p["iterations"], 1) if st_p else nan
     pcg_gbs = B_PCG_ITER * n / (us_iter * 1e-6) / 1e9 if st_p else nan
+    thermal = None
+    if "thermal" in legs:
+        thermal = conduction_leg(core, torch, gmat, mat, cfg, nx, ny, nz, n)
+
     mixed = None
     if "mixed" in legs:
         # Config 3's mixed-precision variant: FP64 refinement around Jacobi-PCG
@@ -306,6 +353,7 @@ def main():
                                    "frac": fin(pcg_gbs / HBM_PEAK_GBS), "bytes_per_cell": B_PCG_ITER,
                                    "us_per_iteration": fin(us_iter)},
         "mixed_precision_refinement": mixed,
+        "thermal_conduction": thermal,
         "sequence_ar1": seq,
         "cpu_baseline": cpu,
     }

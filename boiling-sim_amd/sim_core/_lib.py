@@ -114,6 +114,8 @@ SIGNATURES = {
     "bs_unet_adam": [VP, VP, VP, VP, I, F, F, S],
     "bs_fcg_workspace_bytes": [I, I, I, I],
     "bs_fcg_solve": [VP, VP, VP, I, I, I, F, F, F, I, I, c_double, I, I, I, VP, VP, VP, c_size_t, S, c_void_p],
+    "bs_conduction_sweep_launch": [VP, VP, VP, VP, VP, VP, VP, I, I, I, I, F, F, F, F, I, I, S],
+    "bs_conduction_solve_launch": [VP, VP, VP, VP, VP, VP, VP, I, I, I, I, F, F, F, F, I, I, I, S],
     "bs_refine_workspace_bytes": [I, I, I],
     "bs_fcg_refine_workspace_bytes": [I, I, I, I],
     "bs_fcg_refine_solve": [VP, VP, VP, VP, I, I, I, F, F, F, I, I, c_double, I, c_double, I, I, I, VP, VP, VP,

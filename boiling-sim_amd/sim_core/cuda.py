@@ -270,3 +270,23 @@ class SimCore:
             _ptr("stream", stream), ctypes.byref(st),
         )
         return st.as_dict()
+
+    # ---- backward-Euler heat conduction (thermal.py:259-388, 631-665) -------
+    def conduction_sweep(self, t_dst_ptr, t_iter_ptr, t_old_ptr, mat_ptr, k_ptr, rho_ptr, cp_ptr, n_mat, nx, ny, nz,
+                         dx, dt, h_conv_air, t_amb_k, mat_air=2, mat_fluid=0, stream=0) -> None:
+        """One apply_implicit_conduction_update sweep (asynchronous)."""
+        _lib.call("bs_conduction_sweep_launch", _ptr("t_dst_ptr", t_dst_ptr), _ptr("t_iter_ptr", t_iter_ptr),
+                  _ptr("t_old_ptr", t_old_ptr), _ptr("mat_ptr", mat_ptr), _ptr("k_ptr", k_ptr),
+                  _ptr("rho_ptr", rho_ptr), _ptr("cp_ptr", cp_ptr), _i32("n_mat", n_mat), _usize("nx", nx),
+                  _usize("ny", ny), _usize("nz", nz), float(dx), float(dt), float(h_conv_air), float(t_amb_k),
+                  _i32("mat_air", mat_air), _i32("mat_fluid", mat_fluid), _ptr("stream", stream))
+
+    def conduction_solve(self, t_ptr, t_alt_ptr, t_old_ptr, mat_ptr, k_ptr, rho_ptr, cp_ptr, n_mat, nx, ny, nz, dx,
+                         dt, h_conv_air, t_amb_k, n_iter, mat_air=2, mat_fluid=0, stream=0) -> None:
+        """The backward-Euler branch of conduct_one_step (asynchronous)."""
+        _lib.call("bs_conduction_solve_launch", _ptr("t_ptr", t_ptr), _ptr("t_alt_ptr", t_alt_ptr),
+                  _ptr("t_old_ptr", t_old_ptr), _ptr("mat_ptr", mat_ptr), _ptr("k_ptr", k_ptr),
+                  _ptr("rho_ptr", rho_ptr), _ptr("cp_ptr", cp_ptr), _i32("n_mat", n_mat), _usize("nx", nx),
+                  _usize("ny", ny), _usize("nz", nz), float(dx), float(dt), float(h_conv_air), float(t_amb_k),
+                  _usize("n_iter", n_iter), _i32("mat_air", mat_air), _i32("mat_fluid", mat_fluid),
+                  _ptr("stream", stream))

@@ -238,6 +238,27 @@ int bs_fcg_refine_solve(double* x, float* x32, const float* div_u, const int* ma
                         const void* unet_packed, void* workspace, size_t workspace_bytes, bs_stream_t stream,
                         bs_refine_stats* stats);
 
+/* ---- Backward-Euler Jacobi heat conduction --------------------------------- */
+
+/* One sweep of apply_implicit_conduction_update
+ * (python/boilingsim/thermal.py:259-388): per non-air cell
+ * T_dst = (T_old + sum g*T_nbr)/(1 + sum g) with the harmonic face
+ * conductivity between materials and h_conv_air*dx at air faces; air cells
+ * are not written.  k_arr, rho_arr, cp_arr: n_mat (<= 16) floats on the
+ * device, indexed by material id.  Bit-identical to the reference arithmetic
+ * with each float operation rounded separately.  Asynchronous. */
+int bs_conduction_sweep_launch(float* t_dst, const float* t_iter, const float* t_old, const int* mat,
+                               const float* k_arr, const float* rho_arr, const float* cp_arr, int n_mat,
+                               int nx, int ny, int nz, float dx, float dt, float h_conv_air, float t_amb_k,
+                               int mat_air, int mat_fluid, bs_stream_t stream);
+/* The backward-Euler branch of conduct_one_step (thermal.py:648-665):
+ * t_old = t, n_iter sweeps ping-ponging between t and t_alt, the result in t
+ * (an odd count copies t_alt to t, as the reference).  Asynchronous. */
+int bs_conduction_solve_launch(float* t, float* t_alt, float* t_old, const int* mat, const float* k_arr,
+                               const float* rho_arr, const float* cp_arr, int n_mat, int nx, int ny, int nz,
+                               float dx, float dt, float h_conv_air, float t_amb_k, int n_iter, int mat_air,
+                               int mat_fluid, bs_stream_t stream);
+
 /* Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on FP32 params; gscale
  * multiplies the gradient (1/batch). */
 int bs_unet_adam(float* params, const float* grads, float* m, float* v, int step, float lr,

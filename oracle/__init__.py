@@ -21,6 +21,14 @@ vectors and Warp cannot be imported here):
 * ``laplacian_f64`` / ``pcg_f64`` / ``pcg_ref`` / ``residual_f64`` — scipy
   sparse matrix assembled independently from the class rules, scipy spsolve,
   and the reference's tests' gates (test_pressure_cg.py:49-434).
+* ``conduction_sweep`` / ``conduction_solve`` (oracle/thermal_ref.c, the
+  backward-Euler Jacobi conduction of thermal.py:259-388 and 631-665) —
+  closed forms (one solid cell cooled through one air face; a water/steel
+  cell pair with the harmonic face conductivity; the uniform field as a fixed
+  point), the converged sweep against an independently assembled FP64
+  backward-Euler system solved by scipy, exact energy conservation of that
+  solve in a closed box, and the discrete amplification factor of a cosine
+  mode (test_thermal.py:135-296 restated for the implicit scheme).
 """
 
 from __future__ import annotations
@@ -57,6 +65,8 @@ def lib():
         L.orc_divergence.argtypes = [P, P, P, P, P, I, I, I, F, I]
         L.orc_jacobi_step.argtypes = [P, P, P, P, I, I, I, F, F, F, I, I]
         L.orc_pressure_solve.argtypes = [P, P, P, P, I, I, I, F, F, F, I, I, I]
+        L.orc_conduction_sweep.argtypes = [P, P, P, P, P, P, P, I, I, I, F, F, F, F, I]
+        L.orc_conduction_solve.argtypes = [P, P, P, P, P, P, P, I, I, I, F, F, F, F, I, I]
         L.orc_gradient.argtypes = [P, P, P, P, P, I, I, I, F, F, F, I]
         L.orc_max_abs.argtypes = [P, c_int64]
         L.orc_max_abs.restype = c_double
@@ -206,3 +216,26 @@ def pcg_ref(div, mat, dx, dt, rho, tol, max_iter, mat_fluid=MAT_FLUID, mat_air=M
     it = lib().orc_pcg_ref(_p(p), _p(div), _p(mat), nx, ny, nz, float(dx), float(dt), float(rho),
                            float(tol), int(max_iter), mat_fluid, mat_air, ctypes.addressof(bsq))
     return p, int(it), bsq.value
+
+
+def conduction_sweep(t_dst, t_iter, t_old, mat, k, rho, cp, dx, dt, h_conv_air, t_amb, mat_air=MAT_AIR):
+    """One backward-Euler Jacobi sweep into t_dst (float32, in place; air
+    cells of t_dst are left as they are)."""
+    mat = _i32(mat)
+    nx, ny, nz = mat.shape
+    t_iter, t_old = _f32(t_iter), _f32(t_old)
+    k, rho, cp = _f32(k), _f32(rho), _f32(cp)
+    assert t_dst.dtype == np.float32 and t_dst.flags.c_contiguous
+    lib().orc_conduction_sweep(_p(t_dst), _p(t_iter), _p(t_old), _p(mat), _p(k), _p(rho), _p(cp), nx, ny, nz,
+                               float(dx), float(dt), float(h_conv_air), float(t_amb), mat_air)
+
+
+def conduction_solve(t, t_alt, t_old, mat, k, rho, cp, dx, dt, h_conv_air, t_amb, n_iter, mat_air=MAT_AIR):
+    """conduct_one_step's backward-Euler branch, in place on t, t_alt, t_old."""
+    mat = _i32(mat)
+    nx, ny, nz = mat.shape
+    k, rho, cp = _f32(k), _f32(rho), _f32(cp)
+    for a in (t, t_alt, t_old):
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    lib().orc_conduction_solve(_p(t), _p(t_alt), _p(t_old), _p(mat), _p(k), _p(rho), _p(cp), nx, ny, nz,
+                               float(dx), float(dt), float(h_conv_air), float(t_amb), int(n_iter), mat_air)
