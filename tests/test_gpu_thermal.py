@@ -133,3 +133,33 @@ def test_too_many_materials_is_an_error(simcore, torch_gpu):
     with pytest.raises(RuntimeError):
         simcore.conduction_sweep(t.data_ptr(), t.data_ptr() + 0, t.data_ptr(), m.data_ptr(), k.data_ptr(),
                                  k.data_ptr(), k.data_ptr(), 17, 4, 4, 4, 1e-3, 0.1, H_CONV, T_AMB)
+
+
+@pytest.mark.parametrize("shape,n_iter", [((96, 96, 64), 15), ((64, 440, 280), 9), ((37, 29, 44), 16),
+                                          ((9, 7, 8), 15)])
+@pytest.mark.parametrize("geometry", ["pot", "random"])
+def test_driver_bit_exact_larger_grids(simcore, torch_gpu, shape, n_iter, geometry):
+    """All three buffers stay bit-identical to the reference's ping-pong on
+    larger grids: the pot (uniform-material chunks take the short form) and
+    random materials with fluid on the domain edges."""
+    torch = torch_gpu
+    from boilsim.fixtures import pot_mask
+
+    P, (k, rho, cp) = _props(torch, "aluminum")
+    if geometry == "pot":
+        mat = pot_mask(*shape)
+        rng = np.random.default_rng(2)
+        t0 = (295.0 + 200.0 * rng.random(shape)).astype(np.float32)
+        dx, dt = 1e-3, 0.2
+    else:
+        mat, t0, _, dx, dt = _case(31, shape)
+    alt0 = np.full(shape, -3.0, np.float32)
+    t_r, alt_r, old_r = t0.copy(), alt0.copy(), np.zeros(shape, np.float32)
+    oracle.conduction_solve(t_r, alt_r, old_r, mat, P.k, P.rho, P.c_p, dx, dt, H_CONV, T_AMB, n_iter)
+    t, alt, old = dev(torch, t0), dev(torch, alt0), torch.zeros(shape, dtype=torch.float32, device="cuda")
+    md = dev(torch, mat)
+    simcore.conduction_solve(t.data_ptr(), alt.data_ptr(), old.data_ptr(), md.data_ptr(), k.data_ptr(),
+                             rho.data_ptr(), cp.data_ptr(), k.numel(), *shape, dx, dt, H_CONV, T_AMB, n_iter)
+    torch.cuda.synchronize()
+    for g, r in ((host(t), t_r), (host(alt), alt_r), (host(old), old_r)):
+        assert same_bits(g, r), bit_diff_report(g, r)
