@@ -9,7 +9,9 @@
 // the coarse classes.  Restriction r_c = 4 * mean over the 8 children (the
 // 4 = (2h/h)^2 keeps the unscaled stencil consistent across levels);
 // prolongation is cell-centred trilinear (weights 3/4, 1/4 per axis).
-// Smoother: z += w D^-1 (r - A z), w = 0.8, two sweeps before and after.
+// Smoother: z += w D^-1 (r - A z), w = 0.8: two sweeps before on both fine
+// levels, two after on the 2x-coarsened one and one after on the fine one
+// (a second fine post-sweep costs more time than the iterations it saves).
 // The network's correction on the coarsest grid is polished with damped-Jacobi
 // sweeps there, which keeps the preconditioner positive where the network
 // alone is not.
@@ -488,8 +490,8 @@ __global__ __launch_bounds__(256) void prolong_vec_kernel(Tiles F, Tiles C, cons
 // (clamped at the edges) over the 4-value k window are loaded once and
 // shared; per fine cell the arithmetic is that of prolong_vec_kernel.
 __global__ __launch_bounds__(256) void prolong4_kernel(Tiles F, Tiles CV, const uint8_t* __restrict__ fcode,
-                                                       const float* __restrict__ ec, float* __restrict__ z,
-                                                       const int* __restrict__ done) {
+                                                       const float* __restrict__ ec, const float* __restrict__ z_in,
+                                                       float* __restrict__ z, const int* __restrict__ done) {
     if (done && *done) return;
     // CV tiles the (coarse nx, coarse ny, fine nz) index space, 4 k per thread.
     int I, J, k0;
@@ -526,7 +528,7 @@ __global__ __launch_bounds__(256) void prolong4_kernel(Tiles F, Tiles CV, const 
             uint8_t cc[4];
             float zv[4];
             ldc<4>(fcode, c0, cc);
-            ldf<4>(z, c0, zv);
+            ldf<4>(z_in, c0, zv);
             // rows (I,J), (I,J2), (I2,J), (I2,J2) with I2 = I -/+ 1 for even/odd i
             const int a2 = a ? 2 : 0, b2 = b ? 2 : 0;
             float cw[4];
@@ -553,9 +555,11 @@ __global__ __launch_bounds__(256) void prolong4_kernel(Tiles F, Tiles CV, const 
         }
 }
 
-// z += P ec from level LC to its refinement LF (tiles TF, vector width vf).
-static void prolong_level(hipStream_t st, const Tiles& TF, int vf, const MLvl& LF, const MLvl& LC,
-                          const Tiles& TC, const uint8_t* fcode, const float* ec, float* z, const int* done);
+// z_out = z_in + P ec from level LC to its refinement LF (tiles TF, vector
+// width vf); z_out may be z_in.
+static int prolong_level(hipStream_t st, const Tiles& TF, int vf, const MLvl& LF, const MLvl& LC,
+                         const Tiles& TC, const uint8_t* fcode, const float* ec, const float* z_in, float* z_out,
+                         const int* done);
 
 static inline size_t al(size_t v) { return (v + 255) & ~(size_t)255; }
 
@@ -658,14 +662,18 @@ unsigned mg_dots_groups(int nx, int ny, int nz) { return make_tiles(nx, ny, nz, 
         else KERNEL<1><<<(GRID), 256, 0, (ST)>>>(__VA_ARGS__);                     \
     } while (0)
 
-static void prolong_level(hipStream_t st, const Tiles& TF, int vf, const MLvl& LF, const MLvl& LC,
-                          const Tiles& TC, const uint8_t* fcode, const float* ec, float* z, const int* done) {
+static int prolong_level(hipStream_t st, const Tiles& TF, int vf, const MLvl& LF, const MLvl& LC,
+                         const Tiles& TC, const uint8_t* fcode, const float* ec, const float* z_in, float* z_out,
+                         const int* done) {
     if (vf == 4) {
         const Tiles CV = make_tiles(LC.nx, LC.ny, LF.nz, 4);
-        prolong4_kernel<<<CV.ntiles, 256, 0, st>>>(TF, CV, fcode, ec, z, done);
+        prolong4_kernel<<<CV.ntiles, 256, 0, st>>>(TF, CV, fcode, ec, z_in, z_out, done);
     } else {
-        MG_VEC(prolong_vec_kernel, vf, TF.ntiles, st, TF, TC, fcode, ec, z, done);
+        if (z_out != z_in)
+            BS_RETURN_IF_ERR(hipMemcpyAsync(z_out, z_in, sizeof(float) * (size_t)LF.n, hipMemcpyDeviceToDevice, st));
+        MG_VEC(prolong_vec_kernel, vf, TF.ntiles, st, TF, TC, fcode, ec, z_out, done);
     }
+    return (int)hipGetLastError();
 }
 
 // Restriction of the residual r - A z from level F to its coarsening: the
@@ -732,12 +740,13 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
         sweeps(2);
     }
     // back up: level 1
-    prolong_level(st, T1, v1, W.L1, W.L2, C2, W.code1, W.e2, W.z1, done);
+    int rc = prolong_level(st, T1, v1, W.L1, W.L2, C2, W.code1, W.e2, W.z1, W.z1, done);
+    if (rc != hipSuccess) return rc;
     MG_VEC(smooth_kernel, v1, T1.ntiles, st, T1, W.code1, W.r1, W.z1, W.zt1, done);
     MG_VEC(smooth_kernel, v1, T1.ntiles, st, T1, W.code1, W.r1, W.zt1, W.z1, done);
-    // fine: prolongation, one sweep, and the last sweep fused with r.z, r.z_old
-    prolong_level(st, T0, v0, W.L0, W.L1, C1, code0, W.z1, z, done);
-    MG_VEC(smooth_kernel, v0, T0.ntiles, st, T0, code0, r, z, W.zt0, done);
+    // fine: prolongation, then one sweep fused with r.z, r.z_old
+    rc = prolong_level(st, T0, v0, W.L0, W.L1, C1, code0, W.z1, z, W.zt0, done);
+    if (rc != hipSuccess) return rc;
     if (T0.ntiles != mg_dots_groups(nx, ny, nz)) return (int)hipErrorInvalidValue;  // misaligned caller buffers
     MG_VEC(smooth_dots_kernel, v0, T0.ntiles, st, T0, code0, r, W.zt0, z, z_old, dots_part, done);
     return (int)hipGetLastError();
