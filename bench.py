@@ -66,7 +66,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-ablation", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=12, help="oracle PCG iterations in the CPU sample")
-    ap.add_argument("--legs", default="learned,pcg,ablation,jacobi,mixed,thermal,sequence,cpu",
+    ap.add_argument("--legs", default="learned,pcg,ablation,jacobi,mixed,thermal,faces,sequence,cpu",
                     help="comma list of the legs to run (the JSON fields of skipped legs are null)")
     ap.add_argument("--seq-frames", type=int, default=6,
                     help="frames of the warm-started AR(1) sequence leg (0: skip; the full run is "
@@ -150,6 +150,75 @@ def conduction_leg(core, torch, gmat, mat, cfg, nx, ny, nz, n):
     log(f"conduction: {us:.1f} us/sweep ({gbs:.0f} GB/s algorithmic, 16 B/cell), {ms_step:.2f} ms per 15-sweep step")
     return {"us_per_sweep": us, "achieved_gbs": gbs, "frac_of_peak": gbs / HBM_PEAK_GBS, "bytes_per_cell": 16.0,
             "ms_per_step_15_sweeps": ms_step, "kernel": "conduction_sweep_kernel<4>"}
+
+
+def faces_leg(core, torch, gmat, ux0, uy0, uz0, gux, guy, guz, div, cfg, nx, ny, nz, n, reps=10):
+    """The face passes around the projection (pipeline.py:606-620,
+    fluid.py:111-162, 726-734, 796-833), unfused against fused, each variant
+    timed with HIP events on the null stream over `reps` runs.
+      pre:  enforce_no_slip + compute_divergence  vs  divergence_no_slip,
+            on the seed-11 faces (before any no-slip)
+      post: compute_divergence + max|div| + enforce_no_slip +
+            compute_max_velocity  vs  divergence_max, on no-slipped faces
+    The gradient subtraction between them is the same in both and not timed.
+    Algorithmic bytes of each fused pass: mat and the three face arrays read,
+    div written = 20 B/cell; the pre pass also stores every face it zeroes
+    (on these faces, all the faces that are not between two fluid cells:
+    4 B x that count per cell)."""
+    ptr = lambda t: t.data_ptr()
+    F = (ptr(gux), ptr(guy), ptr(guz))
+    mb = torch.zeros(2, dtype=torch.int32, device="cuda")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def reset(no_slip):
+        gux.copy_(ux0)
+        guy.copy_(uy0)
+        guz.copy_(uz0)
+        if no_slip:
+            core.enforce_no_slip(*F, ptr(gmat), nx, ny, nz)
+        torch.cuda.synchronize()
+
+    def timed(fn, no_slip):
+        ts = []
+        for _ in range(reps + 1):
+            reset(no_slip)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e3)
+        return float(np.mean(ts[1:]))
+
+    div_ = lambda: core.compute_divergence(ptr(div), *F, ptr(gmat), nx, ny, nz, cfg.dx)
+    ns = lambda: core.enforce_no_slip(*F, ptr(gmat), nx, ny, nz)
+    res = {
+        "pre_unfused_us": timed(lambda: (ns(), div_()), False),
+        "pre_fused_us": timed(lambda: core.divergence_no_slip(ptr(div), *F, ptr(gmat), nx, ny, nz, cfg.dx), False),
+        "post_unfused_us": timed(lambda: (div_(), core.max_abs(ptr(div), n, ptr(mb)), ns(),
+                                          core.max_velocity(*F, nx, ny, nz, ptr(mb) + 4)), True),
+        "post_fused_us": timed(lambda: core.divergence_max(ptr(div), *F, ptr(gmat), nx, ny, nz, cfg.dx, ptr(mb)),
+                               True),
+        "enforce_no_slip_us": timed(ns, False),
+        "max_velocity_us": timed(lambda: core.max_velocity(*F, nx, ny, nz, ptr(mb)), True),
+        "divergence_us": timed(div_, True),
+    }
+    fl = torch.nn.functional.pad((gmat == 0).to(torch.uint8), (1, 1, 1, 1, 1, 1)).bool()
+    kept = int((fl[:-1, 1:-1, 1:-1] & fl[1:, 1:-1, 1:-1]).sum() + (fl[1:-1, :-1, 1:-1] & fl[1:-1, 1:, 1:-1]).sum() +
+               (fl[1:-1, 1:-1, :-1] & fl[1:-1, 1:-1, 1:]).sum())
+    del fl
+    zeroed = ((nx + 1) * ny * nz + nx * (ny + 1) * nz + nx * ny * (nz + 1) - kept) / n
+    res["zeroed_faces_per_cell"] = zeroed
+    res["bytes_per_cell_pre_fused"] = 20.0 + 4.0 * zeroed
+    res["pre_fused_gbs"] = res["bytes_per_cell_pre_fused"] * n / (res["pre_fused_us"] * 1e-6) / 1e9
+    res["post_fused_gbs"] = 20.0 * n / (res["post_fused_us"] * 1e-6) / 1e9
+    res["saved_us_per_step"] = (res["pre_unfused_us"] + res["post_unfused_us"] - res["pre_fused_us"] -
+                                res["post_fused_us"])
+    res["bytes_per_cell_post_fused"] = 20.0
+    log(f"faces: pre {res['pre_unfused_us']:.1f} -> {res['pre_fused_us']:.1f} us "
+        f"({res['pre_fused_gbs']:.0f} GB/s), post {res['post_unfused_us']:.1f} -> {res['post_fused_us']:.1f} us "
+        f"({res['post_fused_gbs']:.0f} GB/s); enforce_no_slip {res['enforce_no_slip_us']:.1f} us, "
+        f"max_velocity {res['max_velocity_us']:.1f} us, divergence {res['divergence_us']:.1f} us")
+    return res
 
 
 def main():
@@ -261,6 +330,9 @@ def main():
     thermal = None
     if "thermal" in legs:
         thermal = conduction_leg(core, torch, gmat, mat, cfg, nx, ny, nz, n)
+    faces = None
+    if "faces" in legs:
+        faces = faces_leg(core, torch, gmat, ux0, uy0, uz0, gux, guy, guz, div, cfg, nx, ny, nz, n)
 
     mixed = None
     if "mixed" in legs:
@@ -354,6 +426,7 @@ def main():
                                    "us_per_iteration": fin(us_iter)},
         "mixed_precision_refinement": mixed,
         "thermal_conduction": thermal,
+        "no_slip_faces": faces,
         "sequence_ar1": seq,
         "cpu_baseline": cpu,
     }

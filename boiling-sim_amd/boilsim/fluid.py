@@ -12,6 +12,14 @@ relative residual ``pressure_tol``, FP64 residual check) and ``lpcg-hip``
 The Warp kernels of the reference (divergence, gradient subtraction) run as
 HIP kernels here, and the returned max|div| is reduced on the device instead of
 copying the whole field to the host (fluid.py:733-734).
+
+The faces around the projection: ``enforce_no_slip`` (fluid.py:111-162) is one
+launch, ``compute_max_velocity`` / ``compute_cfl_dt`` (fluid.py:796-844) reduce
+on the device, and ``projection_step`` runs a step's no-slip, projection,
+no-slip (pipeline.py:606-620) with the first no-slip fused into the
+divergence pass, the second one omitted because it would change no bit, and
+max|u| for the next step's CFL time step taken from the diagnostic divergence
+pass and read back with max|div|.
 """
 
 from __future__ import annotations
@@ -28,10 +36,13 @@ VALID_SOLVERS = ("jacobi", "cg-rust", "pcg-hip", "lpcg-hip")
 
 @dataclass
 class SolverConfig:
-    """The two knobs the projection reads (python/boilingsim/config.py:161-190)."""
+    """The solver knobs the projection and the CFL time step read
+    (python/boilingsim/config.py:161-190)."""
 
     pressure_tol: float = 1e-5
     pressure_max_iter: int = 200
+    cfl_safety_factor: float = 0.4  # config.py:164
+    max_dt_s: float = 0.1  # config.py:165
 
 
 @dataclass
@@ -65,7 +76,8 @@ class FluidWorkspace:
 
     div_u: object
     p_tmp: object
-    max_bits: object  # int32[1]: max|div| as float bits
+    max_bits: object  # int32[2]: max|div| and max|u| as float bits
+    u_max: float = 0.0  # max|u| after the last projection_step
 
 
 def make_grid(mat: np.ndarray, ux: np.ndarray, uy: np.ndarray, uz: np.ndarray, dx: float,
@@ -85,7 +97,7 @@ def allocate_fluid_workspace(grid: Grid) -> FluidWorkspace:
     return FluidWorkspace(
         div_u=torch.zeros(grid.shape, dtype=torch.float32, device=dev),
         p_tmp=torch.zeros(grid.shape, dtype=torch.float32, device=dev),
-        max_bits=torch.zeros(1, dtype=torch.int32, device=dev),
+        max_bits=torch.zeros(2, dtype=torch.int32, device=dev),
     )
 
 
@@ -168,6 +180,38 @@ def _select_pressure_solver() -> str:
 LAST_SOLVE_STATS: dict = {}
 
 
+def _solve(core, solver, grid: Grid, ws: FluidWorkspace, cfg: ScenarioConfig, dt, rho, preconditioner):
+    """The non-Jacobi solvers of pressure_projection (fluid.py:643-687): p from
+    ws.div_u."""
+    nx, ny, nz = grid.shape
+    dx = grid.dx
+    mi = int(cfg.solver.pressure_max_iter)
+    if solver == "cg-rust":
+        w = _get_pcg_workspace(grid.shape, grid.p.device)
+        iters = core.pressure_solve_pcg(
+            grid.p.data_ptr(), ws.div_u.data_ptr(), grid.mat.data_ptr(), nx, ny, nz, dx, dt, rho,
+            float(cfg.solver.pressure_tol), mi,
+            *(w[k].data_ptr() for k in ("b", "r", "z", "p_search", "Ap", "dot_ws", "alpha", "beta",
+                                        "rzold", "rznew", "bsq", "rsq", "pAp")),
+            MAT_FLUID, MAT_AIR)
+        st = {"iterations": iters}
+    elif solver == "pcg-hip":
+        buf = _get_solve_workspace(core, grid.shape, grid.p.device)
+        st = core.pcg_solve(grid.p.data_ptr(), ws.div_u.data_ptr(), grid.mat.data_ptr(), nx, ny, nz,
+                            dx, dt, rho, float(cfg.solver.pressure_tol), mi, buf.data_ptr(),
+                            buf.numel(), warm_start=False)
+    else:  # lpcg-hip
+        from .learned import learned_pcg_solve
+
+        st = learned_pcg_solve(core, grid, ws, dt, rho, float(cfg.solver.pressure_tol), mi, preconditioner)
+    LAST_SOLVE_STATS.clear()
+    LAST_SOLVE_STATS.update({"solver": solver, **st})
+
+
+def _bits_to_floats(t) -> np.ndarray:
+    return t.cpu().numpy().view(np.float32)
+
+
 def pressure_projection(grid: Grid, ws: FluidWorkspace, cfg: ScenarioConfig, dt: float, rho: float,
                         device: str = "cuda", preconditioner=None) -> float:
     """Enforce incompressibility (fluid.py:612-734); returns max|div u| after
@@ -187,34 +231,74 @@ def pressure_projection(grid: Grid, ws: FluidWorkspace, cfg: ScenarioConfig, dt:
     else:
         core.compute_divergence(ws.div_u.data_ptr(), grid.ux.data_ptr(), grid.uy.data_ptr(),
                                 grid.uz.data_ptr(), grid.mat.data_ptr(), nx, ny, nz, dx, MAT_FLUID)
-        if solver == "cg-rust":
-            w = _get_pcg_workspace(grid.shape, grid.p.device)
-            iters = core.pressure_solve_pcg(
-                grid.p.data_ptr(), ws.div_u.data_ptr(), grid.mat.data_ptr(), nx, ny, nz, dx, dt, rho,
-                float(cfg.solver.pressure_tol), mi,
-                *(w[k].data_ptr() for k in ("b", "r", "z", "p_search", "Ap", "dot_ws", "alpha", "beta",
-                                            "rzold", "rznew", "bsq", "rsq", "pAp")),
-                MAT_FLUID, MAT_AIR)
-            LAST_SOLVE_STATS.clear()
-            LAST_SOLVE_STATS.update({"solver": solver, "iterations": iters})
-        elif solver == "pcg-hip":
-            buf = _get_solve_workspace(core, grid.shape, grid.p.device)
-            st = core.pcg_solve(grid.p.data_ptr(), ws.div_u.data_ptr(), grid.mat.data_ptr(), nx, ny, nz,
-                                dx, dt, rho, float(cfg.solver.pressure_tol), mi, buf.data_ptr(),
-                                buf.numel(), warm_start=False)
-            LAST_SOLVE_STATS.clear()
-            LAST_SOLVE_STATS.update({"solver": solver, **st})
-        else:  # lpcg-hip
-            from .learned import learned_pcg_solve
-
-            st = learned_pcg_solve(core, grid, ws, dt, rho, float(cfg.solver.pressure_tol), mi,
-                                   preconditioner)
-            LAST_SOLVE_STATS.clear()
-            LAST_SOLVE_STATS.update({"solver": solver, **st})
+        _solve(core, solver, grid, ws, cfg, dt, rho, preconditioner)
         core.subtract_pressure_gradient(grid.ux.data_ptr(), grid.uy.data_ptr(), grid.uz.data_ptr(),
                                         grid.p.data_ptr(), grid.mat.data_ptr(), nx, ny, nz, dx, dt, rho,
                                         MAT_FLUID)
         core.compute_divergence(ws.div_u.data_ptr(), grid.ux.data_ptr(), grid.uy.data_ptr(),
                                 grid.uz.data_ptr(), grid.mat.data_ptr(), nx, ny, nz, dx, MAT_FLUID)
         core.max_abs(ws.div_u.data_ptr(), grid.nx * grid.ny * grid.nz, ws.max_bits.data_ptr())
-    return float(ws.max_bits.cpu().numpy().view(np.float32)[0])
+    return float(_bits_to_floats(ws.max_bits)[0])
+
+
+def enforce_no_slip(grid: Grid, device: str = "cuda") -> None:
+    """Zero every MAC face that touches a non-fluid cell or the domain edge
+    (fluid.py:153-162), in one launch."""
+    core = _get_sim_core()
+    core.enforce_no_slip(grid.ux.data_ptr(), grid.uy.data_ptr(), grid.uz.data_ptr(), grid.mat.data_ptr(),
+                         *grid.shape, MAT_FLUID)
+
+
+def compute_max_velocity(grid: Grid, ws: FluidWorkspace | None = None) -> float:
+    """max|u| over all MAC faces (fluid.py:807-833), reduced on the device;
+    one 4-byte readback."""
+    import torch
+
+    core = _get_sim_core()
+    out = ws.max_bits[1:] if ws is not None else torch.zeros(1, dtype=torch.int32, device=grid.ux.device)
+    core.max_velocity(grid.ux.data_ptr(), grid.uy.data_ptr(), grid.uz.data_ptr(), *grid.shape, out.data_ptr())
+    return float(_bits_to_floats(out)[0])
+
+
+def compute_cfl_dt(grid: Grid, cfg: ScenarioConfig, ws: FluidWorkspace | None = None,
+                   u_max: float | None = None) -> float:
+    """CFL-limited time step from the current max velocity (fluid.py:836-844).
+    ``u_max`` may be passed in, e.g. ``ws.u_max`` after ``projection_step``,
+    to skip the reduction."""
+    if u_max is None:
+        u_max = compute_max_velocity(grid, ws=ws)
+    if u_max < 1e-8:
+        return cfg.solver.max_dt_s
+    return cfg.solver.cfl_safety_factor * grid.dx / u_max
+
+
+def projection_step(grid: Grid, ws: FluidWorkspace, cfg: ScenarioConfig, dt: float, rho: float,
+                    device: str = "cuda", preconditioner=None) -> float:
+    """enforce_no_slip, pressure_projection, enforce_no_slip (pipeline.py:
+    606-620) with identical results in fewer passes: the first no-slip runs
+    inside the first divergence pass, the second is omitted (the faces it
+    would zero are +0.0 already: the first pass zeroed them and the gradient
+    subtraction writes only interior faces between fluid cells), and the
+    diagnostic divergence pass also reduces max|u|.  Returns max|div u| as
+    pressure_projection does and leaves max|u| in ``ws.u_max``, both from one
+    8-byte readback."""
+    core = _get_sim_core()
+    nx, ny, nz = grid.shape
+    dx = grid.dx
+    solver = _select_pressure_solver()
+    faces = (grid.ux.data_ptr(), grid.uy.data_ptr(), grid.uz.data_ptr())
+    if solver in ("jacobi", "rust-jacobi"):
+        core.projection_step_jacobi(grid.p.data_ptr(), ws.p_tmp.data_ptr(), ws.div_u.data_ptr(), *faces,
+                                    grid.mat.data_ptr(), nx, ny, nz, dx, dt, rho,
+                                    int(cfg.solver.pressure_max_iter), ws.max_bits.data_ptr(), MAT_FLUID,
+                                    MAT_AIR)
+    else:
+        core.divergence_no_slip(ws.div_u.data_ptr(), *faces, grid.mat.data_ptr(), nx, ny, nz, dx, MAT_FLUID)
+        _solve(core, solver, grid, ws, cfg, dt, rho, preconditioner)
+        core.subtract_pressure_gradient(*faces, grid.p.data_ptr(), grid.mat.data_ptr(), nx, ny, nz, dx, dt,
+                                        rho, MAT_FLUID)
+        core.divergence_max(ws.div_u.data_ptr(), *faces, grid.mat.data_ptr(), nx, ny, nz, dx,
+                            ws.max_bits.data_ptr(), MAT_FLUID)
+    m = _bits_to_floats(ws.max_bits)
+    ws.u_max = float(m[1])
+    return float(m[0])

@@ -147,7 +147,12 @@ __global__ __launch_bounds__(256) void jacobi_sweep_kernel(
                 continue;
             }
             const float rhs = (c2 * dv[e]) / dt;
-            out[e] = (0.0f - rhs) / 6.0f;
+            // s - rhs with s = +0: -rhs, except +0 (not -0) when rhs is a
+            // zero.  Written as a select because the gfx950 instruction
+            // selector folds (0.0f - x) into a negate modifier on the
+            // division's operand, which turns 0 - (+0) into -0.
+            const float num = (rhs == 0.0f) ? 0.0f : -rhs;
+            out[e] = num / 6.0f;
         }
     } else {
         int mc[VEC], ml[VEC], mr[VEC], md[VEC], mu[VEC];

@@ -241,6 +241,214 @@ static RowMap row_map(int ny, int nz, int vec, dim3& grid, int nx) {
     return rm;
 }
 
+// ---------------------------------------------------------------------------
+// No-slip faces, and the maxima the step reads back (fluid.py:111-162,
+// 726-734, 796-844).
+//
+// enforce_no_slip_u{x,y,z} (fluid.py:111-150): a MAC face keeps its value only
+// when it is interior and both cells it separates are fluid; every other face
+// becomes +0.0.  In faces_kernel the thread of cell (i,j,k) owns that cell's
+// low x-, y- and z-faces, and the cells on the high domain edges also own the
+// edge faces ux[nx], uy[:,ny], uz[:,:,nz], so every face has one owner.
+// A face is stored only when its bits change (a kept face, or one that is
+// already +0.0, is not rewritten).
+//
+//   kNoSlip     the three no-slip launches in one pass
+//   kDivNoSlip  no-slip, then compute_divergence of the result, in one pass:
+//               a fluid cell reads its high faces (owned by its neighbours)
+//               and applies the same keep rule to them, so the divergence
+//               sees the post-no-slip values whether or not the owner has
+//               stored them yet
+//   kDivMax     compute_divergence, with max|div| over the cells and max|u|
+//               over the faces reduced on the device (no face is written)
+// ---------------------------------------------------------------------------
+enum FaceMode { kNoSlip = 0, kDivNoSlip = 1, kDivMax = 2 };
+
+__device__ __forceinline__ uint32_t abs_bits(float v) { return __float_as_uint(v) & 0x7FFFFFFFu; }
+__device__ __forceinline__ uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
+
+// Both maxima of a workgroup into max_bits[0] (|div|) and max_bits[1] (|u|).
+// The maxima only grow, so a block whose value does not exceed one it reads
+// back (possibly stale, hence never too large) has nothing to add.
+__device__ __forceinline__ void block_max2(uint32_t a, uint32_t b, unsigned int* __restrict__ max_bits) {
+    for (int off = 32; off > 0; off >>= 1) {
+        a = umax(a, (uint32_t)__shfl_xor((int)a, off, 64));
+        b = umax(b, (uint32_t)__shfl_xor((int)b, off, 64));
+    }
+    __shared__ uint32_t sm[2][4];
+    if ((threadIdx.x & 63) == 0) {
+        sm[0][threadIdx.x >> 6] = a;
+        sm[1][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a = umax(umax(sm[0][0], sm[0][1]), umax(sm[0][2], sm[0][3]));
+        b = umax(umax(sm[1][0], sm[1][1]), umax(sm[1][2], sm[1][3]));
+        if (a > __hip_atomic_load(max_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(max_bits, a);
+        if (b > __hip_atomic_load(max_bits + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(max_bits + 1, b);
+    }
+}
+
+template <int VEC>
+__device__ __forceinline__ bool changed(const float (&a)[VEC], const float (&b)[VEC]) {
+    bool c = false;
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) c |= __float_as_uint(a[e]) != __float_as_uint(b[e]);
+    return c;
+}
+
+template <int VEC, int MODE>
+__global__ __launch_bounds__(256) void faces_kernel(float* __restrict__ div, float* __restrict__ ux,
+                                                    float* __restrict__ uy, float* __restrict__ uz,
+                                                    const int* __restrict__ mat, int nx, int ny, int nz,
+                                                    RowMap rm, float dx, int mf,
+                                                    unsigned int* __restrict__ max_bits) {
+    int i, j, k0;
+    const bool live = row_coords(rm, ny, VEC, i, j, k0);
+    uint32_t best_div = 0, best_u = 0;
+    if (live) {
+        const int64_t plane = (int64_t)ny * nz;
+        const int64_t c0 = (int64_t)i * plane + (int64_t)j * nz + k0;
+        const int64_t yo = ((int64_t)i * (ny + 1) + j) * nz + k0;
+        const int64_t zo = ((int64_t)i * ny + j) * (nz + 1) + k0;
+        const bool top_x = i == nx - 1, top_y = j == ny - 1, top_z = k0 + VEC == nz;
+        int m[VEC];
+        float x0[VEC], x1[VEC], y0[VEC], y1[VEC], z[VEC + 1];
+        ld<VEC>(mat, c0, m);
+        ld<VEC>(ux, c0, x0);
+        ld<VEC>(ux, c0 + plane, x1);
+        ld<VEC>(uy, yo, y0);
+        ld<VEC>(uy, yo + nz, y1);
+#pragma unroll
+        for (int e = 0; e <= VEC; ++e) z[e] = uz[zo + e];
+        if constexpr (MODE != kDivMax) {
+            // Neighbour classes; a missing neighbour (domain edge) is ~mf,
+            // which is never fluid, so edge faces are never kept.
+            const int nf = ~mf;
+            int ml[VEC], mr[VEC], md[VEC], mu[VEC];
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) ml[e] = mr[e] = md[e] = mu[e] = nf;
+            if (i > 0) ld<VEC>(mat, c0 - plane, ml);
+            if (!top_x) ld<VEC>(mat, c0 + plane, mr);
+            if (j > 0) ld<VEC>(mat, c0 - nz, md);
+            if (!top_y) ld<VEC>(mat, c0 + nz, mu);
+            const int mb = k0 > 0 ? mat[c0 - 1] : nf;
+            const int mt = top_z ? nf : mat[c0 + VEC];
+            float nx0[VEC], nx1[VEC], ny0[VEC], ny1[VEC], nz_[VEC + 1];
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                const bool fl = m[e] == mf;
+                nx0[e] = (fl && ml[e] == mf) ? x0[e] : 0.0f;
+                nx1[e] = (fl && mr[e] == mf) ? x1[e] : 0.0f;
+                ny0[e] = (fl && md[e] == mf) ? y0[e] : 0.0f;
+                ny1[e] = (fl && mu[e] == mf) ? y1[e] : 0.0f;
+            }
+#pragma unroll
+            for (int e = 0; e <= VEC; ++e) {
+                const int lo = e > 0 ? m[e - 1] : mb;
+                const int hi = e < VEC ? m[e] : mt;
+                nz_[e] = (lo == mf && hi == mf) ? z[e] : 0.0f;
+            }
+            // the owned faces
+            if (changed<VEC>(nx0, x0)) st<VEC>(ux, c0, nx0);
+            if (top_x && changed<VEC>(nx1, x1)) st<VEC>(ux, c0 + plane, nx1);
+            if (changed<VEC>(ny0, y0)) st<VEC>(uy, yo, ny0);
+            if (top_y && changed<VEC>(ny1, y1)) st<VEC>(uy, yo + nz, ny1);
+#pragma unroll
+            for (int e = 0; e < VEC; ++e)
+                if (__float_as_uint(nz_[e]) != __float_as_uint(z[e])) uz[zo + e] = nz_[e];
+            if (top_z && __float_as_uint(nz_[VEC]) != __float_as_uint(z[VEC])) uz[zo + VEC] = nz_[VEC];
+            if constexpr (MODE == kDivNoSlip) {
+                float out[VEC];
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) {
+                    if (m[e] != mf) { out[e] = 0.0f; continue; }
+                    const float dudx = (nx1[e] - nx0[e]) / dx;
+                    const float dvdy = (ny1[e] - ny0[e]) / dx;
+                    const float dwdz = (nz_[e + 1] - nz_[e]) / dx;
+                    out[e] = dudx + dvdy + dwdz;
+                }
+                st<VEC>(div, c0, out);
+            }
+        } else {
+            float out[VEC];
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                if (m[e] != mf) {
+                    out[e] = 0.0f;
+                } else {
+                    const float dudx = (x1[e] - x0[e]) / dx;
+                    const float dvdy = (y1[e] - y0[e]) / dx;
+                    const float dwdz = (z[e + 1] - z[e]) / dx;
+                    out[e] = dudx + dvdy + dwdz;
+                }
+                best_div = umax(best_div, abs_bits(out[e]));
+                best_u = umax(best_u, umax(abs_bits(x0[e]), umax(abs_bits(y0[e]), abs_bits(z[e]))));
+                if (top_x) best_u = umax(best_u, abs_bits(x1[e]));
+                if (top_y) best_u = umax(best_u, abs_bits(y1[e]));
+            }
+            if (top_z) best_u = umax(best_u, abs_bits(z[VEC]));
+            st<VEC>(div, c0, out);
+        }
+    }
+    if constexpr (MODE == kDivMax) block_max2(best_div, best_u, max_bits);
+}
+
+// max|u| over the three face arrays (compute_max_velocity, fluid.py:807-833):
+// one grid-stride pass over each array, float4 loads on 16-byte-aligned
+// arrays, one atomicMax per workgroup.
+__global__ __launch_bounds__(256) void max_faces_kernel(const float* __restrict__ a, int64_t na,
+                                                        const float* __restrict__ b, int64_t nb,
+                                                        const float* __restrict__ c, int64_t nc,
+                                                        unsigned int* __restrict__ out_bits) {
+    uint32_t best = 0;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    const int64_t t0 = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const float* arr[3] = {a, b, c};
+    const int64_t len[3] = {na, nb, nc};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const float* x = arr[q];
+        const int64_t n = len[q];
+        int64_t head = 0;
+        if (((uintptr_t)x & 15u) == 0) {
+            const int64_t n4 = n / 4;
+            const float4* x4 = reinterpret_cast<const float4*>(x);
+            for (int64_t t = t0; t < n4; t += stride) {
+                const float4 v = x4[t];
+                best = umax(best, umax(umax(abs_bits(v.x), abs_bits(v.y)), umax(abs_bits(v.z), abs_bits(v.w))));
+            }
+            head = n4 * 4;
+        }
+        for (int64_t t = head + t0; t < n; t += stride) best = umax(best, abs_bits(x[t]));
+    }
+    for (int off = 32; off > 0; off >>= 1) best = umax(best, (uint32_t)__shfl_xor((int)best, off, 64));
+    __shared__ uint32_t sm[4];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(out_bits, umax(umax(sm[0], sm[1]), umax(sm[2], sm[3])));
+}
+
+__global__ void zero_u32x2_kernel(unsigned int* p) {
+    p[0] = 0u;
+    p[1] = 0u;
+}
+
+template <int MODE>
+static int launch_faces(float* div, float* ux, float* uy, float* uz, const int* mat, int nx, int ny, int nz,
+                        float dx, int mf, unsigned int* max_bits, hipStream_t st) {
+    const int vec = pick_vec2(nz, {div, ux, uy, mat});
+    dim3 grid;
+    const RowMap rm = row_map(ny, nz, vec, grid, nx);
+    switch (vec) {
+        case 4: faces_kernel<4, MODE><<<grid, 256, 0, st>>>(div, ux, uy, uz, mat, nx, ny, nz, rm, dx, mf, max_bits); break;
+        case 2: faces_kernel<2, MODE><<<grid, 256, 0, st>>>(div, ux, uy, uz, mat, nx, ny, nz, rm, dx, mf, max_bits); break;
+        default: faces_kernel<1, MODE><<<grid, 256, 0, st>>>(div, ux, uy, uz, mat, nx, ny, nz, rm, dx, mf, max_bits); break;
+    }
+    return (int)hipGetLastError();
+}
+
 }  // namespace bs
 
 using namespace bs;
@@ -286,6 +494,50 @@ extern "C" int bs_max_abs_launch(const float* x, long long n, unsigned int* out_
     return (int)hipGetLastError();
 }
 
+// enforce_no_slip (fluid.py:111-162): the three face launches in one.
+extern "C" int bs_enforce_no_slip_launch(float* ux, float* uy, float* uz, const int* mat, int nx, int ny, int nz,
+                                         int mat_fluid, hipStream_t st) {
+    if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
+    return launch_faces<kNoSlip>(nullptr, ux, uy, uz, mat, nx, ny, nz, 1.0f, mat_fluid, nullptr, st);
+}
+
+// enforce_no_slip followed by compute_divergence (pipeline.py:606-608, then
+// fluid.py:632-637), in one pass: the faces end as enforce_no_slip leaves
+// them and div_u as compute_divergence computes it from those faces.
+extern "C" int bs_divergence_no_slip_launch(float* div_u, float* ux, float* uy, float* uz, const int* mat, int nx,
+                                            int ny, int nz, float dx, int mat_fluid, hipStream_t st) {
+    if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
+    return launch_faces<kDivNoSlip>(div_u, ux, uy, uz, mat, nx, ny, nz, dx, mat_fluid, nullptr, st);
+}
+
+// compute_divergence with max|div| (fluid.py:727-734) and max|u| over all
+// faces (compute_max_velocity, fluid.py:807-833) from the same pass, left on
+// the device as float bits: max_bits[0] = max|div|, max_bits[1] = max|u|.
+extern "C" int bs_divergence_max_launch(float* div_u, const float* ux, const float* uy, const float* uz,
+                                        const int* mat, int nx, int ny, int nz, float dx, int mat_fluid,
+                                        unsigned int* max_bits, hipStream_t st) {
+    zero_u32x2_kernel<<<1, 1, 0, st>>>(max_bits);
+    if (nx <= 0 || ny <= 0 || nz <= 0) return (int)hipGetLastError();
+    return launch_faces<kDivMax>(div_u, const_cast<float*>(ux), const_cast<float*>(uy), const_cast<float*>(uz),
+                                 mat, nx, ny, nz, dx, mat_fluid, max_bits, st);
+}
+
+// compute_max_velocity's device reduction (fluid.py:807-833) in one launch
+// over ux (nx+1,ny,nz), uy (nx,ny+1,nz) and uz (nx,ny,nz+1); max|u| is left
+// on the device as float bits in *out_bits.
+extern "C" int bs_max_velocity_launch(const float* ux, const float* uy, const float* uz, int nx, int ny, int nz,
+                                      unsigned int* out_bits, hipStream_t st) {
+    zero_u32_kernel<<<1, 1, 0, st>>>(out_bits);
+    if (nx <= 0 || ny <= 0 || nz <= 0) return (int)hipGetLastError();
+    const int64_t na = (int64_t)(nx + 1) * ny * nz, nb = (int64_t)nx * (ny + 1) * nz,
+                  nc = (int64_t)nx * ny * (nz + 1);
+    int64_t blocks = (na / 4 + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) blocks = 1;
+    max_faces_kernel<<<(unsigned)blocks, 256, 0, st>>>(ux, na, uy, nb, uz, nc, out_bits);
+    return (int)hipGetLastError();
+}
+
 extern "C" int bs_jacobi_solve_launch(float* p, float* p_tmp, const float* div_u, const int* mat,
                                       int nx, int ny, int nz, float dx, float dt, float rho,
                                       int n_iter, int mat_fluid, int mat_air, int zero_init,
@@ -322,4 +574,38 @@ extern "C" int bs_projection_jacobi_launch(float* p, float* p_tmp, float* div_u,
     if (rc) return rc;
     if (max_div_bits) rc = bs_max_abs_launch(div_u, (long long)n, max_div_bits, st);
     return rc;
+}
+
+// One fluid step's no-slip and projection (pipeline.py:606-620 around
+// fluid.py:612-734, Jacobi solver): enforce_no_slip, pressure_projection,
+// enforce_no_slip, with
+//   - the first no-slip fused into the first divergence pass,
+//   - the second no-slip left out: it would zero exactly the faces that are
+//     on the domain edge or touch a non-fluid cell; the first one has already
+//     set all of them to +0.0 and neither the solver nor the gradient
+//     subtraction (which writes only interior fluid-fluid faces) touches them
+//     afterwards, so it changes no bit,
+//   - max|div| and max|u| (what compute_max_velocity returns once the step
+//     has ended) from the diagnostic divergence pass, as float bits in
+//     max_bits[0] and max_bits[1].
+extern "C" int bs_projection_step_jacobi_launch(float* p, float* p_tmp, float* div_u, float* ux, float* uy,
+                                                float* uz, const int* mat, int nx, int ny, int nz, float dx,
+                                                float dt, float rho, int n_iter, int mat_fluid, int mat_air,
+                                                unsigned int* max_bits, hipStream_t st) {
+    if (nx <= 0 || ny <= 0 || nz <= 0) {
+        zero_u32x2_kernel<<<1, 1, 0, st>>>(max_bits);
+        return (int)hipGetLastError();
+    }
+    const size_t n = (size_t)nx * ny * nz;
+    if (n_iter <= 0) {
+        fill_zero(p, (int64_t)n, st);
+        fill_zero(p_tmp, (int64_t)n, st);
+    }
+    int rc = bs_divergence_no_slip_launch(div_u, ux, uy, uz, mat, nx, ny, nz, dx, mat_fluid, st);
+    if (rc) return rc;
+    rc = bs_jacobi_solve_launch(p, p_tmp, div_u, mat, nx, ny, nz, dx, dt, rho, n_iter, mat_fluid, mat_air, 1, st);
+    if (rc) return rc;
+    rc = bs_subtract_gradient_launch(ux, uy, uz, p, mat, nx, ny, nz, dx, dt, rho, mat_fluid, st);
+    if (rc) return rc;
+    return bs_divergence_max_launch(div_u, ux, uy, uz, mat, nx, ny, nz, dx, mat_fluid, max_bits, st);
 }

@@ -92,6 +92,11 @@ SIGNATURES = {
     "bs_subtract_gradient_launch": [VP, VP, VP, VP, VP, I, I, I, F, F, F, I, S],
     "bs_max_abs_launch": [VP, c_longlong, VP, S],
     "bs_projection_jacobi_launch": [VP, VP, VP, VP, VP, VP, VP, I, I, I, F, F, F, I, I, I, VP, S],
+    "bs_enforce_no_slip_launch": [VP, VP, VP, VP, I, I, I, I, S],
+    "bs_divergence_no_slip_launch": [VP, VP, VP, VP, VP, I, I, I, F, I, S],
+    "bs_divergence_max_launch": [VP, VP, VP, VP, VP, I, I, I, F, I, VP, S],
+    "bs_max_velocity_launch": [VP, VP, VP, I, I, I, VP, S],
+    "bs_projection_step_jacobi_launch": [VP, VP, VP, VP, VP, VP, VP, I, I, I, F, F, F, I, I, I, VP, S],
     "bs_pressure_solve_pcg_launch": [VP, VP, VP, I, I, I, F, F, F, I, I, F, I] + [VP] * 13 + [POINTER(c_int)],
     "bs_pcg_workspace_bytes": [I, I, I],
     "bs_pcg_solve": [VP, VP, VP, I, I, I, F, F, F, I, I, c_double, I, I, I, VP, c_size_t, S, c_void_p],

@@ -139,6 +139,52 @@ class SimCore:
             _ptr("max_div_bits_ptr", max_div_bits_ptr), _ptr("stream", stream),
         )
 
+    # -- no-slip faces and the CFL velocity maximum (fluid.py:111-162, 796-844) ----
+    def enforce_no_slip(self, ux_ptr, uy_ptr, uz_ptr, mat_ptr, nx, ny, nz, mat_fluid=0, stream=0) -> None:
+        """enforce_no_slip (fluid.py:111-162): the three face kernels in one launch."""
+        _lib.call("bs_enforce_no_slip_launch", _ptr("ux_ptr", ux_ptr), _ptr("uy_ptr", uy_ptr),
+                  _ptr("uz_ptr", uz_ptr), _ptr("mat_ptr", mat_ptr), _usize("nx", nx), _usize("ny", ny),
+                  _usize("nz", nz), _i32("mat_fluid", mat_fluid), _ptr("stream", stream))
+
+    def divergence_no_slip(self, div_u_ptr, ux_ptr, uy_ptr, uz_ptr, mat_ptr, nx, ny, nz, dx, mat_fluid=0,
+                           stream=0) -> None:
+        """enforce_no_slip, then compute_divergence of the result, in one pass."""
+        _lib.call("bs_divergence_no_slip_launch", _ptr("div_u_ptr", div_u_ptr), _ptr("ux_ptr", ux_ptr),
+                  _ptr("uy_ptr", uy_ptr), _ptr("uz_ptr", uz_ptr), _ptr("mat_ptr", mat_ptr), _usize("nx", nx),
+                  _usize("ny", ny), _usize("nz", nz), float(dx), _i32("mat_fluid", mat_fluid),
+                  _ptr("stream", stream))
+
+    def divergence_max(self, div_u_ptr, ux_ptr, uy_ptr, uz_ptr, mat_ptr, nx, ny, nz, dx, max_bits_ptr,
+                       mat_fluid=0, stream=0) -> None:
+        """compute_divergence; max|div| and max|u| as float bits in the two
+        uint32s at max_bits_ptr."""
+        _lib.call("bs_divergence_max_launch", _ptr("div_u_ptr", div_u_ptr), _ptr("ux_ptr", ux_ptr),
+                  _ptr("uy_ptr", uy_ptr), _ptr("uz_ptr", uz_ptr), _ptr("mat_ptr", mat_ptr), _usize("nx", nx),
+                  _usize("ny", ny), _usize("nz", nz), float(dx), _i32("mat_fluid", mat_fluid),
+                  _ptr("max_bits_ptr", max_bits_ptr), _ptr("stream", stream))
+
+    def max_velocity(self, ux_ptr, uy_ptr, uz_ptr, nx, ny, nz, out_bits_ptr, stream=0) -> None:
+        """compute_max_velocity's device reduction (fluid.py:807-833); max|u|
+        as float bits in the uint32 at out_bits_ptr."""
+        _lib.call("bs_max_velocity_launch", _ptr("ux_ptr", ux_ptr), _ptr("uy_ptr", uy_ptr),
+                  _ptr("uz_ptr", uz_ptr), _usize("nx", nx), _usize("ny", ny), _usize("nz", nz),
+                  _ptr("out_bits_ptr", out_bits_ptr), _ptr("stream", stream))
+
+    def projection_step_jacobi(self, p_ptr, p_tmp_ptr, div_u_ptr, ux_ptr, uy_ptr, uz_ptr, mat_ptr, nx, ny, nz,
+                               dx, dt, rho, n_iter, max_bits_ptr, mat_fluid=0, mat_air=2, stream=0) -> None:
+        """no-slip, Jacobi pressure_projection, no-slip (pipeline.py:606-620)
+        in four passes plus the sweeps; max|div| and max|u| as float bits in
+        the two uint32s at max_bits_ptr."""
+        _lib.call(
+            "bs_projection_step_jacobi_launch",
+            _ptr("p_ptr", p_ptr), _ptr("p_tmp_ptr", p_tmp_ptr), _ptr("div_u_ptr", div_u_ptr),
+            _ptr("ux_ptr", ux_ptr), _ptr("uy_ptr", uy_ptr), _ptr("uz_ptr", uz_ptr),
+            _ptr("mat_ptr", mat_ptr), _usize("nx", nx), _usize("ny", ny), _usize("nz", nz),
+            float(dx), float(dt), float(rho), _usize("n_iter", n_iter),
+            _i32("mat_fluid", mat_fluid), _i32("mat_air", mat_air),
+            _ptr("max_bits_ptr", max_bits_ptr), _ptr("stream", stream),
+        )
+
     # -- PCG (cuda.rs:368-412) ------------------------------------------------------
     def pressure_solve_pcg(self, p_out_ptr, div_u_ptr, mat_ptr, nx, ny, nz, dx, dt, rho,
                            pressure_tol, max_iter, ws_b_ptr, ws_r_ptr, ws_z_ptr,

@@ -85,6 +85,40 @@ int bs_projection_jacobi_launch(float* p, float* p_tmp, float* div_u, float* ux,
                                 int n_iter, int mat_fluid, int mat_air, unsigned int* max_div_bits,
                                 bs_stream_t stream);
 
+/* ---- no-slip faces and the CFL velocity maximum --------------------------- */
+
+/* enforce_no_slip (python/boilingsim/fluid.py:111-162): one launch for the
+ * three enforce_no_slip_u{x,y,z} kernels.  Faces on the domain edge or next
+ * to a non-fluid cell become +0.0; the others are not written. */
+int bs_enforce_no_slip_launch(float* ux, float* uy, float* uz, const int* mat, int nx, int ny, int nz,
+                              int mat_fluid, bs_stream_t stream);
+/* enforce_no_slip then compute_divergence (pipeline.py:606-608 followed by
+ * fluid.py:632-637) in one pass over the faces. */
+int bs_divergence_no_slip_launch(float* div_u, float* ux, float* uy, float* uz, const int* mat, int nx,
+                                 int ny, int nz, float dx, int mat_fluid, bs_stream_t stream);
+/* compute_divergence plus, from the same pass, max|div| (fluid.py:727-734)
+ * and max|u| over every face (compute_max_velocity, fluid.py:807-833), as
+ * float bits in max_bits[0] and max_bits[1] (two device uint32s). */
+int bs_divergence_max_launch(float* div_u, const float* ux, const float* uy, const float* uz,
+                             const int* mat, int nx, int ny, int nz, float dx, int mat_fluid,
+                             unsigned int* max_bits, bs_stream_t stream);
+/* compute_max_velocity's device reduction (fluid.py:807-833: one zeroing
+ * and three atomic-max launches there) in one launch; max|u| as float bits in
+ * *out_bits (a device uint32). */
+int bs_max_velocity_launch(const float* ux, const float* uy, const float* uz, int nx, int ny, int nz,
+                           unsigned int* out_bits, bs_stream_t stream);
+/* One step's no-slip + projection + no-slip (pipeline.py:606-620) with the
+ * Jacobi solver: the pre no-slip fused into the first divergence, the post
+ * no-slip omitted (it would change no bit: the faces it zeroes were zeroed by
+ * the pre pass and are never written by the solve or the gradient), and
+ * max_bits[0] = max|div| after the correction (pressure_projection's return
+ * value), max_bits[1] = max|u| of the final faces (what compute_max_velocity
+ * returns after the step). */
+int bs_projection_step_jacobi_launch(float* p, float* p_tmp, float* div_u, float* ux, float* uy,
+                                     float* uz, const int* mat, int nx, int ny, int nz, float dx,
+                                     float dt, float rho, int n_iter, int mat_fluid, int mat_air,
+                                     unsigned int* max_bits, bs_stream_t stream);
+
 /* ---- PCG ------------------------------------------------------------------ */
 
 /* Jacobi-preconditioned CG with the reference's control flow and workspace

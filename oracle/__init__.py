@@ -21,6 +21,14 @@ vectors and Warp cannot be imported here):
 * ``laplacian_f64`` / ``pcg_f64`` / ``pcg_ref`` / ``residual_f64`` — scipy
   sparse matrix assembled independently from the class rules, scipy spsolve,
   and the reference's tests' gates (test_pressure_cg.py:49-434).
+* ``enforce_no_slip`` / ``max_velocity`` / ``projection_step`` (the
+  no-slip faces of fluid.py:111-162 and compute_max_velocity, fluid.py:796-833)
+  — bit-exact agreement with the independent numpy restatement
+  (``np_ref.enforce_no_slip``) and with per-face brute-force loops on tiny
+  grids, the reference's own check (test_fluid.py:94-117: every face touching a
+  wall cell is zero), the untouched interior fluid-fluid faces, idempotence,
+  -0.0 becoming +0.0 as the reference's store of 0.0 makes it, and the
+  hydrostatic closed box of test_fluid.py:199-217 (zero velocity stays zero).
 * ``conduction_sweep`` / ``conduction_solve`` (oracle/thermal_ref.c, the
   backward-Euler Jacobi conduction of thermal.py:259-388 and 631-665) —
   closed forms (one solid cell cooled through one air face; a water/steel
@@ -72,6 +80,9 @@ def lib():
         L.orc_max_abs.restype = c_double
         L.orc_projection.argtypes = [P, P, P, P, P, P, P, I, I, I, F, F, F, I, I, I]
         L.orc_projection.restype = c_double
+        L.orc_enforce_no_slip.argtypes = [P, P, P, P, I, I, I, I]
+        L.orc_max_velocity.argtypes = [P, P, P, I, I, I]
+        L.orc_max_velocity.restype = c_double
         L.orc_laplacian_f64.argtypes = [P, P, P, I, I, I, I, I]
         L.orc_diag.argtypes = [P, P, I, I, I, I, I]
         L.orc_rhs_f64.argtypes = [P, P, P, I, I, I, F, F, F, I]
@@ -154,6 +165,32 @@ def projection(ux, uy, uz, mat, dx, dt, rho, n_iter, mat_fluid=MAT_FLUID, mat_ai
     m = lib().orc_projection(_p(p), _p(p_tmp), _p(div), _p(ux), _p(uy), _p(uz), _p(mat), nx, ny, nz,
                              float(dx), float(dt), float(rho), int(n_iter), mat_fluid, mat_air)
     return p, p_tmp, div, float(m)
+
+
+def enforce_no_slip(ux, uy, uz, mat, mat_fluid=MAT_FLUID):
+    """In place on ux, uy, uz (float32 C-contiguous face arrays)."""
+    mat = _i32(mat)
+    nx, ny, nz = mat.shape
+    for a, shp in ((ux, (nx + 1, ny, nz)), (uy, (nx, ny + 1, nz)), (uz, (nx, ny, nz + 1))):
+        assert a.dtype == np.float32 and a.flags.c_contiguous and a.shape == shp
+    lib().orc_enforce_no_slip(_p(ux), _p(uy), _p(uz), _p(mat), nx, ny, nz, mat_fluid)
+
+
+def max_velocity(ux, uy, uz):
+    ux, uy, uz = _f32(ux), _f32(uy), _f32(uz)
+    nx, ny, nz = uy.shape[0], ux.shape[1], ux.shape[2]
+    return float(lib().orc_max_velocity(_p(ux), _p(uy), _p(uz), nx, ny, nz))
+
+
+def projection_step(ux, uy, uz, mat, dx, dt, rho, n_iter, mat_fluid=MAT_FLUID, mat_air=MAT_AIR):
+    """pipeline.py:606-620 with the Jacobi solver, in place on the faces:
+    enforce_no_slip, pressure_projection, enforce_no_slip.  Returns
+    (p, p_tmp, div, max_div, max_u) with max_u = compute_max_velocity of the
+    final faces."""
+    enforce_no_slip(ux, uy, uz, mat, mat_fluid)
+    p, p_tmp, div, md = projection(ux, uy, uz, mat, dx, dt, rho, n_iter, mat_fluid, mat_air)
+    enforce_no_slip(ux, uy, uz, mat, mat_fluid)
+    return p, p_tmp, div, md, max_velocity(ux, uy, uz)
 
 
 def laplacian_f64(x, mat, mat_fluid=MAT_FLUID, mat_air=MAT_AIR):

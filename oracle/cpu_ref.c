@@ -188,6 +188,47 @@ double orc_projection(float* p, float* p_tmp, float* div, float* ux, float* uy, 
     return orc_max_abs(div, n);
 }
 
+/* enforce_no_slip_u{x,y,z} — fluid.py:111-150.  Each face array in its own
+ * loop nest, as the three launches of enforce_no_slip (fluid.py:153-162):
+ * a face on the domain edge, or with a non-fluid cell on either side, is set
+ * to 0.0; the others are left alone. */
+void orc_enforce_no_slip(float* ux, float* uy, float* uz, const int32_t* mat, int nx, int ny, int nz,
+                         int mat_fluid) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i <= nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int64_t f = ((int64_t)i * ny + j) * nz + k;
+                if (i == 0 || i == nx) { ux[f] = 0.0f; continue; }
+                if (mat[CIDX(i - 1, j, k)] != mat_fluid || mat[CIDX(i, j, k)] != mat_fluid) ux[f] = 0.0f;
+            }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j <= ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int64_t f = ((int64_t)i * (ny + 1) + j) * nz + k;
+                if (j == 0 || j == ny) { uy[f] = 0.0f; continue; }
+                if (mat[CIDX(i, j - 1, k)] != mat_fluid || mat[CIDX(i, j, k)] != mat_fluid) uy[f] = 0.0f;
+            }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k <= nz; ++k) {
+                const int64_t f = ((int64_t)i * ny + j) * (nz + 1) + k;
+                if (k == 0 || k == nz) { uz[f] = 0.0f; continue; }
+                if (mat[CIDX(i, j, k - 1)] != mat_fluid || mat[CIDX(i, j, k)] != mat_fluid) uz[f] = 0.0f;
+            }
+}
+
+/* compute_max_velocity — fluid.py:830-833 (the host path): the largest |u|
+ * over the three face arrays. */
+double orc_max_velocity(const float* ux, const float* uy, const float* uz, int nx, int ny, int nz) {
+    const double a = orc_max_abs(ux, (int64_t)(nx + 1) * ny * nz);
+    const double b = orc_max_abs(uy, (int64_t)nx * (ny + 1) * nz);
+    const double c = orc_max_abs(uz, (int64_t)nx * ny * (nz + 1));
+    return a > b ? (a > c ? a : c) : (b > c ? b : c);
+}
+
 /* ---------------------------------------------------------------------------
  * Linear-system view used by the PCG (crates/cuda-kernels/src/
  * laplacian_spmv.cu:60-143, diag_inverse_apply.cu:43-89,

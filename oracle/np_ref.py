@@ -8,6 +8,7 @@ before use, exactly as Warp converts its ``float`` kernel arguments.
 * jacobi_step: python/boilingsim/fluid.py:222-276
 * divergence:  fluid.py:201-219
 * gradient:    fluid.py:279-330
+* enforce_no_slip: fluid.py:111-150, as masks of padded arrays
 * laplacian_csr: the operator of crates/cuda-kernels/src/laplacian_spmv.cu:60-143
   as a scipy sparse matrix (FP64), assembled from the class rules.
 """
@@ -114,3 +115,17 @@ def laplacian_csr(mat, mat_fluid=0, mat_air=2):
     vals.append(diag[fl])
     A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
     return A
+
+
+def enforce_no_slip(ux, uy, uz, mat, mat_fluid=0):
+    """Returns new (ux, uy, uz): a face keeps its value only when the cells on
+    both sides exist and are fluid.  The mat array is padded with a non-fluid
+    ring, so the domain-edge faces fall out of the same rule."""
+    mat = np.asarray(mat, dtype=np.int32)
+    fl = np.pad(mat == mat_fluid, 1, mode="constant", constant_values=False)
+    keep_x = fl[:-1, 1:-1, 1:-1] & fl[1:, 1:-1, 1:-1]
+    keep_y = fl[1:-1, :-1, 1:-1] & fl[1:-1, 1:, 1:-1]
+    keep_z = fl[1:-1, 1:-1, :-1] & fl[1:-1, 1:-1, 1:]
+    z = f32(0.0)
+    return (np.where(keep_x, np.asarray(ux, f32), z), np.where(keep_y, np.asarray(uy, f32), z),
+            np.where(keep_z, np.asarray(uz, f32), z))

@@ -161,3 +161,40 @@ def test_two_sweep_pass_bit_exact(simcore, torch_gpu, shape, n_iter, edge):
     torch.cuda.synchronize()
     assert same_bits(host(p), p_ref), bit_diff_report(host(p), p_ref)
     assert same_bits(host(tmp), tmp_ref), bit_diff_report(host(tmp), tmp_ref)
+
+
+@pytest.mark.parametrize("n_iter", [1, 2, 3])
+def test_projection_zero_divergence_cells_keep_positive_zero(simcore, torch_gpu, n_iter):
+    """The projection's cold start (p = 0, first sweep without reading p) on
+    fluid cells whose divergence is exactly +0 or -0: (s - rhs)/6 with s = +0
+    is +0 for both, bit for bit as the oracle, not -0."""
+    torch = torch_gpu
+    rng = np.random.default_rng(4)
+    shape = (12, 10, 16)
+    mat = rng.choice([0, 1, 2], size=shape, p=[0.7, 0.2, 0.1]).astype(np.int32)
+    nx, ny, nz = shape
+    f = [(rng.standard_normal(s) * 0.05).astype(np.float32)
+         for s in ((nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1))]
+    # A block of zero faces, -0 on odd and +0 on even face indices along each
+    # face's axis: cells there get div = (-0 - +0)/dx + ... = -0 (even cells)
+    # or +0.
+    for q, a in enumerate(f):
+        blk = a[3:9, 2:9, 4:13]
+        idx = np.indices(blk.shape)[q] + (3, 2, 4)[q]
+        blk[...] = np.where(idx % 2 == 1, np.float32(-0.0), np.float32(0.0))
+    div0 = oracle.divergence(*f, mat, 2e-3)
+    zero = (mat == 0) & (div0 == 0)
+    assert zero.sum() > 20 and (zero & np.signbit(div0)).sum() > 5 and (zero & ~np.signbit(div0)).sum() > 5
+    ref = [a.copy() for a in f]
+    p_r, pt_r, div_r, m_r = oracle.projection(*ref, mat, 2e-3, 0.01, 997.0, n_iter)
+    d = [dev(torch, a) for a in f]
+    md = dev(torch, mat)
+    p = torch.full(shape, 3.0, dtype=torch.float32, device="cuda")
+    pt = torch.full(shape, 4.0, dtype=torch.float32, device="cuda")
+    div = torch.empty(shape, dtype=torch.float32, device="cuda")
+    mb = torch.zeros(1, dtype=torch.int32, device="cuda")
+    simcore.projection_jacobi(p.data_ptr(), pt.data_ptr(), div.data_ptr(), *(t.data_ptr() for t in d),
+                              md.data_ptr(), *shape, 2e-3, 0.01, 997.0, n_iter, mb.data_ptr())
+    torch.cuda.synchronize()
+    for g, r in ((host(p), p_r), (host(pt), pt_r), (host(div), div_r), *((host(t), r) for t, r in zip(d, ref))):
+        assert same_bits(g, r), bit_diff_report(g, r)
