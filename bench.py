@@ -66,7 +66,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-ablation", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=12, help="oracle PCG iterations in the CPU sample")
-    ap.add_argument("--legs", default="learned,pcg,ablation,jacobi,mixed,thermal,faces,sequence,cpu",
+    ap.add_argument("--legs", default="learned,pcg,ablation,jacobi,mixed,thermal,faces,advection,sequence,cpu",
                     help="comma list of the legs to run (the JSON fields of skipped legs are null)")
     ap.add_argument("--seq-frames", type=int, default=6,
                     help="frames of the warm-started AR(1) sequence leg (0: skip; the full run is "
@@ -221,6 +221,46 @@ def faces_leg(core, torch, gmat, ux0, uy0, uz0, gux, guy, guz, div, cfg, nx, ny,
     return res
 
 
+def advection_leg(core, torch, gmat, mat, ux0, uy0, uz0, cfg, nx, ny, nz, n, reps=10):
+    """Semi-Lagrangian advection (fluid.py:737-775) of the seed-11 velocities
+    and a temperature field with a hot pot base, the geometry's origin and
+    the CFL-limited dt (0.4 dx / max|u|), each pass timed with HIP events on
+    the null stream over `reps` launches.  Algorithmic bytes of the advection
+    pass: ux, uy, uz, T_ext, T and mat read once, four outputs written =
+    40 B/cell (the 32 gathers per output hit L1/L2)."""
+    ptr = lambda t: t.data_ptr()
+    t0 = np.full(cfg.shape, 295.15, np.float32)
+    t0[(mat == 1) & (np.arange(nz)[None, None, :] < nz // 6)] = 480.0
+    T = torch.from_numpy(t0).cuda()
+    outs = [torch.empty_like(a) for a in (ux0, uy0, uz0, T)]
+    t_ext = torch.empty_like(T)
+    origin = (-nx * cfg.dx / 2, -ny * cfg.dx / 2, -4 * cfg.dx)
+    umax = max(float(a.abs().max()) for a in (ux0, uy0, uz0))
+    dt = 0.4 * cfg.dx / umax
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e3
+
+    ext = lambda: core.extend_temperature(ptr(t_ext), ptr(T), ptr(gmat), nx, ny, nz)
+    adv = lambda: core.advect(*(ptr(o) for o in outs), ptr(ux0), ptr(uy0), ptr(uz0), ptr(T), ptr(t_ext), ptr(gmat),
+                              nx, ny, nz, origin, cfg.dx, dt)
+    us_ext = timed(ext)
+    us_adv = timed(adv)
+    gbs = 40.0 * n / (us_adv * 1e-6) / 1e9
+    log(f"advection: {us_adv:.1f} us per advect pass ({gbs:.0f} GB/s algorithmic, 40 B/cell), "
+        f"extension {us_ext:.1f} us, dt {dt * 1e3:.3f} ms (CFL 0.4)")
+    return {"us_advect": us_adv, "us_extend": us_ext, "achieved_gbs": gbs, "frac_of_peak": gbs / HBM_PEAK_GBS,
+            "bytes_per_cell": 40.0, "dt_s": dt, "kernel": "advect_kernel (u_x, u_y, u_z and T in one pass)"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -330,6 +370,9 @@ def main():
     thermal = None
     if "thermal" in legs:
         thermal = conduction_leg(core, torch, gmat, mat, cfg, nx, ny, nz, n)
+    adv = None
+    if "advection" in legs:
+        adv = advection_leg(core, torch, gmat, mat, ux0, uy0, uz0, cfg, nx, ny, nz, n)
     faces = None
     if "faces" in legs:
         faces = faces_leg(core, torch, gmat, ux0, uy0, uz0, gux, guy, guz, div, cfg, nx, ny, nz, n)
@@ -427,6 +470,7 @@ def main():
         "mixed_precision_refinement": mixed,
         "thermal_conduction": thermal,
         "no_slip_faces": faces,
+        "advection": adv,
         "sequence_ar1": seq,
         "cpu_baseline": cpu,
     }

@@ -64,6 +64,8 @@ class Grid:
     ux: object
     uy: object
     uz: object
+    T: object = None  # cell temperature (K), advected with the velocity
+    origin: tuple = (0.0, 0.0, 0.0)  # world position of cell corner (0,0,0), m
 
     @property
     def shape(self) -> tuple[int, int, int]:
@@ -78,16 +80,23 @@ class FluidWorkspace:
     p_tmp: object
     max_bits: object  # int32[2]: max|div| and max|u| as float bits
     u_max: float = 0.0  # max|u| after the last projection_step
+    # advect_all's scratch (fluid.py:505-524), allocated on first use
+    ux_tmp: object = None
+    uy_tmp: object = None
+    uz_tmp: object = None
+    T_tmp: object = None
+    T_ext: object = None
 
 
 def make_grid(mat: np.ndarray, ux: np.ndarray, uy: np.ndarray, uz: np.ndarray, dx: float,
-              device: str = "cuda") -> Grid:
+              device: str = "cuda", T: np.ndarray | None = None, origin=(0.0, 0.0, 0.0)) -> Grid:
     import torch
 
     nx, ny, nz = mat.shape
     t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(device)
     return Grid(nx, ny, nz, float(dx), torch.zeros((nx, ny, nz), dtype=torch.float32, device=device),
-                t(mat, np.int32), t(ux, np.float32), t(uy, np.float32), t(uz, np.float32))
+                t(mat, np.int32), t(ux, np.float32), t(uy, np.float32), t(uz, np.float32),
+                None if T is None else t(T, np.float32), tuple(float(c) for c in origin))
 
 
 def allocate_fluid_workspace(grid: Grid) -> FluidWorkspace:
@@ -302,3 +311,21 @@ def projection_step(grid: Grid, ws: FluidWorkspace, cfg: ScenarioConfig, dt: flo
     m = _bits_to_floats(ws.max_bits)
     ws.u_max = float(m[1])
     return float(m[0])
+
+
+def advect_all(grid: Grid, ws: FluidWorkspace, dt: float, device: str = "cuda") -> None:
+    """Semi-Lagrangian advection of u and T (fluid.py:737-775): the extended
+    temperature, the four advections in one launch into the workspace, and
+    the results copied back so grid.ux/uy/uz/T hold them."""
+    import torch
+
+    core = _get_sim_core()
+    if grid.T is None:
+        raise ValueError("advect_all needs grid.T (the cell temperature)")
+    if ws.ux_tmp is None:
+        z = lambda a: torch.empty_like(a)
+        ws.ux_tmp, ws.uy_tmp, ws.uz_tmp = z(grid.ux), z(grid.uy), z(grid.uz)
+        ws.T_tmp, ws.T_ext = z(grid.T), z(grid.T)
+    core.advect_all(grid.ux.data_ptr(), grid.uy.data_ptr(), grid.uz.data_ptr(), grid.T.data_ptr(),
+                    ws.ux_tmp.data_ptr(), ws.uy_tmp.data_ptr(), ws.uz_tmp.data_ptr(), ws.T_tmp.data_ptr(),
+                    ws.T_ext.data_ptr(), grid.mat.data_ptr(), *grid.shape, grid.origin, grid.dx, dt, MAT_FLUID)

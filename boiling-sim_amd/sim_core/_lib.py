@@ -93,6 +93,9 @@ SIGNATURES = {
     "bs_max_abs_launch": [VP, c_longlong, VP, S],
     "bs_projection_jacobi_launch": [VP, VP, VP, VP, VP, VP, VP, I, I, I, F, F, F, I, I, I, VP, S],
     "bs_enforce_no_slip_launch": [VP, VP, VP, VP, I, I, I, I, S],
+    "bs_extend_temperature_launch": [VP, VP, VP, I, I, I, I, S],
+    "bs_advect_launch": [VP] * 10 + [I, I, I, F, F, F, F, F, I, S],
+    "bs_advect_all_launch": [VP] * 10 + [I, I, I, F, F, F, F, F, I, S],
     "bs_divergence_no_slip_launch": [VP, VP, VP, VP, VP, I, I, I, F, I, S],
     "bs_divergence_max_launch": [VP, VP, VP, VP, VP, I, I, I, F, I, VP, S],
     "bs_max_velocity_launch": [VP, VP, VP, I, I, I, VP, S],

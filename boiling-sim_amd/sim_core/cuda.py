@@ -185,6 +185,35 @@ class SimCore:
             _ptr("max_bits_ptr", max_bits_ptr), _ptr("stream", stream),
         )
 
+    # -- semi-Lagrangian advection (fluid.py:33-103, 338-492, 737-775) -------------
+    def extend_temperature(self, t_ext_ptr, t_ptr, mat_ptr, nx, ny, nz, mat_fluid=0, stream=0) -> None:
+        """extend_temperature_into_solids (fluid.py:338-378)."""
+        _lib.call("bs_extend_temperature_launch", _ptr("t_ext_ptr", t_ext_ptr), _ptr("t_ptr", t_ptr),
+                  _ptr("mat_ptr", mat_ptr), _usize("nx", nx), _usize("ny", ny), _usize("nz", nz),
+                  _i32("mat_fluid", mat_fluid), _ptr("stream", stream))
+
+    def advect(self, ux_new_ptr, uy_new_ptr, uz_new_ptr, t_new_ptr, ux_ptr, uy_ptr, uz_ptr, t_ptr, t_ext_ptr,
+               mat_ptr, nx, ny, nz, origin, dx, dt, mat_fluid=0, stream=0) -> None:
+        """advect_u{x,y,z} and advect_temperature (fluid.py:381-492) in one launch."""
+        ox, oy, oz = (float(c) for c in origin)
+        _lib.call("bs_advect_launch", *(_ptr(n, v) for n, v in (
+            ("ux_new_ptr", ux_new_ptr), ("uy_new_ptr", uy_new_ptr), ("uz_new_ptr", uz_new_ptr),
+            ("t_new_ptr", t_new_ptr), ("ux_ptr", ux_ptr), ("uy_ptr", uy_ptr), ("uz_ptr", uz_ptr), ("t_ptr", t_ptr),
+            ("t_ext_ptr", t_ext_ptr), ("mat_ptr", mat_ptr))),
+            _usize("nx", nx), _usize("ny", ny), _usize("nz", nz), ox, oy, oz, float(dx), float(dt),
+            _i32("mat_fluid", mat_fluid), _ptr("stream", stream))
+
+    def advect_all(self, ux_ptr, uy_ptr, uz_ptr, t_ptr, ux_tmp_ptr, uy_tmp_ptr, uz_tmp_ptr, t_tmp_ptr, t_ext_ptr,
+                   mat_ptr, nx, ny, nz, origin, dx, dt, mat_fluid=0, stream=0) -> None:
+        """advect_all (fluid.py:737-775): the advected fields end in ux, uy, uz, t."""
+        ox, oy, oz = (float(c) for c in origin)
+        _lib.call("bs_advect_all_launch", *(_ptr(n, v) for n, v in (
+            ("ux_ptr", ux_ptr), ("uy_ptr", uy_ptr), ("uz_ptr", uz_ptr), ("t_ptr", t_ptr),
+            ("ux_tmp_ptr", ux_tmp_ptr), ("uy_tmp_ptr", uy_tmp_ptr), ("uz_tmp_ptr", uz_tmp_ptr),
+            ("t_tmp_ptr", t_tmp_ptr), ("t_ext_ptr", t_ext_ptr), ("mat_ptr", mat_ptr))),
+            _usize("nx", nx), _usize("ny", ny), _usize("nz", nz), ox, oy, oz, float(dx), float(dt),
+            _i32("mat_fluid", mat_fluid), _ptr("stream", stream))
+
     # -- PCG (cuda.rs:368-412) ------------------------------------------------------
     def pressure_solve_pcg(self, p_out_ptr, div_u_ptr, mat_ptr, nx, ny, nz, dx, dt, rho,
                            pressure_tol, max_iter, ws_b_ptr, ws_r_ptr, ws_z_ptr,

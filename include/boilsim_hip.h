@@ -119,6 +119,29 @@ int bs_projection_step_jacobi_launch(float* p, float* p_tmp, float* div_u, float
                                      float dt, float rho, int n_iter, int mat_fluid, int mat_air,
                                      unsigned int* max_bits, bs_stream_t stream);
 
+/* ---- semi-Lagrangian advection -------------------------------------------- */
+
+/* extend_temperature_into_solids (python/boilingsim/fluid.py:338-378): each
+ * non-fluid cell takes T of its first fluid neighbour (order i-1, i+1, j-1,
+ * j+1, k-1, k+1), fluid cells and isolated cells keep their own. */
+int bs_extend_temperature_launch(float* t_ext, const float* t, const int* mat, int nx, int ny, int nz,
+                                 int mat_fluid, bs_stream_t stream);
+/* advect_ux, advect_uy, advect_uz and advect_temperature (fluid.py:381-492)
+ * in one launch: face and cell values traced back along the trilinearly
+ * sampled velocity, written to the *_new arrays (disjoint from the inputs);
+ * t_ext from bs_extend_temperature_launch; origin (ox, oy, oz) and dx as
+ * float32.  Grids whose face arrays exceed 2^31 elements are rejected. */
+int bs_advect_launch(float* ux_new, float* uy_new, float* uz_new, float* t_new, const float* ux,
+                     const float* uy, const float* uz, const float* t, const float* t_ext, const int* mat,
+                     int nx, int ny, int nz, float ox, float oy, float oz, float dx, float dt, int mat_fluid,
+                     bs_stream_t stream);
+/* advect_all (fluid.py:737-775): extension, advection into the workspace
+ * arrays (the reference's ws.ux_tmp, uy_tmp, uz_tmp, T_tmp, T_ext), and the
+ * copies back into ux, uy, uz, t. */
+int bs_advect_all_launch(float* ux, float* uy, float* uz, float* t, float* ux_tmp, float* uy_tmp, float* uz_tmp,
+                         float* t_tmp, float* t_ext, const int* mat, int nx, int ny, int nz, float ox, float oy,
+                         float oz, float dx, float dt, int mat_fluid, bs_stream_t stream);
+
 /* ---- PCG ------------------------------------------------------------------ */
 
 /* Jacobi-preconditioned CG with the reference's control flow and workspace

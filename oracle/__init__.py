@@ -29,6 +29,15 @@ vectors and Warp cannot be imported here):
   wall cell is zero), the untouched interior fluid-fluid faces, idempotence,
   -0.0 becoming +0.0 as the reference's store of 0.0 makes it, and the
   hydrostatic closed box of test_fluid.py:199-217 (zero velocity stays zero).
+* ``extend_temperature`` / ``advect`` (oracle/advect_ref.c, the
+  semi-Lagrangian advection of fluid.py:33-103, 338-492, 737-775) —
+  bit-exact agreement with the independent vectorised numpy restatement
+  (``np_ref.advect``); per-cell brute force of the extension's neighbour
+  order; zero velocity on a power-of-two grid returns the fields unchanged;
+  a uniform velocity moving a whole number of cells shifts them exactly; a
+  linear field under a uniform sub-cell velocity is reproduced to rounding
+  (trilinear interpolation is exact on linear fields); the reference's
+  20-step temperature-drift gate (test_fluid.py:147-184).
 * ``conduction_sweep`` / ``conduction_solve`` (oracle/thermal_ref.c, the
   backward-Euler Jacobi conduction of thermal.py:259-388 and 631-665) —
   closed forms (one solid cell cooled through one air face; a water/steel
@@ -81,6 +90,8 @@ def lib():
         L.orc_projection.argtypes = [P, P, P, P, P, P, P, I, I, I, F, F, F, I, I, I]
         L.orc_projection.restype = c_double
         L.orc_enforce_no_slip.argtypes = [P, P, P, P, I, I, I, I]
+        L.orc_extend_temperature.argtypes = [P, P, P, I, I, I, I]
+        L.orc_advect.argtypes = [P, P, P, P, P, P, P, P, P, P, I, I, I, F, F, F, F, F, I]
         L.orc_max_velocity.argtypes = [P, P, P, I, I, I]
         L.orc_max_velocity.restype = c_double
         L.orc_laplacian_f64.argtypes = [P, P, P, I, I, I, I, I]
@@ -191,6 +202,28 @@ def projection_step(ux, uy, uz, mat, dx, dt, rho, n_iter, mat_fluid=MAT_FLUID, m
     p, p_tmp, div, md = projection(ux, uy, uz, mat, dx, dt, rho, n_iter, mat_fluid, mat_air)
     enforce_no_slip(ux, uy, uz, mat, mat_fluid)
     return p, p_tmp, div, md, max_velocity(ux, uy, uz)
+
+
+def extend_temperature(t, mat, mat_fluid=MAT_FLUID):
+    mat = _i32(mat)
+    nx, ny, nz = mat.shape
+    t = _f32(t)
+    out = np.empty(mat.shape, np.float32)
+    lib().orc_extend_temperature(_p(out), _p(t), _p(mat), nx, ny, nz, mat_fluid)
+    return out
+
+
+def advect(ux, uy, uz, t, mat, origin, dx, dt, mat_fluid=MAT_FLUID):
+    """advect_all (fluid.py:737-775): returns (ux_new, uy_new, uz_new, t_new, t_ext)."""
+    mat = _i32(mat)
+    nx, ny, nz = mat.shape
+    ux, uy, uz, t = _f32(ux), _f32(uy), _f32(uz), _f32(t)
+    assert ux.shape == (nx + 1, ny, nz) and uy.shape == (nx, ny + 1, nz) and uz.shape == (nx, ny, nz + 1)
+    t_ext = extend_temperature(t, mat, mat_fluid)
+    out = [np.empty_like(a) for a in (ux, uy, uz, t)]
+    lib().orc_advect(*(_p(a) for a in out), _p(ux), _p(uy), _p(uz), _p(t), _p(t_ext), _p(mat), nx, ny, nz,
+                     *(float(c) for c in origin), float(dx), float(dt), mat_fluid)
+    return (*out, t_ext)
 
 
 def laplacian_f64(x, mat, mat_fluid=MAT_FLUID, mat_air=MAT_AIR):

@@ -9,6 +9,8 @@ before use, exactly as Warp converts its ``float`` kernel arguments.
 * divergence:  fluid.py:201-219
 * gradient:    fluid.py:279-330
 * enforce_no_slip: fluid.py:111-150, as masks of padded arrays
+* extend_temperature / advect: fluid.py:33-103, 338-492, 737-775, whole
+  output arrays at a time
 * laplacian_csr: the operator of crates/cuda-kernels/src/laplacian_spmv.cu:60-143
   as a scipy sparse matrix (FP64), assembled from the class rules.
 """
@@ -129,3 +131,88 @@ def enforce_no_slip(ux, uy, uz, mat, mat_fluid=0):
     z = f32(0.0)
     return (np.where(keep_x, np.asarray(ux, f32), z), np.where(keep_y, np.asarray(uy, f32), z),
             np.where(keep_z, np.asarray(uz, f32), z))
+
+
+def extend_temperature(t, mat, mat_fluid=0):
+    """Non-fluid cells take T of the first fluid neighbour in the order
+    i-1, i+1, j-1, j+1, k-1, k+1 (none: their own T)."""
+    t = np.asarray(t, f32)
+    fl = np.asarray(mat) == mat_fluid
+    out = t.copy()
+    done = fl.copy()
+    tp = np.pad(t, 1)
+    fp = np.pad(fl, 1, constant_values=False)
+    c = (slice(1, -1),) * 3
+    for ax, d in ((0, -1), (0, 1), (1, -1), (1, 1), (2, -1), (2, 1)):
+        sl = list(c)
+        sl[ax] = slice(1 + d, tp.shape[ax] - 1 + d)
+        sl = tuple(sl)
+        take = ~done & fp[sl]
+        out[take] = tp[sl][take]
+        done |= take
+    return out
+
+
+def _tri(field, fx, fy, fz):
+    n0, n1, n2 = field.shape
+    fx = np.fmin(np.fmax(f32(0.0), fx), f32(n0 - 1) - f32(1.0e-6))
+    fy = np.fmin(np.fmax(f32(0.0), fy), f32(n1 - 1) - f32(1.0e-6))
+    fz = np.fmin(np.fmax(f32(0.0), fz), f32(n2 - 1) - f32(1.0e-6))
+    i0, j0, k0 = fx.astype(np.int64), fy.astype(np.int64), fz.astype(np.int64)
+    tx, ty, tz = fx - i0.astype(f32), fy - j0.astype(f32), fz - k0.astype(f32)
+    i1, j1, k1 = np.minimum(i0 + 1, n0 - 1), np.minimum(j0 + 1, n1 - 1), np.minimum(k0 + 1, n2 - 1)
+    one = f32(1.0)
+    c00 = field[i0, j0, k0] * (one - tx) + field[i1, j0, k0] * tx
+    c10 = field[i0, j1, k0] * (one - tx) + field[i1, j1, k0] * tx
+    c01 = field[i0, j0, k1] * (one - tx) + field[i1, j0, k1] * tx
+    c11 = field[i0, j1, k1] * (one - tx) + field[i1, j1, k1] * tx
+    c0 = c00 * (one - ty) + c10 * ty
+    c1 = c01 * (one - ty) + c11 * ty
+    return c0 * (one - tz) + c1 * tz
+
+
+def _coords(px, py, pz, o, dx, off):
+    dx = f32(dx)
+    return ((px - o[0]) / dx - f32(off[0]), (py - o[1]) / dx - f32(off[1]), (pz - o[2]) / dx - f32(off[2]))
+
+
+_OFFS = {"ux": (0.0, 0.5, 0.5), "uy": (0.5, 0.0, 0.5), "uz": (0.5, 0.5, 0.0), "t": (0.5, 0.5, 0.5)}
+
+
+def advect(ux, uy, uz, t, mat, origin, dx, dt, mat_fluid=0):
+    """Returns (ux_new, uy_new, uz_new, t_new)."""
+    ux, uy, uz, t = (np.asarray(a, f32) for a in (ux, uy, uz, t))
+    mat = np.asarray(mat)
+    nx, ny, nz = mat.shape
+    o = tuple(f32(c) for c in origin)
+    dx, dt = f32(dx), f32(dt)
+    t_ext = extend_temperature(t, mat, mat_fluid)
+    fl = mat == mat_fluid
+    fields = {"ux": ux, "uy": uy, "uz": uz, "t": t_ext}
+    out = {}
+    for name, shape in (("ux", ux.shape), ("uy", uy.shape), ("uz", uz.shape), ("t", t.shape)):
+        off = _OFFS[name]
+        I, J, K = (g.astype(f32) for g in np.indices(shape))
+        # the output point: integer index on the face axis, +0.5 elsewhere
+        px = o[0] + (I + f32(off[0])) * dx
+        py = o[1] + (J + f32(off[1])) * dx
+        pz = o[2] + (K + f32(off[2])) * dx
+        u = _tri(ux, *_coords(px, py, pz, o, dx, _OFFS["ux"]))
+        v = _tri(uy, *_coords(px, py, pz, o, dx, _OFFS["uy"]))
+        w = _tri(uz, *_coords(px, py, pz, o, dx, _OFFS["uz"]))
+        bx, by, bz = px - u * dt, py - v * dt, pz - w * dt
+        val = _tri(fields[name], *_coords(bx, by, bz, o, dx, off))
+        if name == "t":
+            out[name] = np.where(fl, val, t)
+            continue
+        ax = "xyz".index(name[1])
+        src = fields[name]
+        fpad = np.pad(fl, [(1, 1) if a == ax else (0, 0) for a in range(3)], constant_values=False)
+        lo = fpad[tuple(slice(0, -1) if a == ax else slice(None) for a in range(3))]
+        hi = fpad[tuple(slice(1, None) if a == ax else slice(None) for a in range(3))]
+        res = np.where(lo | hi, val, src)
+        edge = np.zeros(shape, bool)
+        edge[tuple(slice(0, 1) if a == ax else slice(None) for a in range(3))] = True
+        edge[tuple(slice(-1, None) if a == ax else slice(None) for a in range(3))] = True
+        out[name] = np.where(edge, f32(0.0), res).astype(f32)
+    return out["ux"], out["uy"], out["uz"], out["t"]
