@@ -69,6 +69,20 @@ __device__ __forceinline__ bool one_tile(const Tiles& T, int VEC, int& i, int& j
     return true;
 }
 
+// VEC consecutive int32 values (material ids).
+template <int VEC>
+__device__ __forceinline__ void ldi(const int* __restrict__ p, int64_t off, int (&v)[VEC]) {
+    if constexpr (VEC == 4) {
+        const int4 t = *reinterpret_cast<const int4*>(p + off);
+        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+    } else if constexpr (VEC == 2) {
+        const int2 t = *reinterpret_cast<const int2*>(p + off);
+        v[0] = t.x; v[1] = t.y;
+    } else {
+        v[0] = p[off];
+    }
+}
+
 template <int VEC>
 __device__ __forceinline__ void ldf(const float* __restrict__ p, int64_t off, float (&v)[VEC]) {
     if constexpr (VEC == 8) {

@@ -54,19 +54,6 @@ __device__ __forceinline__ void build_table(CondTable& tb, const float* __restri
     }
 }
 
-template <int VEC>
-__device__ __forceinline__ void ldi(const int* __restrict__ p, int64_t off, int (&v)[VEC]) {
-    if constexpr (VEC == 4) {
-        const int4 t = *reinterpret_cast<const int4*>(p + off);
-        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
-    } else if constexpr (VEC == 2) {
-        const int2 t = *reinterpret_cast<const int2*>(p + off);
-        v[0] = t.x; v[1] = t.y;
-    } else {
-        v[0] = p[off];
-    }
-}
-
 // Material ids index the table; ids outside [0, n_mat) (an out-of-bounds read
 // in the reference) are clamped to 0 to keep the lookup inside it.
 __device__ __forceinline__ int mid(int m, int nm) { return (unsigned)m < (unsigned)nm ? m : 0; }
