@@ -77,3 +77,40 @@ def bit_diff_report(a, b) -> str:
     idx = np.argwhere(bad)[:5]
     return (f"{int(bad.sum())} cells differ; first {idx.tolist()}: "
             f"{a[tuple(idx[0])]!r} vs {b[tuple(idx[0])]!r}; max abs diff {np.abs(a - b).max():.3e}")
+
+
+class OracleRun:
+    """The oracle's counterpart of boilsim.regression.Run: advection,
+    backward-Euler conduction and no-slip + Jacobi projection + no-slip per
+    step, on host arrays.  The conduction's ping-pong partner and snapshot
+    persist across steps like the device workspace (an odd sweep count copies
+    the partner into T, air cells included, as conduct_one_step does)."""
+
+    def __init__(self, state: dict, cfg):
+        import oracle
+        from boilsim.thermal import MaterialProps
+
+        self.oracle = oracle
+        self.cfg = cfg
+        self.mat = np.ascontiguousarray(state["mat"], np.int32)
+        self.f = {k: np.array(state[k], np.float32) for k in ("ux", "uy", "uz", "T")}
+        self.origin, self.dx = state["origin"], state["dx"]
+        self.t_alt = np.zeros_like(self.f["T"])
+        self.t_old = np.zeros_like(self.f["T"])
+        self.props = MaterialProps.from_pot(cfg.pot_material)
+        self.p = np.zeros_like(self.f["T"])
+        self.div = np.zeros_like(self.f["T"])
+        self.max_div = self.u_max = float("nan")
+
+    def step(self):
+        o, c, f = self.oracle, self.cfg, self.f
+        f["ux"], f["uy"], f["uz"], f["T"], _ = o.advect(f["ux"], f["uy"], f["uz"], f["T"], self.mat, self.origin,
+                                                        self.dx, c.dt)
+        o.conduction_solve(f["T"], self.t_alt, self.t_old, self.mat, self.props.k, self.props.rho, self.props.c_p,
+                           self.dx, c.dt, c.h_conv, c.ambient_c + 273.15, c.n_cond)
+        self.p, _, self.div, self.max_div, self.u_max = o.projection_step(
+            f["ux"], f["uy"], f["uz"], self.mat, self.dx, c.dt, c.rho, int(c.solver.pressure_max_iter))
+
+    def fields(self) -> dict:
+        return {"mat": self.mat, "p": self.p, "ux": self.f["ux"], "uy": self.f["uy"], "uz": self.f["uz"],
+                "T": self.f["T"], "div_u": self.div}
