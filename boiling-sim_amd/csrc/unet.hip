@@ -23,9 +23,9 @@
 // Y^T[Cout x cells] = W^T[Cout x 27*Cin] * X^T[27*Cin x cells]: the A operand
 // is the weight matrix, pre-packed once per weight update into the exact
 // per-lane fragment order (16 contiguous bytes per lane), and the B operand is
-// gathered straight from the activation tensor — lane (16*g + c) loads the 8
-// channels of tap/channel-group g for cell c.  A wave computes 16 output
-// channels for NT x 16 consecutive cells per step.  The input-gradient
+// read from a workgroup's input block staged in LDS — lane (16*g + c) takes
+// the 8 channels of tap/channel-group g for cell c.  A wave computes 16
+// output channels for 4 rows x 16 consecutive k cells per step.  The input-gradient
 // convolution is the same kernel with flipped, transposed weights; weight
 // gradients are FP32 outer-product reductions with deterministic partials.
 #include "boilsim_hip.h"
@@ -116,74 +116,96 @@ __device__ __forceinline__ void cell_ijk(const Lvl& L, uint32_t c, int& i, int& 
 // Implicit-GEMM convolution on MFMA
 //   out[cell][co] = act(sum_{t,ci} in[nbr(cell,t)][ci] * W[t][ci][co] + b[co])
 //   (+ skip[cell][co] into out_sum when given)
-// in is zero outside the grid.  NT 16-cell tiles per wave.
+// in is zero outside the grid.
+//
+// A workgroup owns a 4 x 4 x 16 block of cells (i, j, k): wave w the plane
+// i0 + w, its four 16-cell MFMA tiles the rows j0..j0+3.  The block's input
+// with a one-cell halo (6 x 6 x 18 cells, zeros outside the grid) is staged
+// in LDS once, so the 27 taps of every cell are read from LDS instead of
+// being gathered from L1/L2 27 times (the gathers bounded the earlier
+// version).  LDS layout [channel group][halo cell][8 channels]: the 16 lanes
+// of one B fragment read 16 consecutive 16-byte slots, without bank
+// conflicts for any channel count.
 // ---------------------------------------------------------------------------
-template <int CIN, int COUT, int NT>
+constexpr int kTI = 4, kTJ = 4, kTK = 16;
+constexpr int kHI = kTI + 2, kHJ = kTJ + 2, kHK = kTK + 2;
+constexpr int kHalo = kHI * kHJ * kHK;
+
+struct ConvTiles {
+    uint32_t ti, tj, tk;  // blocks per axis
+};
+
+inline ConvTiles conv_tiles(const Lvl& L) {
+    return ConvTiles{(uint32_t)((L.nx + kTI - 1) / kTI), (uint32_t)((L.ny + kTJ - 1) / kTJ),
+                     (uint32_t)((L.nz + kTK - 1) / kTK)};
+}
+
+template <int CIN, int COUT>
 __global__ __launch_bounds__(256) void conv_mfma_kernel(
-    Lvl L, const __bf16* __restrict__ in, const bf16x8* __restrict__ wfrag, const float* __restrict__ bias,
-    int relu, __bf16* __restrict__ out, const __bf16* __restrict__ skip, __bf16* __restrict__ out_sum,
-    const int* __restrict__ done) {
+    Lvl L, ConvTiles T, const __bf16* __restrict__ in, const bf16x8* __restrict__ wfrag,
+    const float* __restrict__ bias, int relu, __bf16* __restrict__ out, const __bf16* __restrict__ skip,
+    __bf16* __restrict__ out_sum, const int* __restrict__ done) {
     if (done && *done) return;
+    static_assert(CIN % 8 == 0, "channels come in groups of 8");
     constexpr int KS = ksteps_of(CIN);
     constexpr int MT = mtiles_of(COUT);
-    const int lane = threadIdx.x & 63;
-    const int g = lane >> 4, c = lane & 15;
-    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int64_t base = (int64_t)wave * (16 * NT);
-    if (base >= L.n) return;
+    constexpr int NG = CIN / 8;  // channel groups
+    __shared__ bf16x8 tile[NG * kHalo];
+    const uint32_t nblk = gridDim.x;
+    const uint32_t b = xcd_remap(blockIdx.x, nblk);
+    const int tk = (int)(b % T.tk);
+    const uint32_t r = b / T.tk;
+    const int tj = (int)(r % T.tj), ti = (int)(r / T.tj);
+    const int i0 = ti * kTI, j0 = tj * kTJ, k0 = tk * kTK;
     const int64_t plane = (int64_t)L.ny * L.nz;
 
-    int64_t cell[NT];
-    uint32_t okmask[NT];  // bit per tap: neighbour inside the grid
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-        cell[nt] = base + nt * 16 + c;
-        uint32_t m = 0;
-        if (cell[nt] < L.n) {
-            int i, j, k;
-            cell_ijk(L, (uint32_t)cell[nt], i, j, k);
-#pragma unroll
-            for (int t = 0; t < 27; ++t) {
-                const int di = t / 9 - 1, dj = (t / 3) % 3 - 1, dk = t % 3 - 1;
-                const bool ok = (unsigned)(i + di) < (unsigned)L.nx && (unsigned)(j + dj) < (unsigned)L.ny &&
-                                (unsigned)(k + dk) < (unsigned)L.nz;
-                m |= ok ? (1u << t) : 0u;
-            }
-        }
-        okmask[nt] = m;
+    // stage the halo block: element v = (cell h, group q), consecutive
+    // threads on consecutive k of one row
+    for (int v = threadIdx.x; v < NG * kHalo; v += 256) {
+        const int h = v / NG, q = v - (v / NG) * NG;
+        const int hk = h % kHK, hr = h / kHK;
+        const int hj = hr % kHJ, hi = hr / kHJ;
+        const int gi = i0 - 1 + hi, gj = j0 - 1 + hj, gk = k0 - 1 + hk;
+        bf16x8 val = bf16x8{};
+        if ((unsigned)gi < (unsigned)L.nx && (unsigned)gj < (unsigned)L.ny && (unsigned)gk < (unsigned)L.nz)
+            val = *reinterpret_cast<const bf16x8*>(in + ((int64_t)gi * plane + (int64_t)gj * L.nz + gk) * CIN + q * 8);
+        tile[q * kHalo + h] = val;
     }
+    __syncthreads();
 
-    f32x4 acc[MT][NT];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int g = lane >> 4, c = lane & 15;
+    f32x4 acc[MT][kTJ];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int nt = 0; nt < kTJ; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 
 #pragma unroll 2
     for (int s = 0; s < KS; ++s) {
         bf16x8 a[MT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) a[mt] = wfrag[((size_t)s * MT + mt) * 64 + lane];
-        const int k0 = s * 32 + g * 8;
-        const int t = k0 / CIN;
-        const int ci0 = k0 - t * CIN;
-        const int di = t / 9 - 1, dj = (t / 3) % 3 - 1, dk = t % 3 - 1;
-        const int64_t off = (int64_t)di * plane + (int64_t)dj * L.nz + dk;
+        const int kk = s * 32 + g * 8;
+        const int t = kk / CIN;  // 27 on the padding of the last k-step (B = 0 there)
+        const int q = (kk - t * CIN) / 8;
+        const int ta = t < 27 ? t : 26;  // keeps the (discarded) LDS read inside the tile
+        const int di = ta / 9 - 1, dj = (ta / 3) % 3 - 1, dk = ta % 3 - 1;
+        // halo cell of tile row nt: (w+1+di, nt+1+dj, c+1+dk)
+        const int hbase = q * kHalo + ((w + 1 + di) * kHJ + 1 + dj) * kHK + c + 1 + dk;
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-            bf16x8 b;
-            if (t < 27 && ((okmask[nt] >> t) & 1u)) {
-                b = *reinterpret_cast<const bf16x8*>(in + (cell[nt] + off) * CIN + ci0);
-            } else {
-                b = bf16x8{};
-            }
+        for (int nt = 0; nt < kTJ; ++nt) {
+            const bf16x8 bv = t < 27 ? tile[hbase + nt * kHK] : bf16x8{};
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt)
-                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b, acc[mt][nt], 0, 0, 0);
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], bv, acc[mt][nt], 0, 0, 0);
         }
     }
 
-    // Epilogue: lane holds rows (channels) 4g..4g+3 of tile mt for cell c.
+    // Epilogue: lane holds rows (channels) 4g..4g+3 of tile mt for cell c of row nt.
+    const int ci = i0 + w, ck = k0 + c;
+    if (ci >= L.nx || ck >= L.nz) return;
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
         const int co0 = mt * 16 + g * 4;
@@ -192,8 +214,9 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(
 #pragma unroll
         for (int q = 0; q < 4; ++q) bv[q] = bias ? bias[co0 + q] : 0.0f;
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-            if (cell[nt] >= L.n) continue;
+        for (int nt = 0; nt < kTJ; ++nt) {
+            const int cj = j0 + nt;
+            if (cj >= L.ny) continue;
             __bf16 o4[4], s4[4];
             float v[4];
 #pragma unroll
@@ -202,7 +225,7 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(
                 if (relu) v[q] = v[q] > 0.0f ? v[q] : 0.0f;
                 o4[q] = (__bf16)v[q];
             }
-            const int64_t o = cell[nt] * COUT + co0;
+            const int64_t o = ((int64_t)ci * plane + (int64_t)cj * L.nz + ck) * COUT + co0;
             if (out) *reinterpret_cast<uint2*>(out + o) = *reinterpret_cast<const uint2*>(o4);
             if (skip) {
                 const uint2 sk = *reinterpret_cast<const uint2*>(skip + o);
@@ -757,16 +780,13 @@ static unsigned elem_grid(int64_t n) {
     return (unsigned)(b < 1 ? 1 : b);
 }
 
-constexpr int kNT = 4;  // 16-cell tiles per wave
-
 template <int CIN, int COUT>
 static void conv_fwd(hipStream_t st, const Lvl& L, const __bf16* in, const __bf16* packed, int l,
                      const float* params, int relu, __bf16* out, const __bf16* skip, __bf16* out_sum,
                      const int* done) {
-    const int64_t waves = (L.n + 16 * kNT - 1) / (16 * kNT);
-    const unsigned blocks = (unsigned)((waves + 3) / 4);
-    conv_mfma_kernel<CIN, COUT, kNT><<<blocks, 256, 0, st>>>(
-        L, in, reinterpret_cast<const bf16x8*>(packed + frag_off(l, 0)), params + conv_b_off(l), relu, out,
+    const ConvTiles T = conv_tiles(L);
+    conv_mfma_kernel<CIN, COUT><<<T.ti * T.tj * T.tk, 256, 0, st>>>(
+        L, T, in, reinterpret_cast<const bf16x8*>(packed + frag_off(l, 0)), params + conv_b_off(l), relu, out,
         skip, out_sum, done);
 }
 
@@ -774,10 +794,9 @@ static void conv_fwd(hipStream_t st, const Lvl& L, const __bf16* in, const __bf1
 template <int GC, int OC>
 static void conv_bwd_data(hipStream_t st, const Lvl& L, const __bf16* g, const __bf16* packed, int l,
                           __bf16* gin) {
-    const int64_t waves = (L.n + 16 * kNT - 1) / (16 * kNT);
-    const unsigned blocks = (unsigned)((waves + 3) / 4);
-    conv_mfma_kernel<GC, OC, kNT><<<blocks, 256, 0, st>>>(
-        L, g, reinterpret_cast<const bf16x8*>(packed + frag_off(l, 1)), nullptr, 0, gin, nullptr, nullptr,
+    const ConvTiles T = conv_tiles(L);
+    conv_mfma_kernel<GC, OC><<<T.ti * T.tj * T.tk, 256, 0, st>>>(
+        L, T, g, reinterpret_cast<const bf16x8*>(packed + frag_off(l, 1)), nullptr, 0, gin, nullptr, nullptr,
         nullptr);
 }
 
