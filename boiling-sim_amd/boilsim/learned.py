@@ -165,12 +165,21 @@ def evaluate_loss(net: UNet, code, n_fluid: int, rs, es) -> float:
 
 
 _FCG_WS = {"key": None, "buf": None}
+_REFINE_WS = {"key": None, "buf": None, "x64": None}
 
 
 def learned_pcg_solve(core, grid, ws, dt, rho, tol, max_iter, net: UNet | None = None,
-                      precond: int = PRECOND_MG_UNET, warm_start: bool = False) -> dict:
+                      precond: int = PRECOND_MG_UNET, warm_start: bool = False, refine: bool = True) -> dict:
     """The lpcg-hip solve of the projection: flexible CG with the learned
-    preconditioner on grid.p from ws.div_u (b = -rho dx^2 div / dt)."""
+    preconditioner on grid.p from ws.div_u (b = -rho dx^2 div / dt).
+
+    FP32 vectors cannot reach every tolerance: on right-hand sides with a
+    smooth component (a buoyant plume's divergence, say) the true relative
+    residual of an FP32 iterate stalls near 1e-6.  With ``refine`` an
+    unconverged solve continues from its iterate in FP64 iterative refinement
+    around the same learned solver (bs_fcg_refine_solve) and grid.p receives
+    the refined solution rounded to FP32; the stats then carry
+    ``refined: True`` and the refinement's true residual."""
     import torch
 
     if net is None:
@@ -182,9 +191,26 @@ def learned_pcg_solve(core, grid, ws, dt, rho, tol, max_iter, net: UNet | None =
         _FCG_WS["buf"] = torch.empty(nb, dtype=torch.uint8, device=grid.p.device)
         _FCG_WS["key"] = key
     buf = _FCG_WS["buf"]
-    return core.fcg_solve(grid.p.data_ptr(), ws.div_u.data_ptr(), grid.mat.data_ptr(), *shape, grid.dx, dt, rho,
-                          tol, max_iter, precond, net.params.data_ptr(), net.packed.data_ptr(), buf.data_ptr(),
-                          buf.numel(), warm_start=warm_start)
+    st = core.fcg_solve(grid.p.data_ptr(), ws.div_u.data_ptr(), grid.mat.data_ptr(), *shape, grid.dx, dt, rho,
+                        tol, max_iter, precond, net.params.data_ptr(), net.packed.data_ptr(), buf.data_ptr(),
+                        buf.numel(), warm_start=warm_start)
+    if st["converged"] or not refine:
+        return st
+    if _REFINE_WS["key"] != key:
+        nb = core.fcg_refine_workspace_bytes(*shape, precond)
+        _REFINE_WS["buf"] = torch.empty(nb, dtype=torch.uint8, device=grid.p.device)
+        _REFINE_WS["x64"] = torch.empty(shape, dtype=torch.float64, device=grid.p.device)
+        _REFINE_WS["key"] = key
+    x64, rbuf = _REFINE_WS["x64"], _REFINE_WS["buf"]
+    x64.copy_(grid.p)
+    rs = core.fcg_refine_solve(x64.data_ptr(), ws.div_u.data_ptr(), grid.mat.data_ptr(), *shape, grid.dx, dt, rho,
+                               tol, precond, net.params.data_ptr(), net.packed.data_ptr(), rbuf.data_ptr(),
+                               rbuf.numel(), warm_start=True, x32_ptr=grid.p.data_ptr())
+    out = dict(st)
+    out.update({"refined": True, "fp32_iterations": st["iterations"], "fp32_relres_true": st["relres_true"],
+                "iterations": st["iterations"] + rs["inner_iterations"], "refine_steps": rs["outer"],
+                "converged": rs["converged"], "relres_true": rs["relres_true"]})
+    return out
 
 
 def fluid_count(mat_t) -> int:
