@@ -252,6 +252,120 @@ __global__ __launch_bounds__(256) void pcg_spmv1_kernel(Tiles T, const uint8_t* 
     write_partials<1>(v, partA);
 }
 
+// Ap = A p and p.Ap for FP32 vectors on 4-cell chunks, marching down i: a
+// workgroup owns a 14 x 56 (j,k) tile and an i segment; each of its 256
+// threads owns one 4-cell chunk of the tile or of its one-chunk halo ring.
+// i-neighbours are the thread's own registers (planes rolled down i), j and
+// k neighbours come from two double-buffered LDS planes, so a cell costs
+// one global load of p and one of its codes instead of seven of each (the
+// one-tile kernel above is bound by those load requests).  Same float
+// operations per cell as Nbhd::apply, so Ap is bit-identical; the FP64 p.Ap
+// partials are grouped per workgroup.
+constexpr int kRollKT = 14, kRollTJ = 14;
+
+struct RollShape {
+    int jtiles, ktiles, ni;
+    unsigned grid;
+};
+
+// About 4096 workgroups: tiles over (j,k), i split into segments.
+static RollShape roll_shape(int nx, int ny, int nz) {
+    RollShape s;
+    s.jtiles = (ny + kRollTJ - 1) / kRollTJ;
+    s.ktiles = (nz + 4 * kRollKT - 1) / (4 * kRollKT);
+    const int cols = s.jtiles * s.ktiles;
+    int nseg = (4096 + cols / 2) / cols;
+    nseg = nseg < 1 ? 1 : (nseg > nx ? nx : nseg);
+    s.ni = (nx + nseg - 1) / nseg;
+    nseg = (nx + s.ni - 1) / s.ni;
+    s.grid = (unsigned)(nseg * cols);
+    return s;
+}
+
+__global__ __launch_bounds__(256) void pcg_spmv_roll_kernel(int nx, int ny, int nz, int ni_seg, int jtiles,
+                                                            int ktiles, const uint8_t* __restrict__ code,
+                                                            const float* __restrict__ p, float* __restrict__ ap,
+                                                            double* __restrict__ part, const PcgCtrl* ctrl) {
+    if (ctrl->done) return;
+    constexpr int CH = kRollKT + 2, SR = kRollTJ + 2, W = 4 * CH;
+    __shared__ __align__(16) float sV[2][SR * W + 8];
+    __shared__ __align__(16) uint8_t sC[2][SR * W + 16];
+    const int tid = threadIdx.x;
+    const bool act = tid < CH * SR;
+    const int r = tid / CH, c = tid - (tid / CH) * CH;
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const int cols = jtiles * ktiles;
+    const int seg = (int)(b / cols), col = (int)(b % cols);
+    const int jt = col / ktiles, kt = col - (col / ktiles) * ktiles;
+    const int j0 = jt * kRollTJ, k0 = kt * 4 * kRollKT;
+    const int i0 = seg * ni_seg, i1 = min(nx, i0 + ni_seg);
+    const int j = j0 - 1 + r, k = k0 - 4 + 4 * c;
+    const bool jk_in = act && j >= 0 && j < ny && k >= 0 && k < nz;
+    const bool interior = act && r >= 1 && r <= kRollTJ && c >= 1 && c <= kRollKT && j < ny && k < nz;
+    const int64_t pl = (int64_t)ny * nz;
+    const int64_t off = (int64_t)j * nz + k;
+    // planes outside the grid and chunks outside (j,k) read as air (value 0)
+    auto ldv = [&](int q) {
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (jk_in && q >= 0 && q < nx) v = *reinterpret_cast<const float4*>(p + (int64_t)q * pl + off);
+        return v;
+    };
+    auto ldk = [&](int q) {
+        uint32_t v = 0;
+        if (jk_in && q >= 0 && q < nx) v = *reinterpret_cast<const uint32_t*>(code + (int64_t)q * pl + off);
+        return v;
+    };
+    float4 Vm = ldv(i0 - 1), Vc = ldv(i0), Vn = ldv(i0 + 1);
+    uint32_t Km = ldk(i0 - 1), Kc = ldk(i0), Kn = ldk(i0 + 1);
+    double pap = 0.0;
+    for (int q = i0; q < i1; ++q) {
+        float* V = sV[q & 1] + 4;
+        uint8_t* C = sC[q & 1] + 8;
+        if (act) {
+            *reinterpret_cast<float4*>(V + r * W + 4 * c) = Vc;
+            *reinterpret_cast<uint32_t*>(C + r * W + 4 * c) = Kc;
+        }
+        const float4 Vf = ldv(q + 2);
+        const uint32_t Kf = ldk(q + 2);
+        __syncthreads();
+        if (interior) {
+            const float4 d4 = *reinterpret_cast<const float4*>(V + (r - 1) * W + 4 * c);
+            const float4 u4 = *reinterpret_cast<const float4*>(V + (r + 1) * W + 4 * c);
+            const uint32_t kd = *reinterpret_cast<const uint32_t*>(C + (r - 1) * W + 4 * c);
+            const uint32_t ku = *reinterpret_cast<const uint32_t*>(C + (r + 1) * W + 4 * c);
+            const float vb = V[r * W + 4 * c - 1], vf = V[r * W + 4 * c + 4];
+            const uint8_t cb = C[r * W + 4 * c - 1], cf = C[r * W + 4 * c + 4];
+            const float vc[4] = {Vc.x, Vc.y, Vc.z, Vc.w}, vl[4] = {Vm.x, Vm.y, Vm.z, Vm.w};
+            const float vr[4] = {Vn.x, Vn.y, Vn.z, Vn.w}, vd[4] = {d4.x, d4.y, d4.z, d4.w};
+            const float vu[4] = {u4.x, u4.y, u4.z, u4.w};
+            float out[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const uint8_t cc = (Kc >> (8 * e)) & 0xFF;
+                if (cc < kCodeFluid0) { out[e] = 0.0f; continue; }
+                const float ps = vc[e];
+                const uint8_t c_b = e > 0 ? (uint8_t)((Kc >> (8 * (e - 1))) & 0xFF) : cb;
+                const float v_b = e > 0 ? vc[e - 1] : vb;
+                const uint8_t c_f = e < 3 ? (uint8_t)((Kc >> (8 * (e + 1))) & 0xFF) : cf;
+                const float v_f = e < 3 ? vc[e + 1] : vf;
+                float s = nbr_term((uint8_t)((Km >> (8 * e)) & 0xFF), vl[e], ps) +
+                          nbr_term((uint8_t)((Kn >> (8 * e)) & 0xFF), vr[e], ps);
+                s = s + nbr_term((uint8_t)((kd >> (8 * e)) & 0xFF), vd[e], ps);
+                s = s + nbr_term((uint8_t)((ku >> (8 * e)) & 0xFF), vu[e], ps);
+                s = s + nbr_term(c_b, v_b, ps);
+                s = s + nbr_term(c_f, v_f, ps);
+                out[e] = 6.0f * ps - s;
+                pap += (double)ps * (double)out[e];
+            }
+            *reinterpret_cast<float4*>(ap + (int64_t)q * pl + off) = make_float4(out[0], out[1], out[2], out[3]);
+        }
+        Vm = Vc; Vc = Vn; Vn = Vf;
+        Km = Kc; Kc = Kn; Kn = Kf;
+    }
+    double v[1] = {pap};
+    write_partials<1>(v, part);
+}
+
 // One 1024-thread workgroup: deterministic FP64 total of G partials per value.
 template <int NV>
 __device__ __forceinline__ void reduce1024(const double* __restrict__ part, unsigned G, double (&out)[NV]) {
@@ -465,7 +579,12 @@ static int ensure_poll() {
 // Ap = A p and alpha = r.z / p.Ap into the control block (it < 0: the
 // iteration number is the device counter).
 static void enqueue_spmv(const Tiles& T, int vec, const PcgBuffers& B, int it, hipStream_t st) {
-    if (B.partT) {
+    const RollShape rs = roll_shape(T.nx, T.ny, T.nz);
+    if (B.partT && vec == 4 && rs.grid <= (unsigned)T.ntiles) {
+        pcg_spmv_roll_kernel<<<rs.grid, 256, 0, st>>>(T.nx, T.ny, T.nz, rs.ni, rs.jtiles, rs.ktiles, B.code, B.p,
+                                                      B.ap, B.partT, B.ctrl);
+        pap_reduce_kernel<<<1, 1024, 0, st>>>(B.partT, rs.grid, B.ctrl, it);
+    } else if (B.partT) {
         BS_VEC_LAUNCH(pcg_spmv1_kernel, vec, T.ntiles, st, T, B.code, B.p, B.ap, B.partT, B.ctrl);
         pap_reduce_kernel<<<1, 1024, 0, st>>>(B.partT, T.ntiles, B.ctrl, it);
     } else {
