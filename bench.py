@@ -59,8 +59,8 @@ def log(msg: str) -> None:
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="prod", choices=["small", "dev", "prod"])
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -322,8 +322,11 @@ def main():
     torch.cuda.synchronize()
     ptr = lambda t: t.data_ptr()
 
+    es0, es1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
     def projection(solver):
         core.compute_divergence(ptr(div), ptr(gux), ptr(guy), ptr(guz), ptr(gmat), nx, ny, nz, cfg.dx)
+        es0.record()
         if solver == "pcg":
             st = core.pcg_solve(ptr(p), ptr(div), ptr(gmat), nx, ny, nz, cfg.dx, DT, RHO, args.tol, 20000,
                                 ptr(ws_pcg), ws_pcg.numel())
@@ -331,13 +334,16 @@ def main():
             k = PRECOND_MG_UNET if solver == "learned" else PRECOND_MG_JACOBI
             st = core.fcg_solve(ptr(p), ptr(div), ptr(gmat), nx, ny, nz, cfg.dx, DT, RHO, args.tol, 5000, k,
                                 ptr(net.params), ptr(net.packed), ptr(ws_f[k]), ws_f[k].numel())
+        es1.record()
         core.subtract_pressure_gradient(ptr(gux), ptr(guy), ptr(guz), ptr(p), ptr(gmat), nx, ny, nz, cfg.dx, DT, RHO)
         core.compute_divergence(ptr(div), ptr(gux), ptr(guy), ptr(guz), ptr(gmat), nx, ny, nz, cfg.dx)
         core.max_abs(ptr(div), n, ptr(bits))
         return st
 
     def leg(solver):
-        times, stats = [], None
+        """`warmup` untimed then `steps` timed projections (HIP events on the
+        null stream around the whole projection and around the solve)."""
+        times, solve, stats = [], [], None
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         for s in range(args.warmup + args.steps):
             gux.copy_(ux0)
@@ -350,10 +356,13 @@ def main():
             torch.cuda.synchronize()
             if s >= args.warmup:
                 times.append(e0.elapsed_time(e1))
+                solve.append(es0.elapsed_time(es1))
         maxdiv = float(bits.cpu().numpy().view(np.float32)[0])
         ms = float(np.mean(times))
-        log(f"{solver}: {ms:.2f} ms/projection (min {min(times):.2f}), iters {stats['iterations']}, "
-            f"true relres {stats['relres_true']:.3e}, converged {stats['converged']}, max|div| after {maxdiv:.3e}")
+        stats = dict(stats, median_ms=float(np.median(times)), solve_ms_median=float(np.median(solve)))
+        log(f"{solver}: {ms:.2f} ms/projection (median {stats['median_ms']:.2f}, solve {stats['solve_ms_median']:.2f}),"
+            f" iters {stats['iterations']}, true relres {stats['relres_true']:.3e}, converged {stats['converged']}, "
+            f"max|div| after {maxdiv:.3e}")
         return ms, times, stats, maxdiv
 
     # Initial divergence (for the report).
@@ -455,6 +464,37 @@ def main():
                          f"threads ({dt_c:.1f} s incl. set-up), scaled to the GPU plain-PCG count "
                          f"{st_p['iterations']}"}
         log(f"cpu baseline: {ms_per_it:.1f} ms/iteration on {threads} threads")
+        # the same on one thread (a serial loop over cells), a shorter sample
+        oracle.set_threads(1)
+        t0 = time.perf_counter()
+        _, it_1, _ = oracle.pcg_ref(div_np, mat, cfg.dx, DT, RHO, 1e-30, 3)
+        ms_1 = (time.perf_counter() - t0) * 1e3 / max(it_1, 1)
+        oracle.set_threads(threads)
+        cpu["single_thread"] = {"ms_per_iteration": ms_1, "iterations_sampled": it_1,
+                                "ms_per_projection": ms_1 * st_p["iterations"]}
+        log(f"cpu baseline: {ms_1:.1f} ms/iteration on 1 thread")
+        # parity of the shipped Jacobi path with the oracle on config 1
+        # (64x64x48 pot, seed-7 velocities with the plume, 200 sweeps from 0)
+        from boilsim.fixtures import CONFIGS as _C
+
+        c1 = _C["small"]
+        m1 = pot_mask(*c1.shape)
+        v1 = seeded_velocities(*c1.shape, seed=7, plume=True)
+        d1 = oracle.divergence(*v1, m1, c1.dx)
+        pr, tr = np.zeros(c1.shape, np.float32), np.zeros(c1.shape, np.float32)
+        oracle.pressure_solve(pr, tr, d1, m1, c1.dx, DT, RHO, 200)
+        pg, tg = torch.zeros(c1.shape, device="cuda"), torch.zeros(c1.shape, device="cuda")
+        dg, mg = d(d1), d(m1)
+        core.pressure_solve(ptr(pg), ptr(tg), ptr(dg), ptr(mg), *c1.shape, c1.dx, DT, RHO, 200)
+        torch.cuda.synchronize()
+        got = pg.cpu().numpy()
+        fl = m1 == 0
+        rel = np.abs(got - pr)[fl] / np.maximum(np.abs(pr[fl]), 1e-6)
+        cpu["parity"] = {"case": "64x64x48 pot, seed-7 velocities + plume, 200 Jacobi sweeps from 0",
+                         "bits_equal": bool(np.array_equal(got.view(np.uint32), pr.view(np.uint32))),
+                         "max_rel_fluid": float(rel.max()), "gate": 1e-4}
+        log(f"parity (Jacobi, 200 sweeps, 64x64x48): bits equal {cpu['parity']['bits_equal']}, "
+            f"max rel {cpu['parity']['max_rel_fluid']:.1e}")
 
     fin = lambda v: v if v == v else None  # NaN (leg skipped) -> null
     out = {
@@ -474,10 +514,11 @@ def main():
                    "grid": [nx, ny, nz], "dx_m": cfg.dx, "rhs": "divergence of N(0,0.01) faces, seed 11",
                    "learned_preconditioner": "FCG + Jacobi V-cycle, U-Net (8/16/32) coarse solve"},
         "plain_pcg": None if not st_p else {
-            "ms_per_projection": ms_p, "iterations": st_p["iterations"], "relres_true": st_p["relres_true"],
-            "max_div_after": md_p},
+            "ms_per_projection": ms_p, "median_ms": st_p["median_ms"], "solve_ms_median": st_p["solve_ms_median"],
+            "iterations": st_p["iterations"], "relres_true": st_p["relres_true"], "max_div_after": md_p},
         "learned_pcg": None if not st_l else {
-            "ms_per_projection": ms_l, "iterations": st_l["iterations"], "relres_true": st_l["relres_true"],
+            "ms_per_projection": ms_l, "median_ms": st_l["median_ms"], "solve_ms_median": st_l["solve_ms_median"],
+            "iterations": st_l["iterations"], "relres_true": st_l["relres_true"],
             "fallback": st_l["fallback"], "max_div_after": md_l, "times_ms": t_l},
         "speedup_vs_plain_pcg": fin(ms_p / ms_l),
         "ablation_mg_jacobi_coarse": abl,

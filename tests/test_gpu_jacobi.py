@@ -198,3 +198,44 @@ def test_projection_zero_divergence_cells_keep_positive_zero(simcore, torch_gpu,
     torch.cuda.synchronize()
     for g, r in ((host(p), p_r), (host(pt), pt_r), (host(div), div_r), *((host(t), r) for t, r in zip(d, ref))):
         assert same_bits(g, r), bit_diff_report(g, r)
+
+
+def test_config1_matches_committed_digests(simcore, torch_gpu):
+    """Config 1 on the device (seed-7 velocities with the plume, 64x64x48):
+    the divergence, 200 sweeps (p and p_tmp) and the fused projection hash to
+    the committed digests of the oracle's outputs, bit for bit."""
+    import hashlib
+    import json
+    import os
+
+    torch = torch_gpu
+    from boilsim.fixtures import CONFIGS, DT, RHO, pot_mask, seeded_velocities
+
+    gold = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                       "config_digests.json")))["config1"]
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).view(np.uint8).tobytes()).hexdigest()
+    cfg = CONFIGS["small"]
+    mat = pot_mask(*cfg.shape)
+    ux, uy, uz = seeded_velocities(*cfg.shape, seed=gold["seed"], plume=True)
+    assert sha(mat) == gold["sha256"]["mat"] and sha(uz) == gold["sha256"]["uz"]
+    d = [dev(torch, a) for a in (ux, uy, uz)]
+    md = dev(torch, mat)
+    div = torch.empty(cfg.shape, device="cuda")
+    simcore.compute_divergence(div.data_ptr(), *(t.data_ptr() for t in d), md.data_ptr(), *cfg.shape, cfg.dx)
+    p = torch.zeros(cfg.shape, device="cuda")
+    pt = torch.zeros(cfg.shape, device="cuda")
+    simcore.pressure_solve(p.data_ptr(), pt.data_ptr(), div.data_ptr(), md.data_ptr(), *cfg.shape, cfg.dx, DT, RHO,
+                           gold["sweeps"])
+    torch.cuda.synchronize()
+    assert sha(host(div)) == gold["sha256"]["div"]
+    assert sha(host(p)) == gold["sha256"]["p"] and sha(host(pt)) == gold["sha256"]["p_tmp"]
+    p2 = torch.zeros(cfg.shape, device="cuda")
+    pt2 = torch.zeros(cfg.shape, device="cuda")
+    mb = torch.zeros(1, dtype=torch.int32, device="cuda")
+    simcore.projection_jacobi(p2.data_ptr(), pt2.data_ptr(), div.data_ptr(), *(t.data_ptr() for t in d),
+                              md.data_ptr(), *cfg.shape, cfg.dx, DT, RHO, gold["sweeps"], mb.data_ptr())
+    torch.cuda.synchronize()
+    for k, t in zip(("ux_after", "uy_after", "uz_after"), d):
+        assert sha(host(t)) == gold["sha256"][k], k
+    assert sha(host(div)) == gold["sha256"]["div_after"]
+    assert float(host(mb).view(np.float32)[0]) == gold["max_div_after"]
