@@ -101,6 +101,45 @@ __device__ __forceinline__ void ldf(const float* __restrict__ p, int64_t off, fl
     }
 }
 
+// Non-temporal loads and stores: for streams that the next kernels do not
+// read again, so that they do not displace the fields that they do.
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+template <int VEC>
+__device__ __forceinline__ void ldf_nt(const float* __restrict__ p, int64_t off, float (&v)[VEC]) {
+    if constexpr (VEC == 8) {
+        const f32x4_t a = __builtin_nontemporal_load(reinterpret_cast<const f32x4_t*>(p + off));
+        const f32x4_t b = __builtin_nontemporal_load(reinterpret_cast<const f32x4_t*>(p + off + 4));
+#pragma unroll
+        for (int e = 0; e < 4; ++e) { v[e] = a[e]; v[4 + e] = b[e]; }
+    } else if constexpr (VEC == 4) {
+        const f32x4_t t = __builtin_nontemporal_load(reinterpret_cast<const f32x4_t*>(p + off));
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[e] = t[e];
+    } else if constexpr (VEC == 2) {
+        const f32x2_t t = __builtin_nontemporal_load(reinterpret_cast<const f32x2_t*>(p + off));
+        v[0] = t[0];
+        v[1] = t[1];
+    } else {
+        v[0] = __builtin_nontemporal_load(p + off);
+    }
+}
+
+template <int VEC>
+__device__ __forceinline__ void stf_nt(float* __restrict__ p, int64_t off, const float (&v)[VEC]) {
+    if constexpr (VEC == 8) {
+        __builtin_nontemporal_store(f32x4_t{v[0], v[1], v[2], v[3]}, reinterpret_cast<f32x4_t*>(p + off));
+        __builtin_nontemporal_store(f32x4_t{v[4], v[5], v[6], v[7]}, reinterpret_cast<f32x4_t*>(p + off + 4));
+    } else if constexpr (VEC == 4) {
+        __builtin_nontemporal_store(f32x4_t{v[0], v[1], v[2], v[3]}, reinterpret_cast<f32x4_t*>(p + off));
+    } else if constexpr (VEC == 2) {
+        __builtin_nontemporal_store(f32x2_t{v[0], v[1]}, reinterpret_cast<f32x2_t*>(p + off));
+    } else {
+        __builtin_nontemporal_store(v[0], p + off);
+    }
+}
+
 template <int VEC>
 __device__ __forceinline__ void stf(float* __restrict__ p, int64_t off, const float (&v)[VEC]) {
     if constexpr (VEC == 8) {

@@ -328,8 +328,9 @@ __global__ __launch_bounds__(256) void smooth_dots_kernel(Tiles T, const uint8_t
         Nbhd<VEC> nb;
         nb.load(T, code, zi, i, j, k0, c0);
         float rv[VEC], out[VEC], zo_[VEC];
-        ldf<VEC>(r, c0, rv);
-        if (z_old) ldf<VEC>(z_old, c0, zo_);
+        // r and z_old are next read an iteration later: non-temporal
+        ldf_nt<VEC>(r, c0, rv);
+        if (z_old) ldf_nt<VEC>(z_old, c0, zo_);
 #pragma unroll
         for (int e = 0; e < VEC; ++e) {
             out[e] = nb.cc[e] >= kCodeFluid0 ? nb.vc[e] + winv(nb.cc[e]) * (rv[e] - nb.apply(e)) : 0.0f;
@@ -555,6 +556,203 @@ __global__ __launch_bounds__(256) void prolong4_kernel(Tiles F, Tiles CV, const 
         }
 }
 
+// ---------------------------------------------------------------------------
+// Pre-smoothing and restriction of one level in a single pass (nx, ny even,
+// nz % 4 == 0, 16-byte aligned vectors).  A workgroup owns a 12 x 56 (j,k)
+// tile and walks down an even-length i segment; each of its 256 threads holds
+// one 4-cell k chunk of the tile or of its ring (two rows in j, one chunk in
+// k).  At plane q it forms the two-sweep smoother z = z1 + wD^-1 (r - A z1),
+// z1 = wD^-1 r, over the tile and a one-cell ring (pre2_kernel's arithmetic,
+// so z is bit-identical), then the residual r - A z at plane q-1 over the
+// tile, which it sums over the 2x2x2 children of each coarse cell (times 0.5,
+// the restriction of restrict4_kernel; only the order of the sum differs).
+// i-neighbours are the thread's registers, j/k-neighbours come from LDS
+// planes: r double-buffered, z double-buffered, codes triple-buffered (the
+// residual at q-1 still reads plane q-1's codes while plane q+1's are
+// stored), which leaves one barrier per plane; the global loads run two
+// planes ahead.  The march through a segment is a chain of per-plane
+// latencies, so the pass pays off only where the level fills the chip with
+// several rounds of workgroups (the fine level).
+// ---------------------------------------------------------------------------
+constexpr int kPrTJ = 12, kPrKT = 14;
+
+struct PrShape {
+    int jtiles, ktiles, ni;
+    unsigned grid;
+};
+
+// About `target` workgroups, i segments of even length >= 8 (two planes of
+// each segment's smoother are recomputed by its neighbours).
+static PrShape pr_shape(int nx, int ny, int nz, int target) {
+    PrShape s;
+    s.jtiles = (ny + kPrTJ - 1) / kPrTJ;
+    s.ktiles = (nz + 4 * kPrKT - 1) / (4 * kPrKT);
+    const int cols = s.jtiles * s.ktiles;
+    int nseg = (target + cols / 2) / cols;
+    nseg = nseg < 1 ? 1 : nseg;
+    int ni = (nx + nseg - 1) / nseg;
+    ni = ni < 8 ? 8 : ni;
+    ni += ni & 1;
+    s.ni = ni;
+    nseg = (nx + ni - 1) / ni;
+    s.grid = (unsigned)(nseg * cols);
+    return s;
+}
+
+static bool pr_fits(const MLvl& L, int vec) { return vec == 4 && (L.nx & 1) == 0 && (L.ny & 1) == 0 && (L.nz & 3) == 0; }
+
+__device__ __forceinline__ uint8_t byte_of(uint32_t w, int e) { return (uint8_t)((w >> (8 * e)) & 0xFFu); }
+
+__global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int nz, int ni_seg, int jtiles,
+                                                            int ktiles, const uint8_t* __restrict__ code,
+                                                            const float* __restrict__ r, float* __restrict__ z,
+                                                            float* __restrict__ rc, const int* __restrict__ done) {
+    if (done && *done) return;
+    constexpr int CH = kPrKT + 2, SR = kPrTJ + 4, W = 4 * CH;
+    static_assert(CH * SR == 256, "one thread per chunk of the tile and its ring");
+    __shared__ __align__(16) float sR[2][SR * W + 8];
+    __shared__ __align__(16) float sZ[2][SR * W + 8];
+    __shared__ __align__(16) uint8_t sC[3][SR * W + 16];
+    const int tid = threadIdx.x;
+    const int row = tid / CH, ch = tid - (tid / CH) * CH;
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const int cols = jtiles * ktiles;
+    const int seg = (int)(b / cols), col = (int)(b % cols);
+    const int jt = col / ktiles, kt = col - (col / ktiles) * ktiles;
+    const int j0 = jt * kPrTJ, k0 = kt * 4 * kPrKT;
+    const int i0 = seg * ni_seg, i1 = min(nx, i0 + ni_seg);
+    const int j = j0 - 2 + row, k = k0 - 4 + 4 * ch;
+    const bool jk_in = j >= 0 && j < ny && k >= 0 && k < nz;
+    const bool smooth_row = row >= 1 && row <= kPrTJ + 2;
+    const bool inner = row >= 2 && row < kPrTJ + 2 && ch >= 1 && ch <= kPrKT && j < ny && k < nz;
+    const int64_t pl = (int64_t)ny * nz;
+    const int64_t off = (int64_t)j * nz + k;
+    // planes outside the grid and chunks outside (j,k) read as air (code 0, value 0)
+    auto ldv = [&](int q) {
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (jk_in && q >= 0 && q < nx) v = *reinterpret_cast<const float4*>(r + (int64_t)q * pl + off);
+        return v;
+    };
+    auto ldk = [&](int q) {
+        uint32_t v = 0;
+        if (jk_in && q >= 0 && q < nx) v = *reinterpret_cast<const uint32_t*>(code + (int64_t)q * pl + off);
+        return v;
+    };
+    const int cnx = nx >> 1, cny = ny >> 1, cnz = nz >> 1;
+    (void)cnx;
+    float4 Rm = ldv(i0 - 2), Rc = ldv(i0 - 1), Rn = ldv(i0), Rf = ldv(i0 + 1);
+    uint32_t Kmm = ldk(i0 - 3), Km = ldk(i0 - 2), Kc = ldk(i0 - 1), Kn = ldk(i0), Kf = ldk(i0 + 1);
+    float Zmm[4] = {0.f, 0.f, 0.f, 0.f}, Zm[4] = {0.f, 0.f, 0.f, 0.f};
+    float acc[2] = {0.f, 0.f};
+    int s3 = 0;  // codes slot of plane q (planes q-1, q-2 in the two before it)
+    for (int q = i0 - 1; q <= i1; ++q) {
+        const int par = (q - i0 + 1) & 1;
+        float* VR = sR[par] + 4;
+        float* VZ = sZ[par] + 4;
+        uint8_t* C = sC[s3] + 8;
+        *reinterpret_cast<float4*>(VR + row * W + 4 * ch) = Rc;
+        *reinterpret_cast<uint32_t*>(C + row * W + 4 * ch) = Kc;
+        const float4 Rg = ldv(q + 3);  // two planes of loads in flight
+        const uint32_t Kg = ldk(q + 3);
+        __syncthreads();
+        // --- smoother at plane q (tile and its one-cell ring) ---------------
+        float Zc[4] = {0.f, 0.f, 0.f, 0.f};
+        if (smooth_row) {
+            const float4 d4 = *reinterpret_cast<const float4*>(VR + (row - 1) * W + 4 * ch);
+            const float4 u4 = *reinterpret_cast<const float4*>(VR + (row + 1) * W + 4 * ch);
+            const uint32_t kd = *reinterpret_cast<const uint32_t*>(C + (row - 1) * W + 4 * ch);
+            const uint32_t ku = *reinterpret_cast<const uint32_t*>(C + (row + 1) * W + 4 * ch);
+            // k-neighbours across the chunk edge (air outside the tile row)
+            uint8_t cb = kCodeAir, cf = kCodeAir;
+            float vb = 0.0f, vf = 0.0f;
+            if (ch > 0) { cb = C[row * W + 4 * ch - 1]; vb = VR[row * W + 4 * ch - 1]; }
+            if (ch < CH - 1) { cf = C[row * W + 4 * ch + 4]; vf = VR[row * W + 4 * ch + 4]; }
+            const float rcv[4] = {Rc.x, Rc.y, Rc.z, Rc.w};
+            float vc[4], vl[4], vr[4], vd[4], vu[4];
+            {
+                const float l[4] = {Rm.x, Rm.y, Rm.z, Rm.w}, n[4] = {Rn.x, Rn.y, Rn.z, Rn.w};
+                const float dd[4] = {d4.x, d4.y, d4.z, d4.w}, uu[4] = {u4.x, u4.y, u4.z, u4.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    vc[e] = rcv[e] * winv(byte_of(Kc, e));
+                    vl[e] = l[e] * winv(byte_of(Km, e));
+                    vr[e] = n[e] * winv(byte_of(Kn, e));
+                    vd[e] = dd[e] * winv(byte_of(kd, e));
+                    vu[e] = uu[e] * winv(byte_of(ku, e));
+                }
+            }
+            vb *= winv(cb);
+            vf *= winv(cf);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const uint8_t cc = byte_of(Kc, e);
+                if (cc < kCodeFluid0) continue;
+                const float ps = vc[e];
+                const uint8_t c_b = e > 0 ? byte_of(Kc, e - 1) : cb;
+                const float v_b = e > 0 ? vc[e - 1] : vb;
+                const uint8_t c_f = e < 3 ? byte_of(Kc, e + 1) : cf;
+                const float v_f = e < 3 ? vc[e + 1] : vf;
+                float s = nbr_term(byte_of(Km, e), vl[e], ps) + nbr_term(byte_of(Kn, e), vr[e], ps);
+                s = s + nbr_term(byte_of(kd, e), vd[e], ps);
+                s = s + nbr_term(byte_of(ku, e), vu[e], ps);
+                s = s + nbr_term(c_b, v_b, ps);
+                s = s + nbr_term(c_f, v_f, ps);
+                const float az = 6.0f * ps - s;
+                Zc[e] = ps + winv(cc) * (rcv[e] - az);
+            }
+            if (inner && q >= i0 && q < i1)
+                *reinterpret_cast<float4*>(z + (int64_t)q * pl + off) = make_float4(Zc[0], Zc[1], Zc[2], Zc[3]);
+        }
+        *reinterpret_cast<float4*>(VZ + row * W + 4 * ch) = make_float4(Zc[0], Zc[1], Zc[2], Zc[3]);
+        // --- residual at plane p = q - 1, summed into the coarse cells ------
+        const int p = q - 1;
+        if (p >= i0) {
+            if (inner) {
+                const float* PZ = sZ[par ^ 1] + 4;  // plane p's smoothed values
+                const uint8_t* PC = sC[s3 == 0 ? 2 : s3 - 1] + 8;
+                const float4 d4 = *reinterpret_cast<const float4*>(PZ + (row - 1) * W + 4 * ch);
+                const float4 u4 = *reinterpret_cast<const float4*>(PZ + (row + 1) * W + 4 * ch);
+                const uint32_t kd = *reinterpret_cast<const uint32_t*>(PC + (row - 1) * W + 4 * ch);
+                const uint32_t ku = *reinterpret_cast<const uint32_t*>(PC + (row + 1) * W + 4 * ch);
+                const uint8_t cb = PC[row * W + 4 * ch - 1], cf = PC[row * W + 4 * ch + 4];
+                const float vb = PZ[row * W + 4 * ch - 1], vf = PZ[row * W + 4 * ch + 4];
+                const float rp[4] = {Rm.x, Rm.y, Rm.z, Rm.w};
+                const float vd[4] = {d4.x, d4.y, d4.z, d4.w}, vu[4] = {u4.x, u4.y, u4.z, u4.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const uint8_t cc = byte_of(Km, e);
+                    if (cc < kCodeFluid0) continue;
+                    const float ps = Zm[e];
+                    const uint8_t c_b = e > 0 ? byte_of(Km, e - 1) : cb;
+                    const float v_b = e > 0 ? Zm[e - 1] : vb;
+                    const uint8_t c_f = e < 3 ? byte_of(Km, e + 1) : cf;
+                    const float v_f = e < 3 ? Zm[e + 1] : vf;
+                    float s = nbr_term(byte_of(Kmm, e), Zmm[e], ps) + nbr_term(byte_of(Kc, e), Zc[e], ps);
+                    s += nbr_term(byte_of(kd, e), vd[e], ps);
+                    s += nbr_term(byte_of(ku, e), vu[e], ps);
+                    s += nbr_term(c_b, v_b, ps);
+                    s += nbr_term(c_f, v_f, ps);
+                    acc[e >> 1] += rp[e] - (6.0f * ps - s);
+                }
+            }
+            if (p & 1) {  // both fine planes of coarse plane p/2 are in: add row pairs
+                const float o0 = __shfl_down(acc[0], CH, 64);
+                const float o1 = __shfl_down(acc[1], CH, 64);
+                if (inner && (row & 1) == 0) {
+                    const int64_t cc0 = ((int64_t)(p >> 1) * cny + (j >> 1)) * cnz + (k >> 1);
+                    *reinterpret_cast<float2*>(rc + cc0) = make_float2(0.5f * (acc[0] + o0), 0.5f * (acc[1] + o1));
+                }
+                acc[0] = acc[1] = 0.0f;
+            }
+        }
+        Rm = Rc; Rc = Rn; Rn = Rf; Rf = Rg;
+        Kmm = Km; Km = Kc; Kc = Kn; Kn = Kf; Kf = Kg;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) { Zmm[e] = Zm[e]; Zm[e] = Zc[e]; }
+        s3 = s3 == 2 ? 0 : s3 + 1;
+    }
+}
+
 // z_out = z_in + P ec from level LC to its refinement LF (tiles TF, vector
 // width vf); z_out may be z_in.
 static int prolong_level(hipStream_t st, const Tiles& TF, int vf, const MLvl& LF, const MLvl& LC,
@@ -692,6 +890,23 @@ static void restrict_level(hipStream_t st, const Tiles& TF, int vf, const MLvl& 
     }
 }
 
+// Two smoothing sweeps from zero on level LF and the restricted residual on
+// LC: one pass when the shape allows it (fused != 0), else pre2_kernel and
+// the restriction kernels.
+static int pre_restrict(hipStream_t st, const Tiles& TF, int vf, const MLvl& LF, const MLvl& LC, const Tiles& TC1,
+                        const uint8_t* code, const float* r, float* z, float* rc, const int* done, int fused,
+                        int target_wg) {
+    if (fused && pr_fits(LF, vf)) {
+        const PrShape S = pr_shape(LF.nx, LF.ny, LF.nz, target_wg);
+        pre2_restrict_kernel<<<S.grid, 256, 0, st>>>(LF.nx, LF.ny, LF.nz, S.ni, S.jtiles, S.ktiles, code, r, z, rc,
+                                                     done);
+    } else {
+        MG_VEC(pre2_kernel, vf, TF.ntiles, st, TF, code, r, z, done);
+        restrict_level(st, TF, vf, LF, LC, TC1, code, r, z, rc, done);
+    }
+    return (int)hipGetLastError();
+}
+
 int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* code0, const float* r, float* z,
              const float* z_old, double* dots_part, int coarse_kind, const float* params, const void* packed,
              double n_fluid_coarse, const int* done) {
@@ -704,11 +919,12 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
     const Tiles C2 = make_tiles(W.L2.nx, W.L2.ny, W.L2.nz, 1);
     const unsigned g2 = grid_for(W.L2.n);
     // fine: two pre-smoothing sweeps from zero, restriction of the residual
-    MG_VEC(pre2_kernel, v0, T0.ntiles, st, T0, code0, r, z, done);
-    restrict_level(st, T0, v0, W.L0, W.L1, C1, code0, r, z, W.r1, done);
-    // level 1
-    MG_VEC(pre2_kernel, v1, T1.ntiles, st, T1, W.code1, W.r1, W.z1, done);
-    restrict_level(st, T1, v1, W.L1, W.L2, C2, W.code1, W.r1, W.z1, W.r2, done);
+    int rc = pre_restrict(st, T0, v0, W.L0, W.L1, C1, code0, r, z, W.r1, done, 1, 7680);
+    if (rc != hipSuccess) return rc;
+    // level 1 (the single pass walks its i segments plane by plane: with the
+    // few workgroups this level fills, that chain is slower than two kernels)
+    rc = pre_restrict(st, T1, v1, W.L1, W.L2, C2, W.code1, W.r1, W.z1, W.r2, done, 0, 0);
+    if (rc != hipSuccess) return rc;
     // level 2: coarse solve.  coarse_kind 0: 11 damped-Jacobi sweeps;
     // 1: the network (A-norm-optimal step), then 10 sweeps polishing its
     // correction; 2: 5 sweeps, the network on their residual, then 4 sweeps.
@@ -722,8 +938,8 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
         jacobi0_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, done);
         sweeps(5);
     } else if (coarse_kind == 1) {
-        const int rc = unet_apply(st, W.unet, W.L2.nx, W.L2.ny, W.L2.nz, params, packed, W.r2, W.code2,
-                                  n_fluid_coarse, nullptr, 0, W.e2, nullptr, nullptr, done);
+        rc = unet_apply(st, W.unet, W.L2.nx, W.L2.ny, W.L2.nz, params, packed, W.r2, W.code2, n_fluid_coarse,
+                        nullptr, 0, W.e2, nullptr, nullptr, done);
         if (rc != hipSuccess) return rc;
         optimal_step(st, W.L2, W.code2, W.r2, W.e2, W.part, done);
         sweeps(5);
@@ -732,15 +948,15 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
         jacobi_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, W.zt2, done);
         jacobi_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.zt2, W.e2, done);
         residual_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, W.rs2, done);
-        const int rc = unet_apply(st, W.unet, W.L2.nx, W.L2.ny, W.L2.nz, params, packed, W.rs2, W.code2,
-                                  n_fluid_coarse, nullptr, 0, W.en2, nullptr, nullptr, done);
+        rc = unet_apply(st, W.unet, W.L2.nx, W.L2.ny, W.L2.nz, params, packed, W.rs2, W.code2, n_fluid_coarse,
+                        nullptr, 0, W.en2, nullptr, nullptr, done);
         if (rc != hipSuccess) return rc;
         optimal_step(st, W.L2, W.code2, W.rs2, W.en2, W.part, done);
         add_kernel<<<g2, 256, 0, st>>>(W.L2.n, W.en2, W.e2, done);
         sweeps(2);
     }
     // back up: level 1
-    int rc = prolong_level(st, T1, v1, W.L1, W.L2, C2, W.code1, W.e2, W.z1, W.z1, done);
+    rc = prolong_level(st, T1, v1, W.L1, W.L2, C2, W.code1, W.e2, W.z1, W.z1, done);
     if (rc != hipSuccess) return rc;
     MG_VEC(smooth_kernel, v1, T1.ntiles, st, T1, W.code1, W.r1, W.z1, W.zt1, done);
     MG_VEC(smooth_kernel, v1, T1.ntiles, st, T1, W.code1, W.r1, W.zt1, W.z1, done);
@@ -753,3 +969,22 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
 }
 
 }  // namespace bs
+
+// The V-cycle's first pass on one level (diagnostic entry, no counterpart in
+// the reference): z = two damped-Jacobi sweeps from zero on A z = r, and
+// rc[I,J,K] = 0.5 x the sum of r - A z over the 8 children of coarse cell
+// (I,J,K) (coarse dims (n+1)/2).  fused = 1 runs the single-pass kernel and is
+// an error unless nx, ny are even, nz % 4 == 0 and the vectors are 16-byte
+// aligned; fused = 0 runs the two-kernel path.
+extern "C" int bs_mg_smooth_restrict(const unsigned char* code, const float* r, float* z, float* rc, int nx, int ny,
+                                     int nz, int fused, bs_stream_t stream) {
+    using namespace bs;
+    if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
+    const MLvl LF = mk(nx, ny, nz);
+    const MLvl LC = mk((nx + 1) / 2, (ny + 1) / 2, (nz + 1) / 2);
+    const int vf = vec_for(nz, {r, z, code});
+    if (fused && !pr_fits(LF, vf)) return (int)hipErrorInvalidValue;
+    const Tiles TF = make_tiles(nx, ny, nz, vf);
+    const Tiles TC1 = make_tiles(LC.nx, LC.ny, LC.nz, 1);
+    return pre_restrict((hipStream_t)stream, TF, vf, LF, LC, TC1, code, r, z, rc, nullptr, fused, 4096);
+}

@@ -880,9 +880,12 @@ __global__ __launch_bounds__(256) void fcg_update_kernel(Tiles T, const float* _
         if (!tile_cell<VEC>(T, t, i, j, k0)) continue;
         const int64_t c0 = (int64_t)i * plane + (int64_t)j * T.nz + k0;
         float pv[VEC], av[VEC], xv[VEC], rv[VEC];
+        // Ap and x are not read again before the next iteration's update:
+        // non-temporal, so that they do not displace r (read next by the
+        // preconditioner) in the caches.
         ldf<VEC>(p, c0, pv);
-        ldf<VEC>(ap, c0, av);
-        ldf<VEC>(x, c0, xv);
+        ldf_nt<VEC>(ap, c0, av);
+        ldf_nt<VEC>(x, c0, xv);
         ldf<VEC>(r, c0, rv);
 #pragma unroll
         for (int e = 0; e < VEC; ++e) {
@@ -890,7 +893,7 @@ __global__ __launch_bounds__(256) void fcg_update_kernel(Tiles T, const float* _
             rv[e] = rv[e] - af * av[e];
             rr += (double)rv[e] * (double)rv[e];
         }
-        stf<VEC>(x, c0, xv);
+        stf_nt<VEC>(x, c0, xv);
         stf<VEC>(r, c0, rv);
     }
     double v[1] = {rr};
@@ -977,7 +980,7 @@ __global__ __launch_bounds__(256) void fcg_direction_kernel(Tiles T, const float
         if (!tile_cell<VEC>(T, t, i, j, k0)) continue;
         const int64_t c0 = (int64_t)i * plane + (int64_t)j * T.nz + k0;
         float zv[VEC], pv[VEC];
-        ldf<VEC>(z, c0, zv);
+        ldf_nt<VEC>(z, c0, zv);  // z is next read as z_old an iteration later
         if (init) {
             stf<VEC>(p, c0, zv);
         } else {

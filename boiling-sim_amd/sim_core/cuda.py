@@ -286,6 +286,14 @@ class SimCore:
         )
         return st.as_dict()
 
+    def mg_smooth_restrict(self, code_ptr, r_ptr, z_ptr, rc_ptr, nx, ny, nz, fused=True, stream=0) -> None:
+        """The V-cycle's first pass on one level: z = two damped-Jacobi sweeps
+        from zero, rc = 0.5 x the children's sum of r - A z (see
+        bs_mg_smooth_restrict); fused selects the single-pass kernel."""
+        _lib.call("bs_mg_smooth_restrict", _ptr("code_ptr", code_ptr), _ptr("r_ptr", r_ptr), _ptr("z_ptr", z_ptr),
+                  _ptr("rc_ptr", rc_ptr), _usize("nx", nx), _usize("ny", ny), _usize("nz", nz), 1 if fused else 0,
+                  _ptr("stream", stream))
+
     # ---- mixed-precision iterative refinement -------------------------------
     STORAGE = {"fp32": 0, "fp16": 1, "bf16": 2}
     # Residual reduction between replacements by storage type: 16-bit vectors
