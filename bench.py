@@ -5,9 +5,11 @@ pot (dx = 0.5 mm, ~54.2 M cells, FP32 fields), RHS the divergence of seeded
 N(0, 0.01) face velocities (seed 11, no plume), solved to a TRUE relative
 residual ||b - A x||_2/||b||_2 <= 1e-6 (FP64 check), inputs resident in HBM:
   * learned-PCG: flexible CG preconditioned by a damped-Jacobi V-cycle whose
-    4x-coarsened grid is solved by the trained U-Net  -> `value`
+    4x-coarsened grid is solved by the trained U-Net, its correction polished
+    by Chebyshev steps  -> `value`
   * plain PCG: Jacobi-preconditioned CG, FP32 vectors / FP64 scalars
-  * (ablation) the same V-cycle with Jacobi sweeps instead of the network
+  * (ablation) the same V-cycle with one Jacobi sweep from zero instead of
+    the network, followed by the same Chebyshev polish
 ms/projection is the whole projection: divergence -> solve -> gradient
 subtraction -> divergence -> on-device max|div|.  Each step restores the
 input velocities (untimed) so every projection solves the same system.
@@ -512,7 +514,7 @@ def main():
         "data": "synthetic: seeded pot mask, N(0,0.01) face velocities (seed 11), trained U-Net weights",
         "config": {"workload": f"pressure projection, {nx}x{ny}x{nz} pot, true rel-res {args.tol:g}",
                    "grid": [nx, ny, nz], "dx_m": cfg.dx, "rhs": "divergence of N(0,0.01) faces, seed 11",
-                   "learned_preconditioner": "FCG + Jacobi V-cycle, U-Net (8/16/32) coarse solve"},
+                   "learned_preconditioner": "FCG + Jacobi V-cycle, U-Net (8/16/32) coarse solve + 20 Chebyshev steps"},
         "plain_pcg": None if not st_p else {
             "ms_per_projection": ms_p, "median_ms": st_p["median_ms"], "solve_ms_median": st_p["solve_ms_median"],
             "iterations": st_p["iterations"], "relres_true": st_p["relres_true"], "max_div_after": md_p},

@@ -12,9 +12,9 @@
 // Smoother: z += w D^-1 (r - A z), w = 0.8: two sweeps before on both fine
 // levels, two after on the 2x-coarsened one and one after on the fine one
 // (a second fine post-sweep costs more time than the iterations it saves).
-// The network's correction on the coarsest grid is polished with damped-Jacobi
-// sweeps there, which keeps the preconditioner positive where the network
-// alone is not.
+// The network's correction on the coarsest grid is polished there by 20
+// Chebyshev steps on the Jacobi splitting, which also keeps the
+// preconditioner positive where the network alone is not.
 #include "boilsim_hip.h"
 #include "grid.hpp"
 #include "precond_api.hpp"
@@ -47,6 +47,14 @@ __device__ __forceinline__ void ijk(const MLvl& L, uint32_t c, int& i, int& j, i
 
 constexpr float kOmega = 0.8f;
 constexpr unsigned kStepWg = 512;
+// Polish of the coarsest level's correction: Chebyshev steps on the Jacobi
+// splitting over the eigenvalue interval [kChebLo, 2] of D^-1 A.  Measured
+// at 440x440x280 (velocity seeds 11/21/31, flexible-CG iterations): 10
+// damped-Jacobi sweeps 34/40/37, 40 sweeps 26/28/26, 20 Chebyshev steps over
+// [0.02, 2] 21/23/21 (12 steps: 31; [0.0067, 2] needs 20-24 steps for the
+// same; [0.2, 2] is worse); each step costs about one sweep (~5.6 us).
+constexpr int kChebSteps = 20;
+constexpr double kChebLo = 0.02;
 
 __device__ __forceinline__ float inv_diag(uint8_t code) {
     const int dg = 8 - (int)code;
@@ -311,6 +319,33 @@ __global__ __launch_bounds__(256) void smooth_kernel(Tiles T, const uint8_t* __r
     for (int e = 0; e < VEC; ++e)
         out[e] = nb.cc[e] >= kCodeFluid0 ? nb.vc[e] + winv(nb.cc[e]) * (rv[e] - nb.apply(e)) : 0.0f;
     stf<VEC>(zo, c0, out);
+}
+
+// Chebyshev semi-iteration step on A x = r with the Jacobi splitting:
+// x_out = x_prev + om (x_cur - x_prev + ga D^-1 (r - A x_cur)) on fluid
+// cells, 0 elsewhere (x_out may be x_prev: read and written per cell).
+template <int VEC>
+__global__ __launch_bounds__(256) void cheb_kernel(Tiles T, const uint8_t* __restrict__ code,
+                                                   const float* __restrict__ r, const float* __restrict__ x_cur,
+                                                   const float* x_prev, float* x_out, float om, float ga,
+                                                   const int* __restrict__ done) {
+    if (done && *done) return;
+    int i, j, k0;
+    int64_t c0;
+    if (!one_tile(T, VEC, i, j, k0, c0)) return;
+    Nbhd<VEC> nb;
+    nb.load(T, code, x_cur, i, j, k0, c0);
+    float rv[VEC], pv[VEC], out[VEC];
+    ldf<VEC>(r, c0, rv);
+    ldf<VEC>(x_prev, c0, pv);
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) {
+        const uint8_t cc = nb.cc[e];
+        const int dg = 8 - (int)cc;
+        const float dinv = dg > 0 ? __builtin_amdgcn_rcpf((float)dg) : 1.0f;
+        out[e] = cc >= kCodeFluid0 ? pv[e] + om * (nb.vc[e] - pv[e] + ga * dinv * (rv[e] - nb.apply(e))) : 0.0f;
+    }
+    stf<VEC>(x_out, c0, out);
 }
 
 // Last smoothing sweep of the V-cycle fused with the flexible-CG products:
@@ -925,24 +960,52 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
     // few workgroups this level fills, that chain is slower than two kernels)
     rc = pre_restrict(st, T1, v1, W.L1, W.L2, C2, W.code1, W.r1, W.z1, W.r2, done, 0, 0);
     if (rc != hipSuccess) return rc;
-    // level 2: coarse solve.  coarse_kind 0: 11 damped-Jacobi sweeps;
-    // 1: the network (A-norm-optimal step), then 10 sweeps polishing its
-    // correction; 2: 5 sweeps, the network on their residual, then 4 sweeps.
+    // level 2: coarse solve.  coarse_kind 0: one damped-Jacobi sweep from
+    // zero, then the polish; 1: the network (A-norm-optimal step), then the
+    // polish; 2: 5 sweeps, the network on their residual, then 4 sweeps.
+    // Coarsest-level sweeps: the vectorised one-tile smoother, in ping-pong
+    // pairs ending in e2.
+    const int v2 = vec_for(W.L2.nz, {W.r2, W.e2, W.zt2, W.code2});
+    const Tiles T2 = make_tiles(W.L2.nx, W.L2.ny, W.L2.nz, v2);
     auto sweeps = [&](int pairs) {
         for (int q = 0; q < pairs; ++q) {
-            jacobi_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, W.zt2, done);
-            jacobi_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.zt2, W.e2, done);
+            MG_VEC(smooth_kernel, v2, T2.ntiles, st, T2, W.code2, W.r2, W.e2, W.zt2, done);
+            MG_VEC(smooth_kernel, v2, T2.ntiles, st, T2, W.code2, W.r2, W.zt2, W.e2, done);
+        }
+    };
+    // The polish: kChebSteps Chebyshev steps over [kChebLo, 2] from e2, ending
+    // in e2 (x_{k+1} overwrites x_{k-1}, so the iterates alternate between e2
+    // and zt2).  om_1 = 1, om_2 = 1/(1 - 1/(2 s^2)), om_{k+1} = 1/(1 - om_k/(4 s^2))
+    // with s = (b + a)/(b - a): the Chebyshev semi-iterative method on the
+    // Jacobi splitting (Golub and Varga).
+    static_assert(kChebSteps % 2 == 0, "the polish ends in e2");
+    auto polish = [&]() {
+        const double hi = 2.0, lo = kChebLo;
+        const double th = 0.5 * (hi + lo), sg = th / (0.5 * (hi - lo));
+        const float ga = (float)(1.0 / th);
+        double om = 1.0;
+        float* cur = W.e2;
+        float* prv = W.zt2;
+        for (int q = 0; q < kChebSteps; ++q) {
+            if (q == 1) om = 1.0 / (1.0 - 1.0 / (2.0 * sg * sg));
+            else if (q > 1) om = 1.0 / (1.0 - om / (4.0 * sg * sg));
+            // step 0 has no previous iterate: x_prev = x_0
+            const float* xp = q == 0 ? cur : prv;
+            MG_VEC(cheb_kernel, v2, T2.ntiles, st, T2, W.code2, W.r2, cur, xp, prv, (float)om, ga, done);
+            float* t = cur;
+            cur = prv;
+            prv = t;
         }
     };
     if (coarse_kind == 0) {
         jacobi0_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, done);
-        sweeps(5);
+        polish();
     } else if (coarse_kind == 1) {
         rc = unet_apply(st, W.unet, W.L2.nx, W.L2.ny, W.L2.nz, params, packed, W.r2, W.code2, n_fluid_coarse,
                         nullptr, 0, W.e2, nullptr, nullptr, done);
         if (rc != hipSuccess) return rc;
         optimal_step(st, W.L2, W.code2, W.r2, W.e2, W.part, done);
-        sweeps(5);
+        polish();
     } else {
         jacobi0_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, done);
         jacobi_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, W.zt2, done);

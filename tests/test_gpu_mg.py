@@ -125,3 +125,4 @@ def test_fused_pass_rejects_unsupported_shapes(simcore, torch_gpu):
             simcore.mg_smooth_restrict(code.data_ptr(), r.data_ptr(), z.data_ptr(), rc.data_ptr(), *shape, 1)
         simcore.mg_smooth_restrict(code.data_ptr(), r.data_ptr(), z.data_ptr(), rc.data_ptr(), *shape, 0)
     torch.cuda.synchronize()
+
