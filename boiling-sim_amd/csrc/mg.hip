@@ -9,11 +9,11 @@
 // the coarse classes.  Restriction r_c = 4 * mean over the 8 children (the
 // 4 = (2h/h)^2 keeps the unscaled stencil consistent across levels);
 // prolongation is cell-centred trilinear (weights 3/4, 1/4 per axis).
-// Smoother: z += w D^-1 (r - A z), w = 0.8: two sweeps before on both fine
-// levels, two after on the 2x-coarsened one and one after on the fine one
-// (a second fine post-sweep costs more time than the iterations it saves).
-// The network's correction on the coarsest grid is polished there by 20
-// Chebyshev steps on the Jacobi splitting, which also keeps the
+// Smoothing: two damped-Jacobi sweeps z += w D^-1 (r - A z), w = 0.8, before
+// on both fine levels; after, one on the fine level (a second costs more
+// time than the iterations it saves) and 4 Chebyshev steps on the Jacobi
+// splitting on the 2x-coarsened one.  The network's correction on the
+// coarsest grid is polished there by 20 Chebyshev steps, which also keeps the
 // preconditioner positive where the network alone is not.
 #include "boilsim_hip.h"
 #include "grid.hpp"
@@ -55,6 +55,11 @@ constexpr unsigned kStepWg = 512;
 // same; [0.2, 2] is worse); each step costs about one sweep (~5.6 us).
 constexpr int kChebSteps = 20;
 constexpr double kChebLo = 0.02;
+// Post-smoothing of the 2x-coarsened level after the prolongation: 4
+// Chebyshev steps over [0.02, 2] instead of two damped-Jacobi sweeps (seeds
+// 11/21/31: 21/23/21 -> 17/20/19 iterations; 2 steps or [0.067, 2]: 20-22).
+constexpr int kL1ChebSteps = 4;
+constexpr double kL1ChebLo = 0.02;
 
 __device__ __forceinline__ float inv_diag(uint8_t code) {
     const int dg = 8 - (int)code;
@@ -347,6 +352,15 @@ __global__ __launch_bounds__(256) void cheb_kernel(Tiles T, const uint8_t* __res
     }
     stf<VEC>(x_out, c0, out);
 }
+
+// `steps` Chebyshev steps on A x = r over the interval [lo, 2] of D^-1 A from
+// x, ending in x (x_{k+1} overwrites x_{k-1}, so the iterates alternate
+// between x and tmp; steps even): om_1 = 1, om_2 = 1/(1 - 1/(2 s^2)),
+// om_{k+1} = 1/(1 - om_k/(4 s^2)), ga = 2/(2 + lo), s = (2 + lo)/(2 - lo) —
+// the Chebyshev semi-iterative method on the Jacobi splitting (Golub and
+// Varga).  Graph-capturable: the coefficients are kernel arguments.
+static void chebyshev(hipStream_t st, const Tiles& T, int vec, const uint8_t* code, const float* r, float* x,
+                      float* tmp, int steps, double lo, const int* done);
 
 // Last smoothing sweep of the V-cycle fused with the flexible-CG products:
 // zo = zi + wD^-1 (r - A zi) and one partial of r.zo and of r.z_old per tile.
@@ -942,6 +956,25 @@ static int pre_restrict(hipStream_t st, const Tiles& TF, int vf, const MLvl& LF,
     return (int)hipGetLastError();
 }
 
+static void chebyshev(hipStream_t st, const Tiles& T, int vec, const uint8_t* code, const float* r, float* x,
+                      float* tmp, int steps, double lo, const int* done) {
+    const double hi = 2.0;
+    const double th = 0.5 * (hi + lo), sg = th / (0.5 * (hi - lo));
+    const float ga = (float)(1.0 / th);
+    double om = 1.0;
+    float* cur = x;
+    float* prv = tmp;
+    for (int q = 0; q < steps; ++q) {
+        if (q == 1) om = 1.0 / (1.0 - 1.0 / (2.0 * sg * sg));
+        else if (q > 1) om = 1.0 / (1.0 - om / (4.0 * sg * sg));
+        const float* xp = q == 0 ? cur : prv;  // step 0 has no previous iterate
+        MG_VEC(cheb_kernel, vec, T.ntiles, st, T, code, r, cur, xp, prv, (float)om, ga, done);
+        float* t = cur;
+        cur = prv;
+        prv = t;
+    }
+}
+
 int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* code0, const float* r, float* z,
              const float* z_old, double* dots_part, int coarse_kind, const float* params, const void* packed,
              double n_fluid_coarse, const int* done) {
@@ -973,30 +1006,8 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
             MG_VEC(smooth_kernel, v2, T2.ntiles, st, T2, W.code2, W.r2, W.zt2, W.e2, done);
         }
     };
-    // The polish: kChebSteps Chebyshev steps over [kChebLo, 2] from e2, ending
-    // in e2 (x_{k+1} overwrites x_{k-1}, so the iterates alternate between e2
-    // and zt2).  om_1 = 1, om_2 = 1/(1 - 1/(2 s^2)), om_{k+1} = 1/(1 - om_k/(4 s^2))
-    // with s = (b + a)/(b - a): the Chebyshev semi-iterative method on the
-    // Jacobi splitting (Golub and Varga).
-    static_assert(kChebSteps % 2 == 0, "the polish ends in e2");
-    auto polish = [&]() {
-        const double hi = 2.0, lo = kChebLo;
-        const double th = 0.5 * (hi + lo), sg = th / (0.5 * (hi - lo));
-        const float ga = (float)(1.0 / th);
-        double om = 1.0;
-        float* cur = W.e2;
-        float* prv = W.zt2;
-        for (int q = 0; q < kChebSteps; ++q) {
-            if (q == 1) om = 1.0 / (1.0 - 1.0 / (2.0 * sg * sg));
-            else if (q > 1) om = 1.0 / (1.0 - om / (4.0 * sg * sg));
-            // step 0 has no previous iterate: x_prev = x_0
-            const float* xp = q == 0 ? cur : prv;
-            MG_VEC(cheb_kernel, v2, T2.ntiles, st, T2, W.code2, W.r2, cur, xp, prv, (float)om, ga, done);
-            float* t = cur;
-            cur = prv;
-            prv = t;
-        }
-    };
+    static_assert(kChebSteps % 2 == 0 && kL1ChebSteps % 2 == 0, "Chebyshev polishes end in their input");
+    auto polish = [&]() { chebyshev(st, T2, v2, W.code2, W.r2, W.e2, W.zt2, kChebSteps, kChebLo, done); };
     if (coarse_kind == 0) {
         jacobi0_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, done);
         polish();
@@ -1021,8 +1032,7 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
     // back up: level 1
     rc = prolong_level(st, T1, v1, W.L1, W.L2, C2, W.code1, W.e2, W.z1, W.z1, done);
     if (rc != hipSuccess) return rc;
-    MG_VEC(smooth_kernel, v1, T1.ntiles, st, T1, W.code1, W.r1, W.z1, W.zt1, done);
-    MG_VEC(smooth_kernel, v1, T1.ntiles, st, T1, W.code1, W.r1, W.zt1, W.z1, done);
+    chebyshev(st, T1, v1, W.code1, W.r1, W.z1, W.zt1, kL1ChebSteps, kL1ChebLo, done);
     // fine: prolongation, then one sweep fused with r.z, r.z_old
     rc = prolong_level(st, T0, v0, W.L0, W.L1, C1, code0, W.z1, z, W.zt0, done);
     if (rc != hipSuccess) return rc;
