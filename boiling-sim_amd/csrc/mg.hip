@@ -9,11 +9,12 @@
 // the coarse classes.  Restriction r_c = 4 * mean over the 8 children (the
 // 4 = (2h/h)^2 keeps the unscaled stencil consistent across levels);
 // prolongation is cell-centred trilinear (weights 3/4, 1/4 per axis).
-// Smoothing: two damped-Jacobi sweeps z += w D^-1 (r - A z), w = 0.8, before
-// on both fine levels; after, one on the fine level (a second costs more
-// time than the iterations it saves) and 4 Chebyshev steps on the Jacobi
-// splitting on the 2x-coarsened one.  The network's correction on the
-// coarsest grid is polished there by 20 Chebyshev steps, which also keeps the
+// Smoothing (all on the Jacobi splitting z += c D^-1 (r - A z)): before, on
+// both fine levels, the two-step Chebyshev polynomial over [0.3, 2] from
+// zero; after, one undamped sweep on the fine level (a second costs more time
+// than the iterations it saves) and 4 Chebyshev steps over [0.02, 2] on the
+// 2x-coarsened one.  The network's correction on the coarsest grid is
+// polished there by 20 Chebyshev steps over [0.02, 2], which also keeps the
 // preconditioner positive where the network alone is not.
 #include "boilsim_hip.h"
 #include "grid.hpp"
@@ -60,6 +61,12 @@ constexpr double kChebLo = 0.02;
 // 11/21/31: 21/23/21 -> 17/20/19 iterations; 2 steps or [0.067, 2]: 20-22).
 constexpr int kL1ChebSteps = 4;
 constexpr double kL1ChebLo = 0.02;
+// Pre-smoothing on both fine levels: the two-step Chebyshev polynomial over
+// [0.3, 2] instead of two sweeps damped by 0.8; post-smoothing on the fine
+// level: one undamped Jacobi sweep (seeds 11/21/31: 17/20/19 -> 15/15/15
+// iterations; [0.1, 2] or [0.5, 2] and post weights 0.7 or 0.9 are worse).
+constexpr double kPreLo = 0.3;
+constexpr float kPostW = 1.0f;
 
 __device__ __forceinline__ float inv_diag(uint8_t code) {
     const int dg = 8 - (int)code;
@@ -269,18 +276,32 @@ __global__ __launch_bounds__(256) void prolong_kernel(MLvl F, MLvl C, const uint
 
 // w / diag for the smoother; v_rcp_f32 (1 ulp) instead of an IEEE divide —
 // the smoother needs no exact rounding and this runs ~26 times per thread.
-__device__ __forceinline__ float winv(uint8_t code) {
+__device__ __forceinline__ float winv(uint8_t code, float w = kOmega) {
     const int dg = 8 - (int)code;
     const float inv = dg > 0 ? __builtin_amdgcn_rcpf((float)dg) : 1.0f;
-    return code >= kCodeFluid0 ? kOmega * inv : 0.0f;
+    return code >= kCodeFluid0 ? w * inv : 0.0f;
 }
 
-// Two damped-Jacobi sweeps from z = 0 in one pass: z1 = wD^-1 r is formed on
-// the fly at the cell and its neighbours, z = z1 + wD^-1 (r - A z1).
+// Coefficients of the two-step smoother from zero: z1 = c1 D^-1 r,
+// z = c2 (z1 + c3 D^-1 (r - A z1)); damped Jacobi is (w, 1, w).
+struct Pre2Coef {
+    float c1, c2, c3;
+};
+
+// The two-step Chebyshev polynomial over [lo, 2]: c1 = c3 = g = 2/(2 + lo),
+// c2 = om_2 = 1/(1 - 1/(2 s^2)), s = (2 + lo)/(2 - lo).
+static Pre2Coef pre2_chebyshev(double lo) {
+    const double th = 0.5 * (2.0 + lo), sg = th / (0.5 * (2.0 - lo));
+    const float g = (float)(1.0 / th);
+    return Pre2Coef{g, (float)(1.0 / (1.0 - 1.0 / (2.0 * sg * sg))), g};
+}
+
+// Two smoothing steps from z = 0 in one pass: z1 = c1 D^-1 r is formed on the
+// fly at the cell and its neighbours, z = c2 (z1 + c3 D^-1 (r - A z1)).
 template <int VEC>
 __global__ __launch_bounds__(256) void pre2_kernel(Tiles T, const uint8_t* __restrict__ code,
                                                    const float* __restrict__ r, float* __restrict__ z,
-                                                   const int* __restrict__ done) {
+                                                   const int* __restrict__ done, Pre2Coef K) {
     if (done && *done) return;
     int i, j, k0;
     int64_t c0;
@@ -291,18 +312,18 @@ __global__ __launch_bounds__(256) void pre2_kernel(Tiles T, const uint8_t* __res
 #pragma unroll
     for (int e = 0; e < VEC; ++e) {
         rc[e] = nb.vc[e];
-        nb.vc[e] *= winv(nb.cc[e]);
-        nb.vl[e] *= winv(nb.cl[e]);
-        nb.vr[e] *= winv(nb.cr[e]);
-        nb.vd[e] *= winv(nb.cd[e]);
-        nb.vu[e] *= winv(nb.cu[e]);
+        nb.vc[e] *= winv(nb.cc[e], K.c1);
+        nb.vl[e] *= winv(nb.cl[e], K.c1);
+        nb.vr[e] *= winv(nb.cr[e], K.c1);
+        nb.vd[e] *= winv(nb.cd[e], K.c1);
+        nb.vu[e] *= winv(nb.cu[e], K.c1);
     }
-    nb.vb *= winv(nb.cb);
-    nb.vf *= winv(nb.cf);
+    nb.vb *= winv(nb.cb, K.c1);
+    nb.vf *= winv(nb.cf, K.c1);
 #pragma unroll
     for (int e = 0; e < VEC; ++e) {
         const float az = nb.apply(e);
-        out[e] = nb.cc[e] >= kCodeFluid0 ? nb.vc[e] + winv(nb.cc[e]) * (rc[e] - az) : 0.0f;
+        out[e] = nb.cc[e] >= kCodeFluid0 ? K.c2 * (nb.vc[e] + winv(nb.cc[e], K.c3) * (rc[e] - az)) : 0.0f;
     }
     stf<VEC>(z, c0, out);
 }
@@ -363,12 +384,13 @@ static void chebyshev(hipStream_t st, const Tiles& T, int vec, const uint8_t* co
                       float* tmp, int steps, double lo, const int* done);
 
 // Last smoothing sweep of the V-cycle fused with the flexible-CG products:
-// zo = zi + wD^-1 (r - A zi) and one partial of r.zo and of r.z_old per tile.
+// zo = zi + w D^-1 (r - A zi) and one partial of r.zo and of r.z_old per tile.
 template <int VEC>
 __global__ __launch_bounds__(256) void smooth_dots_kernel(Tiles T, const uint8_t* __restrict__ code,
                                                           const float* __restrict__ r, const float* __restrict__ zi,
                                                           float* __restrict__ zo, const float* __restrict__ z_old,
-                                                          double* __restrict__ part, const int* __restrict__ done) {
+                                                          double* __restrict__ part, const int* __restrict__ done,
+                                                          float w) {
     if (done && *done) return;
     double a = 0.0, b = 0.0;
     int i, j, k0;
@@ -382,7 +404,7 @@ __global__ __launch_bounds__(256) void smooth_dots_kernel(Tiles T, const uint8_t
         if (z_old) ldf_nt<VEC>(z_old, c0, zo_);
 #pragma unroll
         for (int e = 0; e < VEC; ++e) {
-            out[e] = nb.cc[e] >= kCodeFluid0 ? nb.vc[e] + winv(nb.cc[e]) * (rv[e] - nb.apply(e)) : 0.0f;
+            out[e] = nb.cc[e] >= kCodeFluid0 ? nb.vc[e] + winv(nb.cc[e], w) * (rv[e] - nb.apply(e)) : 0.0f;
             a += (double)rv[e] * (double)out[e];
             if (z_old) b += (double)rv[e] * (double)zo_[e];
         }
@@ -610,9 +632,9 @@ __global__ __launch_bounds__(256) void prolong4_kernel(Tiles F, Tiles CV, const 
 // nz % 4 == 0, 16-byte aligned vectors).  A workgroup owns a 12 x 56 (j,k)
 // tile and walks down an even-length i segment; each of its 256 threads holds
 // one 4-cell k chunk of the tile or of its ring (two rows in j, one chunk in
-// k).  At plane q it forms the two-sweep smoother z = z1 + wD^-1 (r - A z1),
-// z1 = wD^-1 r, over the tile and a one-cell ring (pre2_kernel's arithmetic,
-// so z is bit-identical), then the residual r - A z at plane q-1 over the
+// k).  At plane q it forms the two-step smoother z = c2 (z1 + c3 D^-1 (r - A
+// z1)), z1 = c1 D^-1 r, over the tile and a one-cell ring (pre2_kernel's
+// arithmetic, so z is bit-identical), then the residual r - A z at plane q-1 over the
 // tile, which it sums over the 2x2x2 children of each coarse cell (times 0.5,
 // the restriction of restrict4_kernel; only the order of the sum differs).
 // i-neighbours are the thread's registers, j/k-neighbours come from LDS
@@ -655,7 +677,8 @@ __device__ __forceinline__ uint8_t byte_of(uint32_t w, int e) { return (uint8_t)
 __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int nz, int ni_seg, int jtiles,
                                                             int ktiles, const uint8_t* __restrict__ code,
                                                             const float* __restrict__ r, float* __restrict__ z,
-                                                            float* __restrict__ rc, const int* __restrict__ done) {
+                                                            float* __restrict__ rc, const int* __restrict__ done,
+                                                            Pre2Coef K) {
     if (done && *done) return;
     constexpr int CH = kPrKT + 2, SR = kPrTJ + 4, W = 4 * CH;
     static_assert(CH * SR == 256, "one thread per chunk of the tile and its ring");
@@ -723,15 +746,15 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
                 const float dd[4] = {d4.x, d4.y, d4.z, d4.w}, uu[4] = {u4.x, u4.y, u4.z, u4.w};
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    vc[e] = rcv[e] * winv(byte_of(Kc, e));
-                    vl[e] = l[e] * winv(byte_of(Km, e));
-                    vr[e] = n[e] * winv(byte_of(Kn, e));
-                    vd[e] = dd[e] * winv(byte_of(kd, e));
-                    vu[e] = uu[e] * winv(byte_of(ku, e));
+                    vc[e] = rcv[e] * winv(byte_of(Kc, e), K.c1);
+                    vl[e] = l[e] * winv(byte_of(Km, e), K.c1);
+                    vr[e] = n[e] * winv(byte_of(Kn, e), K.c1);
+                    vd[e] = dd[e] * winv(byte_of(kd, e), K.c1);
+                    vu[e] = uu[e] * winv(byte_of(ku, e), K.c1);
                 }
             }
-            vb *= winv(cb);
-            vf *= winv(cf);
+            vb *= winv(cb, K.c1);
+            vf *= winv(cf, K.c1);
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const uint8_t cc = byte_of(Kc, e);
@@ -747,7 +770,7 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
                 s = s + nbr_term(c_b, v_b, ps);
                 s = s + nbr_term(c_f, v_f, ps);
                 const float az = 6.0f * ps - s;
-                Zc[e] = ps + winv(cc) * (rcv[e] - az);
+                Zc[e] = K.c2 * (ps + winv(cc, K.c3) * (rcv[e] - az));
             }
             if (inner && q >= i0 && q < i1)
                 *reinterpret_cast<float4*>(z + (int64_t)q * pl + off) = make_float4(Zc[0], Zc[1], Zc[2], Zc[3]);
@@ -944,13 +967,13 @@ static void restrict_level(hipStream_t st, const Tiles& TF, int vf, const MLvl& 
 // the restriction kernels.
 static int pre_restrict(hipStream_t st, const Tiles& TF, int vf, const MLvl& LF, const MLvl& LC, const Tiles& TC1,
                         const uint8_t* code, const float* r, float* z, float* rc, const int* done, int fused,
-                        int target_wg) {
+                        int target_wg, Pre2Coef K) {
     if (fused && pr_fits(LF, vf)) {
         const PrShape S = pr_shape(LF.nx, LF.ny, LF.nz, target_wg);
         pre2_restrict_kernel<<<S.grid, 256, 0, st>>>(LF.nx, LF.ny, LF.nz, S.ni, S.jtiles, S.ktiles, code, r, z, rc,
-                                                     done);
+                                                     done, K);
     } else {
-        MG_VEC(pre2_kernel, vf, TF.ntiles, st, TF, code, r, z, done);
+        MG_VEC(pre2_kernel, vf, TF.ntiles, st, TF, code, r, z, done, K);
         restrict_level(st, TF, vf, LF, LC, TC1, code, r, z, rc, done);
     }
     return (int)hipGetLastError();
@@ -987,11 +1010,12 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
     const Tiles C2 = make_tiles(W.L2.nx, W.L2.ny, W.L2.nz, 1);
     const unsigned g2 = grid_for(W.L2.n);
     // fine: two pre-smoothing sweeps from zero, restriction of the residual
-    int rc = pre_restrict(st, T0, v0, W.L0, W.L1, C1, code0, r, z, W.r1, done, 1, 7680);
+    const Pre2Coef K0 = pre2_chebyshev(kPreLo), K1 = K0;
+    int rc = pre_restrict(st, T0, v0, W.L0, W.L1, C1, code0, r, z, W.r1, done, 1, 7680, K0);
     if (rc != hipSuccess) return rc;
     // level 1 (the single pass walks its i segments plane by plane: with the
     // few workgroups this level fills, that chain is slower than two kernels)
-    rc = pre_restrict(st, T1, v1, W.L1, W.L2, C2, W.code1, W.r1, W.z1, W.r2, done, 0, 0);
+    rc = pre_restrict(st, T1, v1, W.L1, W.L2, C2, W.code1, W.r1, W.z1, W.r2, done, 0, 0, K1);
     if (rc != hipSuccess) return rc;
     // level 2: coarse solve.  coarse_kind 0: one damped-Jacobi sweep from
     // zero, then the polish; 1: the network (A-norm-optimal step), then the
@@ -1037,7 +1061,7 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
     rc = prolong_level(st, T0, v0, W.L0, W.L1, C1, code0, W.z1, z, W.zt0, done);
     if (rc != hipSuccess) return rc;
     if (T0.ntiles != mg_dots_groups(nx, ny, nz)) return (int)hipErrorInvalidValue;  // misaligned caller buffers
-    MG_VEC(smooth_dots_kernel, v0, T0.ntiles, st, T0, code0, r, W.zt0, z, z_old, dots_part, done);
+    MG_VEC(smooth_dots_kernel, v0, T0.ntiles, st, T0, code0, r, W.zt0, z, z_old, dots_part, done, kPostW);
     return (int)hipGetLastError();
 }
 
@@ -1059,5 +1083,6 @@ extern "C" int bs_mg_smooth_restrict(const unsigned char* code, const float* r, 
     if (fused && !pr_fits(LF, vf)) return (int)hipErrorInvalidValue;
     const Tiles TF = make_tiles(nx, ny, nz, vf);
     const Tiles TC1 = make_tiles(LC.nx, LC.ny, LC.nz, 1);
-    return pre_restrict((hipStream_t)stream, TF, vf, LF, LC, TC1, code, r, z, rc, nullptr, fused, 4096);
+    return pre_restrict((hipStream_t)stream, TF, vf, LF, LC, TC1, code, r, z, rc, nullptr, fused, 4096,
+                        Pre2Coef{kOmega, 1.0f, kOmega});
 }
