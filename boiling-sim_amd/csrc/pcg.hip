@@ -557,15 +557,17 @@ constexpr unsigned kMaxWg = 1024;
 struct HostPoll {
     int* slots = nullptr;      // [2][2]: (done, iter) of the last two batches
     PcgCtrl* final = nullptr;  // full control block after a solve
+    PcgCtrl* ctl = nullptr;    // [2]: the control block after each of the last two batches
 };
 static HostPoll g_poll;
 
 static int ensure_poll() {
     if (g_poll.slots) return hipSuccess;
     void* h = nullptr;
-    BS_RETURN_IF_ERR(hipHostMalloc(&h, 512, hipHostMallocDefault));
+    BS_RETURN_IF_ERR(hipHostMalloc(&h, 1024, hipHostMallocDefault));
     g_poll.slots = (int*)h;
     g_poll.final = (PcgCtrl*)((char*)h + 256);
+    g_poll.ctl = (PcgCtrl*)((char*)h + 512);
     return hipSuccess;
 }
 
@@ -1124,37 +1126,50 @@ static int fcg_iterate(const Tiles& T, int vec, const FcgBuffers& F, int max_ite
     hipEvent_t ev[2];
     BS_RETURN_IF_ERR(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     BS_RETURN_IF_ERR(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-    const size_t done_off = offsetof(PcgCtrl, done);
-    // One pair per poll: a pair is milliseconds of GPU work, so the host stays
-    // ahead, and at most two queued pairs run (as no-ops) after convergence.
-    const int batch = 1;
-    int it = 0, slot = 0;
-    bool pending[2] = {false, false};
-    g_poll.slots[0] = g_poll.slots[2] = 0;
-    while (it < max_iter && rc == hipSuccess) {
-        for (int q = 0; q < batch && it < max_iter && rc == hipSuccess; ++q, it += 2) {
+    // Iteration pairs are queued two deep, each followed by a copy of the
+    // control block into a pinned slot, and polled oldest first.  A pair is
+    // milliseconds of GPU work, so the host stays ahead; once the residual's
+    // reduction per pair predicts that the pairs in flight reach the stopping
+    // test, no further pair is queued until that is confirmed, so at most the
+    // rest of the converging pair runs as no-ops (a wrong prediction costs
+    // one host round trip of GPU idle time).
+    int it = 0, head = 0, inflight = 0;
+    double rr_prev = -1.0, rr_last = -1.0, stop = -1.0;
+    bool finished = false;
+    while (!finished && rc == hipSuccess) {
+        int want = 2;
+        if (rr_prev > 0.0 && rr_last > 0.0 && inflight > 0) {
+            const double q = rr_last / rr_prev;  // per pair
+            double pred = rr_last;
+            for (int k = 0; k < inflight; ++k) pred *= q;
+            if (q < 1.0 && pred < stop) want = inflight;
+        }
+        while (inflight < want && it < max_iter && rc == hipSuccess) {
+            const int slot = (head + inflight) & 1;
             if (eager) {
                 rc = fcg_enqueue_iteration(T, vec, F, 0, st);
                 if (rc == hipSuccess) rc = fcg_enqueue_iteration(T, vec, F, 1, st);
             } else {
                 rc = hipGraphLaunch(exec, st);
             }
+            if (rc == hipSuccess)
+                rc = hipMemcpyAsync(g_poll.ctl + slot, B.ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, st);
+            if (rc == hipSuccess) rc = hipEventRecord(ev[slot], st);
+            if (rc == hipSuccess) {
+                ++inflight;
+                it += 2;
+            }
         }
+        if (rc != hipSuccess || inflight == 0) break;
+        rc = hipEventSynchronize(ev[head]);
         if (rc != hipSuccess) break;
-        rc = hipMemcpyAsync(g_poll.slots + 2 * slot, (const char*)B.ctrl + done_off, 2 * sizeof(int),
-                            hipMemcpyDeviceToHost, st);
-        if (rc != hipSuccess) break;
-        rc = hipEventRecord(ev[slot], st);
-        if (rc != hipSuccess) break;
-        pending[slot] = true;
-        const int prev = slot ^ 1;
-        if (pending[prev]) {
-            rc = hipEventSynchronize(ev[prev]);
-            if (rc != hipSuccess) break;
-            pending[prev] = false;
-            if (g_poll.slots[2 * prev] != 0) break;
-        }
-        slot ^= 1;
+        const PcgCtrl& h = g_poll.ctl[head];
+        head ^= 1;
+        --inflight;
+        if (h.done) finished = true;
+        stop = h.stop;
+        rr_prev = rr_last;
+        rr_last = h.rr;
     }
     if (rc == hipSuccess) {
         rc = hipMemcpyAsync(g_poll.final, B.ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, st);
