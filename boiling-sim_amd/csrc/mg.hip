@@ -1068,13 +1068,14 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
 }  // namespace bs
 
 // The V-cycle's first pass on one level (diagnostic entry, no counterpart in
-// the reference): z = two damped-Jacobi sweeps from zero on A z = r, and
-// rc[I,J,K] = 0.5 x the sum of r - A z over the 8 children of coarse cell
-// (I,J,K) (coarse dims (n+1)/2).  fused = 1 runs the single-pass kernel and is
-// an error unless nx, ny are even, nz % 4 == 0 and the vectors are 16-byte
-// aligned; fused = 0 runs the two-kernel path.
+// the reference): z = two smoothing steps from zero on A z = r — damped
+// Jacobi (w = 0.8) for lo <= 0, the two-step Chebyshev polynomial over
+// [lo, 2] otherwise — and rc[I,J,K] = 0.5 x the sum of r - A z over the 8
+// children of coarse cell (I,J,K) (coarse dims (n+1)/2).  fused = 1 runs the
+// single-pass kernel and is an error unless nx, ny are even, nz % 4 == 0 and
+// the vectors are 16-byte aligned; fused = 0 runs the two-kernel path.
 extern "C" int bs_mg_smooth_restrict(const unsigned char* code, const float* r, float* z, float* rc, int nx, int ny,
-                                     int nz, int fused, bs_stream_t stream) {
+                                     int nz, int fused, double lo, bs_stream_t stream) {
     using namespace bs;
     if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
     const MLvl LF = mk(nx, ny, nz);
@@ -1083,6 +1084,22 @@ extern "C" int bs_mg_smooth_restrict(const unsigned char* code, const float* r, 
     if (fused && !pr_fits(LF, vf)) return (int)hipErrorInvalidValue;
     const Tiles TF = make_tiles(nx, ny, nz, vf);
     const Tiles TC1 = make_tiles(LC.nx, LC.ny, LC.nz, 1);
+    if (lo >= 2.0) return (int)hipErrorInvalidValue;
     return pre_restrict((hipStream_t)stream, TF, vf, LF, LC, TC1, code, r, z, rc, nullptr, fused, 4096,
-                        Pre2Coef{kOmega, 1.0f, kOmega});
+                        lo > 0.0 ? pre2_chebyshev(lo) : Pre2Coef{kOmega, 1.0f, kOmega});
+}
+
+// `steps` Chebyshev steps (even) over [lo, 2] of D^-1 A on A x = r from x,
+// the result in x (tmp: device scratch of the same size) — the V-cycle's
+// post-smoother and coarsest-level polish (diagnostic entry, no counterpart
+// in the reference).
+extern "C" int bs_mg_chebyshev(const unsigned char* code, const float* r, float* x, float* tmp, int nx, int ny,
+                               int nz, int steps, double lo, bs_stream_t stream) {
+    using namespace bs;
+    if (nx <= 0 || ny <= 0 || nz <= 0 || steps == 0) return hipSuccess;
+    if (steps < 0 || (steps & 1) || !(lo > 0.0 && lo < 2.0)) return (int)hipErrorInvalidValue;
+    const int v = vec_for(nz, {r, x, tmp, code});
+    const Tiles T = make_tiles(nx, ny, nz, v);
+    chebyshev((hipStream_t)stream, T, v, code, r, x, tmp, steps, lo, nullptr);
+    return (int)hipGetLastError();
 }

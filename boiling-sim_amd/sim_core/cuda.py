@@ -286,13 +286,21 @@ class SimCore:
         )
         return st.as_dict()
 
-    def mg_smooth_restrict(self, code_ptr, r_ptr, z_ptr, rc_ptr, nx, ny, nz, fused=True, stream=0) -> None:
-        """The V-cycle's first pass on one level: z = two damped-Jacobi sweeps
-        from zero, rc = 0.5 x the children's sum of r - A z (see
-        bs_mg_smooth_restrict); fused selects the single-pass kernel."""
+    def mg_smooth_restrict(self, code_ptr, r_ptr, z_ptr, rc_ptr, nx, ny, nz, fused=True, lo=0.0, stream=0) -> None:
+        """The V-cycle's first pass on one level: z = two smoothing steps from
+        zero (damped Jacobi for lo <= 0, Chebyshev over [lo, 2] otherwise), rc =
+        0.5 x the children's sum of r - A z (see bs_mg_smooth_restrict); fused
+        selects the single-pass kernel."""
         _lib.call("bs_mg_smooth_restrict", _ptr("code_ptr", code_ptr), _ptr("r_ptr", r_ptr), _ptr("z_ptr", z_ptr),
                   _ptr("rc_ptr", rc_ptr), _usize("nx", nx), _usize("ny", ny), _usize("nz", nz), 1 if fused else 0,
-                  _ptr("stream", stream))
+                  float(lo), _ptr("stream", stream))
+
+    def mg_chebyshev(self, code_ptr, r_ptr, x_ptr, tmp_ptr, nx, ny, nz, steps, lo, stream=0) -> None:
+        """`steps` Chebyshev steps over [lo, 2] on A x = r from and into x
+        (see bs_mg_chebyshev)."""
+        _lib.call("bs_mg_chebyshev", _ptr("code_ptr", code_ptr), _ptr("r_ptr", r_ptr), _ptr("x_ptr", x_ptr),
+                  _ptr("tmp_ptr", tmp_ptr), _usize("nx", nx), _usize("ny", ny), _usize("nz", nz), int(steps),
+                  float(lo), _ptr("stream", stream))
 
     # ---- mixed-precision iterative refinement -------------------------------
     STORAGE = {"fp32": 0, "fp16": 1, "bf16": 2}

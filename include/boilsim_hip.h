@@ -259,15 +259,23 @@ int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx, int ny, i
                  int precond, const float* unet_params, const void* unet_packed, void* workspace,
                  size_t workspace_bytes, bs_stream_t stream, bs_fcg_stats* stats);
 /* The V-cycle's first pass on one level (diagnostic; no counterpart in the
- * reference): z = two damped-Jacobi sweeps (w = 0.8) from zero on A z = r
- * over class codes `code` (0 air, 1 solid, 2 + solid neighbours for fluid),
- * and rc[I,J,K] = 0.5 x the sum of r - A z over the fluid children of coarse
- * cell (I,J,K), coarse dims ((nx+1)/2, (ny+1)/2, (nz+1)/2).  fused = 1 runs
- * the single-pass kernel the V-cycle uses where the shape allows (nx, ny even,
- * nz % 4 == 0, 16-byte aligned vectors; otherwise an error), fused = 0 the
- * two-kernel path.  Asynchronous on `stream`. */
+ * reference): z = two smoothing steps from zero on A z = r over class codes
+ * `code` (0 air, 1 solid, 2 + solid neighbours for fluid) — damped Jacobi
+ * (w = 0.8) for lo <= 0, the two-step Chebyshev polynomial over [lo, 2] of
+ * D^-1 A otherwise (the V-cycle uses lo = 0.3) — and rc[I,J,K] = 0.5 x the sum
+ * of r - A z over the fluid children of coarse cell (I,J,K), coarse dims
+ * ((nx+1)/2, (ny+1)/2, (nz+1)/2).  fused = 1 runs the single-pass kernel the
+ * V-cycle uses where the shape allows (nx, ny even, nz % 4 == 0, 16-byte
+ * aligned vectors; otherwise an error), fused = 0 the two-kernel path.
+ * Asynchronous on `stream`. */
 int bs_mg_smooth_restrict(const unsigned char* code, const float* r, float* z, float* rc, int nx, int ny, int nz,
-                          int fused, bs_stream_t stream);
+                          int fused, double lo, bs_stream_t stream);
+/* `steps` (even, > 0) Chebyshev steps over [lo, 2] of D^-1 A on A x = r from
+ * x, the result in x (tmp: device scratch of nx*ny*nz floats): the V-cycle's
+ * post-smoother (4 steps, lo = 0.02) and coarsest-level polish (20 steps,
+ * lo = 0.02).  Diagnostic; no counterpart in the reference. */
+int bs_mg_chebyshev(const unsigned char* code, const float* r, float* x, float* tmp, int nx, int ny, int nz,
+                    int steps, double lo, bs_stream_t stream);
 /* ---- Mixed-precision iterative refinement ------------------------------
  * No single counterpart in the reference: it wraps the Jacobi-PCG of
  * crates/cuda-kernels/src/pressure_solve_pcg.cu:89-243 (same A, M and b) in

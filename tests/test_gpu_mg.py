@@ -42,14 +42,31 @@ def _apply(code, v):
     return np.where(code >= 2, 6.0 * v - s, 0.0)
 
 
-def _reference(code, r):
-    """numpy restatement (FP64) of the pass: z1 = w D^-1 r, z = z1 + w D^-1 (r - A z1),
+def _dinv(code):
+    fl = code >= 2
+    return np.where(fl, 1.0 / np.where(fl, 8.0 - code, 1.0), 0.0)
+
+
+def _reference(code, r, lo=0.0):
+    """numpy restatement (FP64) of the pass: z = q(M) D^-1 r with M = D^-1 A and
+    q the smoother's polynomial — damped Jacobi twice from zero, q = 2w - w^2 M
+    (w = 0.8), or the two-step Chebyshev polynomial over [lo, 2] in closed form,
+    q(l) = (1 - T2(y(l)) / T2(s)) / l, y = s - 2 l / (2 - lo), s = (2 + lo) / (2 - lo)
+    (written out below, independent of the kernels' coefficients) — and
     rc = 0.5 * sum over children of (r - A z)."""
     fl = code >= 2
-    winv = np.where(fl, 0.8 / np.where(fl, 8.0 - code, 1.0), 0.0)
+    dinv = _dinv(code)
     r = np.where(fl, r.astype(np.float64), 0.0)
-    z1 = winv * r
-    z = np.where(fl, z1 + winv * (r - _apply(code, z1)), 0.0)
+    d = dinv * r
+    md = dinv * _apply(code, d)
+    if lo > 0:
+        b, a = 2.0, lo
+        s = (b + a) / (b - a)
+        q0 = 8.0 * s / ((b - a) * (2 * s * s - 1))
+        q1 = -8.0 / ((b - a) ** 2 * (2 * s * s - 1))
+    else:
+        q0, q1 = 2 * 0.8, -0.8 * 0.8
+    z = np.where(fl, q0 * d + q1 * md, 0.0)
     res = np.where(fl, r - _apply(code, z), 0.0)
     nx, ny, nz = code.shape
     rc = np.zeros(((nx + 1) // 2, (ny + 1) // 2, (nz + 1) // 2))
@@ -83,8 +100,10 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("lo", [0.0, 0.3])
 @pytest.mark.parametrize("kind,shape", CASES)
-def test_fused_pass_matches_two_kernels(simcore, torch_gpu, kind, shape):
+def test_fused_pass_matches_two_kernels(simcore, torch_gpu, kind, shape, lo):
+    """lo = 0: damped Jacobi; lo = 0.3: the V-cycle's Chebyshev pre-smoother."""
     torch = torch_gpu
     rng = np.random.default_rng(sum(shape))
     mat = _mask(kind, shape, rng)
@@ -96,7 +115,7 @@ def test_fused_pass_matches_two_kernels(simcore, torch_gpu, kind, shape):
     for fused in (1, 0):
         z = torch.full(shape, np.nan, dtype=torch.float32, device="cuda")
         rc = torch.full(cshape, np.nan, dtype=torch.float32, device="cuda")
-        simcore.mg_smooth_restrict(cd.data_ptr(), rd.data_ptr(), z.data_ptr(), rc.data_ptr(), *shape, fused)
+        simcore.mg_smooth_restrict(cd.data_ptr(), rd.data_ptr(), z.data_ptr(), rc.data_ptr(), *shape, fused, lo)
         torch.cuda.synchronize()
         out[fused] = (host(z), host(rc))
     (zf, rcf), (zu, rcu) = out[1], out[0]
@@ -109,7 +128,7 @@ def test_fused_pass_matches_two_kernels(simcore, torch_gpu, kind, shape):
     scale = np.abs(rcu).max()
     np.testing.assert_allclose(rcf, rcu, rtol=0, atol=2e-6 * scale)
     # both against the FP64 restatement
-    z64, rc64 = _reference(code, r)
+    z64, rc64 = _reference(code, r, lo)
     np.testing.assert_allclose(zu, z64, rtol=0, atol=1e-6 * np.abs(z64).max())
     np.testing.assert_allclose(rcf, rc64, rtol=0, atol=2e-6 * np.abs(rc64).max())
 
@@ -126,3 +145,72 @@ def test_fused_pass_rejects_unsupported_shapes(simcore, torch_gpu):
         simcore.mg_smooth_restrict(code.data_ptr(), r.data_ptr(), z.data_ptr(), rc.data_ptr(), *shape, 0)
     torch.cuda.synchronize()
 
+
+
+def _chebyshev_reference(code, r, x0, steps, lo):
+    """x_m = x* + T_m(Y)(x0 - x*) / T_m(s) for A x* = r on the fluid cells, with
+    Y = s I - (2 / (b - a)) D^-1 A, s = (b + a) / (b - a), [a, b] = [lo, 2]: the
+    Chebyshev polynomial through its three-term recurrence, a formulation
+    independent of the semi-iterative one the kernels use."""
+    import scipy.sparse.linalg as spla
+    from oracle import np_ref
+
+    fl = (code >= 2).ravel()
+    mat = np.where(code >= 2, 0, np.where(code == 1, 1, 2)).astype(np.int32)
+    A = np_ref.laplacian_csr(mat).tocsr()
+    idx = np.flatnonzero(fl)
+    Af = A[idx][:, idx].tocsc()
+    xs = np.zeros(code.size)
+    xs[idx] = spla.spsolve(Af, r.ravel().astype(np.float64)[idx])
+    dinv = _dinv(code).ravel()
+    b, a = 2.0, lo
+    s = (b + a) / (b - a)
+
+    def Y(v):
+        return s * v - (2.0 / (b - a)) * dinv * (A @ v)
+
+    e0 = np.where(fl, x0.ravel().astype(np.float64) - xs, 0.0)
+    v_prev, v = e0, Y(e0)
+    t_prev, t = 1.0, s
+    for _ in range(steps - 1):
+        v_prev, v = v, 2.0 * Y(v) - v_prev
+        t_prev, t = t, 2.0 * s * t - t_prev
+    return (xs + v / t).reshape(code.shape)
+
+
+@pytest.mark.parametrize("steps,lo", [(4, 0.02), (20, 0.02), (6, 0.3)])
+@pytest.mark.parametrize("kind,shape", [("pot", (24, 24, 16)), ("random", (13, 10, 9))])
+def test_chebyshev_matches_polynomial(simcore, torch_gpu, kind, shape, steps, lo):
+    """The V-cycle's Chebyshev steps (post-smoother: 4 over [0.02, 2]; polish:
+    20 over [0.02, 2]) against the Chebyshev error polynomial in FP64."""
+    torch = torch_gpu
+    rng = np.random.default_rng(3 + steps + sum(shape))
+    mat = _mask(kind, shape, rng)
+    code = _codes(mat)
+    fl = code >= 2
+    # A x* = r with a known smooth-plus-noise x*, started from noise
+    xs = np.where(fl, rng.standard_normal(shape), 0.0)
+    r = np.where(fl, _apply(code, xs), 0.0).astype(np.float32)
+    x0 = np.where(fl, rng.standard_normal(shape), 0.0).astype(np.float32)
+    x = dev(torch, x0.copy())
+    tmp = torch.full(shape, np.nan, dtype=torch.float32, device="cuda")
+    cd, rd = dev(torch, code), dev(torch, r)  # held: the kernels run after the call returns
+    simcore.mg_chebyshev(cd.data_ptr(), rd.data_ptr(), x.data_ptr(), tmp.data_ptr(), *shape, steps, lo)
+    torch.cuda.synchronize()
+    got = host(x)
+    ref = _chebyshev_reference(code, r, x0, steps, lo)
+    err0 = np.abs(x0 - ref).max()
+    print(f"[{kind} {shape} {steps} steps over [{lo}, 2]] max|x - ref| {np.abs(got - ref).max():.2e}, "
+          f"max|x0 - ref| {err0:.2e}")
+    assert not got[~fl].any()
+    np.testing.assert_allclose(got, ref, rtol=0, atol=2e-5 * max(np.abs(ref).max(), 1.0))
+
+
+def test_chebyshev_rejects_bad_arguments(simcore, torch_gpu):
+    torch = torch_gpu
+    shape = (8, 8, 8)
+    code = dev(torch, _codes(np.zeros(shape, np.int32)))
+    z = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    for steps, lo in ((3, 0.1), (-2, 0.1), (4, 0.0), (4, 2.0)):
+        with pytest.raises(RuntimeError):
+            simcore.mg_chebyshev(code.data_ptr(), z.data_ptr(), z.data_ptr(), z.data_ptr(), *shape, steps, lo)
