@@ -1202,10 +1202,10 @@ extern "C" size_t bs_fcg_workspace_bytes(int nx, int ny, int nz, int precond) {
 }
 
 // Flexible CG with preconditioner `precond` (1: U-Net on this grid, 2: V-cycle
-// with the U-Net on the 4x-coarsened grid followed by Jacobi polishing, 3: the
-// same V-cycle with Jacobi sweeps there instead of the network, 4: Jacobi
-// sweeps, the U-Net on their residual, Jacobi sweeps), to a true relative
-// residual tol; on a stall it continues with plain Jacobi-PCG from x.
+// with the U-Net on the 4x-coarsened grid followed by a Chebyshev polish, 3:
+// the same V-cycle with one Jacobi sweep there instead of the network, 4:
+// Jacobi sweeps, the U-Net on their residual, Jacobi sweeps), to a true
+// relative residual tol; on a stall it continues with plain Jacobi-PCG from x.
 extern "C" int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx, int ny, int nz, float dx,
                             float dt, float rho, int mat_fluid, int mat_air, double tol, int max_iter,
                             int warm_start, int precond, const float* unet_params, const void* unet_packed,

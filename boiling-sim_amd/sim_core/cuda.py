@@ -271,7 +271,8 @@ class SimCore:
                   unet_params_ptr, unet_packed_ptr, workspace_ptr, workspace_bytes, warm_start=False,
                   mat_fluid=0, mat_air=2, stream=0) -> dict:
         """Flexible CG with preconditioner kind ``precond`` (1 U-Net, 2 V-cycle +
-        U-Net coarse solve, 3 V-cycle + Jacobi coarse sweeps) to a true relative
+        U-Net coarse solve + Chebyshev polish, 3 the same V-cycle without the
+        network, 4 Jacobi sweeps + U-Net on their residual) to a true relative
         residual ``tol``; falls back to plain PCG on a stall."""
         st = _lib.FcgStats()
         _lib.call(

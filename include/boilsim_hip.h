@@ -248,11 +248,13 @@ typedef struct {
 /* Workspace bytes of bs_fcg_solve for preconditioner kind `precond`. */
 size_t bs_fcg_workspace_bytes(int nx, int ny, int nz, int precond);
 /* Flexible CG (Polak-Ribiere beta) with a learned preconditioner, to a true
- * relative residual tol: precond 1 = the U-Net on this grid, 2 = a damped-
- * Jacobi V-cycle on this grid and its 2x coarsening with the U-Net solving the
- * 4x-coarsened grid (its correction polished by Jacobi sweeps), 3 = the same
- * V-cycle with Jacobi sweeps on the coarsest grid (no network), 4 = as 2 with
- * sweeps before the network, which then corrects their residual.  A stall (no 2 % residual drop in 20 iterations) hands
+ * relative residual tol: precond 1 = the U-Net on this grid, 2 = a V-cycle on
+ * this grid and its 2x coarsening (smoothers: Chebyshev polynomials in the
+ * Jacobi splitting) with the U-Net solving the 4x-coarsened grid, its
+ * correction polished by Chebyshev steps, 3 = the same V-cycle with one
+ * Jacobi sweep and the same polish on the coarsest grid (no network), 4 = as
+ * 2 with Jacobi sweeps before the network, which then corrects their
+ * residual.  A stall (no 2 % residual drop in 20 iterations) hands
  * over to plain Jacobi-PCG from the current x. */
 int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx, int ny, int nz, float dx, float dt,
                  float rho, int mat_fluid, int mat_air, double tol, int max_iter, int warm_start,
