@@ -45,6 +45,12 @@ METRIC = "ms/projection to rel-res 1e-6 at 440x440x280: learned-PCG vs plain PCG
 # Algorithmic bytes per cell (FP32 fields, one-byte class code where used).
 B_JACOBI = 12.0   # p_in, div, p_out (class carried in p, no mask read)
 B_PCG_ITER = 47.0  # spmv 4+1+4, update 4+4+(4+4)+(4+4)+1, direction 4+1+(4+4)
+# Fine-level algorithmic bytes per cell of one learned flexible-CG iteration:
+# spmv 9 (code, p, Ap), update 24 (p, Ap, x, r; x, r), pre-smoothing +
+# residual + restriction 9.5 (code, r; z, rc/8 x 4), prolongation 9.5 (code,
+# z, coarse e/8 x 4; z'), post-sweep + dots 17 (code, z', r, z_old; z),
+# direction 12 (z, p; p).
+B_LEARNED_FINE = 81.0
 # HBM bytes per cell one two-sweep launch of jacobi_pair_kernel actually
 # moves at 440x440x280, measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (one counter per run) and calibrated by kernels of known traffic
@@ -403,6 +409,11 @@ def main():
         jac_gbs = 2 * B_JACOBI * n / (us_pair * 1e-6) / 1e9
     us_iter = ms_p * 1e3 / max(st_p["iterations"], 1) if st_p else nan
     pcg_gbs = B_PCG_ITER * n / (us_iter * 1e-6) / 1e9 if st_p else nan
+    # learned iteration: the solve's median time over its iterations (coarse
+    # levels, the network and set-up included, so a lower bound on the fine
+    # level's rate)
+    us_iter_l = st_l["solve_ms_median"] * 1e3 / max(st_l["iterations"], 1) if st_l else nan
+    learned_gbs = B_LEARNED_FINE * n / (us_iter_l * 1e-6) / 1e9 if st_l else nan
     thermal = None
     if "thermal" in legs:
         thermal = conduction_leg(core, torch, gmat, mat, cfg, nx, ny, nz, n)
@@ -535,6 +546,11 @@ def main():
         "roofline_pcg_iteration": {"bound": "hbm", "achieved": fin(pcg_gbs), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                    "frac": fin(pcg_gbs / HBM_PEAK_GBS), "bytes_per_cell": B_PCG_ITER,
                                    "us_per_iteration": fin(us_iter)},
+        "roofline_learned_iteration": {"bound": "hbm", "achieved": fin(learned_gbs), "peak": HBM_PEAK_GBS,
+                                       "unit": "GB/s", "frac": fin(learned_gbs / HBM_PEAK_GBS),
+                                       "bytes_per_cell": B_LEARNED_FINE, "us_per_iteration": fin(us_iter_l),
+                                       "note": "fine-level bytes over the whole solve time per iteration "
+                                               "(coarse levels, network and set-up included)"},
         "mixed_precision_refinement": mixed,
         "thermal_conduction": thermal,
         "no_slip_faces": faces,
