@@ -901,13 +901,16 @@ void scale_to_optimal(hipStream_t st, int nx, int ny, int nz, const uint8_t* cod
 __global__ void zero_u32(unsigned int* p) { if (threadIdx.x == 0) *p = 0u; }
 
 int mg_setup(hipStream_t st, void* ws, int nx, int ny, int nz, const int* mat, int mf, int ma,
-             double* n_fluid_coarse) {
+             const uint8_t* fcode_in, double* n_fluid_coarse) {
     MgWs W = carve_mg(ws, nx, ny, nz);
-    // Fine codes into a temporary (zt0 is free during set-up), then the two
-    // coarse levels' classes and codes.
-    uint8_t* fcode = (uint8_t*)W.zt0;
+    // The fine codes (given, or formed in a temporary: zt0 is free during
+    // set-up), then the two coarse levels' classes and codes.
+    const uint8_t* fcode = fcode_in;
     zero_u32<<<1, 64, 0, st>>>(W.nfl);
-    codes_from_mat_kernel<<<grid_for(W.L0.n), 256, 0, st>>>(W.L0, mat, mf, ma, fcode, nullptr);
+    if (!fcode) {
+        codes_from_mat_kernel<<<grid_for(W.L0.n), 256, 0, st>>>(W.L0, mat, mf, ma, (uint8_t*)W.zt0, nullptr);
+        fcode = (const uint8_t*)W.zt0;
+    }
     coarse_class_kernel<<<grid_for(W.L1.n), 256, 0, st>>>(W.L0, W.L1, fcode, W.mat1);
     codes_from_mat_kernel<<<grid_for(W.L1.n), 256, 0, st>>>(W.L1, W.mat1, 0, 2, W.code1, nullptr);
     coarse_class_kernel<<<grid_for(W.L2.n), 256, 0, st>>>(W.L1, W.L2, W.code1, W.mat2);

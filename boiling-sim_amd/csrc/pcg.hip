@@ -94,24 +94,32 @@ __global__ __launch_bounds__(256) void pcg_setup_kernel(
         float dv[VEC], bv[VEC], zv[VEC], zero[VEC];
         uint8_t cv[VEC];
         ldf<VEC>(div, c0, dv);
+        // the cell's row of VEC classes and its four neighbour rows as vector
+        // loads, the k-neighbours across the row's ends as scalars; out of
+        // bounds counts as air (not solid)
+        int mv[VEC], ml[VEC], mr[VEC], md[VEC], mu[VEC];
+        ldi<VEC>(mat, c0, mv);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) ml[e] = mr[e] = md[e] = mu[e] = ma;
+        if (i > 0) ldi<VEC>(mat, c0 - plane, ml);
+        if (i < T.nx - 1) ldi<VEC>(mat, c0 + plane, mr);
+        if (j > 0) ldi<VEC>(mat, c0 - T.nz, md);
+        if (j < T.ny - 1) ldi<VEC>(mat, c0 + T.nz, mu);
+        const int mkb = k0 > 0 ? mat[c0 - 1] : ma;
+        const int mkf = k0 + VEC < T.nz ? mat[c0 + VEC] : ma;
 #pragma unroll
         for (int e = 0; e < VEC; ++e) {
             zero[e] = 0.0f;
-            const int k = k0 + e;
-            const int m = mat[c0 + e];
+            const int m = mv[e];
             if (m != mf) {
                 cv[e] = (m == ma) ? kCodeAir : kCodeSolid;
                 bv[e] = 0.0f;
                 zv[e] = 0.0f;
                 continue;
             }
-            int ns = 0;
-            if (i > 0) ns += is_solid_m(mat[c0 + e - plane], mf, ma);
-            if (i < T.nx - 1) ns += is_solid_m(mat[c0 + e + plane], mf, ma);
-            if (j > 0) ns += is_solid_m(mat[c0 + e - T.nz], mf, ma);
-            if (j < T.ny - 1) ns += is_solid_m(mat[c0 + e + T.nz], mf, ma);
-            if (k > 0) ns += is_solid_m(mat[c0 + e - 1], mf, ma);
-            if (k < T.nz - 1) ns += is_solid_m(mat[c0 + e + 1], mf, ma);
+            const int ns = is_solid_m(ml[e], mf, ma) + is_solid_m(mr[e], mf, ma) + is_solid_m(md[e], mf, ma) +
+                           is_solid_m(mu[e], mf, ma) + is_solid_m(e > 0 ? mv[e - 1] : mkb, mf, ma) +
+                           is_solid_m(e < VEC - 1 ? mv[e + 1] : mkf, mf, ma);
             cv[e] = (uint8_t)(kCodeFluid0 + ns);
             bv[e] = bscale * dv[e];
             zv[e] = precond(bv[e], cv[e]);
@@ -1254,7 +1262,7 @@ extern "C" int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx
         F.n_fluid = (double)(unsigned)g_poll.final->pad;
     }
     if (precond >= 2) {
-        const int rc = mg_setup(st, F.pws, nx, ny, nz, mat, mat_fluid, mat_air, &F.n_fluid);
+        const int rc = mg_setup(st, F.pws, nx, ny, nz, mat, mat_fluid, mat_air, B.code, &F.n_fluid);
         if (rc != hipSuccess) return rc;
     }
     fcg_prepare_kernel<<<1, 64, 0, st>>>(B.ctrl, 0);
@@ -1759,7 +1767,7 @@ extern "C" int bs_fcg_refine_solve(double* x, float* x32, const float* div_u, co
         BS_RETURN_IF_ERR(hipStreamSynchronize(st));
         F.n_fluid = (double)h.n_fluid;
     } else {
-        rc = mg_setup(st, F.pws, nx, ny, nz, mat, mat_fluid, mat_air, &F.n_fluid);
+        rc = mg_setup(st, F.pws, nx, ny, nz, mat, mat_fluid, mat_air, B.code, &F.n_fluid);
         if (rc != hipSuccess) return rc;
     }
     int outer = 0, its = 0, conv = 0;

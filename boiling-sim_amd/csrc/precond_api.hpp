@@ -24,9 +24,10 @@ int unet_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const float* pa
 // and its 2x coarsening, with the 4x-coarsened level solved by the U-Net
 // (coarse_kind 1) or by damped Jacobi sweeps (coarse_kind 0).
 size_t mg_ws_bytes(int nx, int ny, int nz);
-// Builds the coarse classes and codes from the fine mask; returns the fluid
-// count of the coarsest level through *n_fluid_coarse (synchronises once).
-int mg_setup(hipStream_t st, void* ws, int nx, int ny, int nz, const int* mat, int mf, int ma,
+// Builds the coarse classes and codes from the fine mask (or from its fine
+// class codes fcode when not null); returns the fluid count of the coarsest
+// level through *n_fluid_coarse (synchronises once).
+int mg_setup(hipStream_t st, void* ws, int nx, int ny, int nz, const int* mat, int mf, int ma, const uint8_t* fcode,
              double* n_fluid_coarse);
 // z = V-cycle(r); also writes partials of r.z and r.z_old (z_old may be null)
 // into dots_part, mg_dots_groups(nx,ny,nz) workgroups of 2 values.
