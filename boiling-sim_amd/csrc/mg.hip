@@ -825,6 +825,201 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Prolongation and post-smoothing of the fine level in one pass (nz % 4 == 0,
+// 16-byte aligned vectors): zt = z_in + P ec on fluid cells (z_in elsewhere),
+// then z_out = zt + w D^-1 (r - A zt) on fluid cells (0 elsewhere) with the
+// partials of r.z_out and r.z_old.  A workgroup owns a 14 x 56 (j,k) tile and
+// walks down an i segment; each of its 256 threads holds one 4-cell k chunk
+// of the tile or of its one-cell ring.  At plane q it forms zt over the tile
+// and ring (prolong4_kernel's float operations), at plane q-1 the sweep over
+// the tile (smooth_dots_kernel's), so z_out is bit-identical to the two
+// kernels'; only the dot products' summation order differs.  The coarse
+// values come from LDS: each coarse plane's 9 x 34 window under the tile is
+// loaded once into a ring of three (a fine plane reads coarse planes q/2 and
+// q/2 -/+ 1); zt planes are double-buffered, codes triple-buffered (the sweep
+// at q-1 still reads plane q-1's codes while plane q+1's are stored).
+// ---------------------------------------------------------------------------
+constexpr int kPpTJ = 14, kPpKT = 14;
+constexpr int kPpER = kPpTJ / 2 + 2, kPpEK = 2 * kPpKT + 6;  // coarse window: 9 rows x 34
+constexpr int kPpEN = kPpER * kPpEK;
+
+struct PpShape {
+    int jtiles, ktiles, ni;
+    unsigned grid;
+};
+
+static PpShape pp_shape(int nx, int ny, int nz) {
+    PpShape s;
+    s.jtiles = (ny + kPpTJ - 1) / kPpTJ;
+    s.ktiles = (nz + 4 * kPpKT - 1) / (4 * kPpKT);
+    const int cols = s.jtiles * s.ktiles;
+    int nseg = (7680 + cols / 2) / cols;
+    nseg = nseg < 1 ? 1 : nseg;
+    int ni = (nx + nseg - 1) / nseg;
+    ni = ni < 4 ? 4 : ni;
+    s.ni = ni;
+    nseg = (nx + ni - 1) / ni;
+    s.grid = (unsigned)(nseg * s.jtiles * s.ktiles);
+    return s;
+}
+
+__global__ __launch_bounds__(256) void prolong_post_kernel(int nx, int ny, int nz, int ni_seg, int jtiles, int ktiles,
+                                                           const uint8_t* __restrict__ code,
+                                                           const float* __restrict__ z_in,
+                                                           const float* __restrict__ ec, const float* __restrict__ r,
+                                                           const float* __restrict__ z_old, float* __restrict__ z_out,
+                                                           double* __restrict__ part, float w,
+                                                           const int* __restrict__ done) {
+    if (done && *done) return;
+    constexpr int CH = kPpKT + 2, SR = kPpTJ + 2, W = 4 * CH;
+    static_assert(CH * SR == 256, "one thread per chunk of the tile and its ring");
+    __shared__ __align__(16) float sZ[2][SR * W + 8];
+    __shared__ __align__(16) uint8_t sC[3][SR * W + 16];
+    __shared__ float sE[3][kPpEN];
+    const int tid = threadIdx.x;
+    const int row = tid / CH, ch = tid - (tid / CH) * CH;
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const int cols = jtiles * ktiles;
+    const int seg = (int)(b / cols), col = (int)(b % cols);
+    const int jt = col / ktiles, kt = col - (col / ktiles) * ktiles;
+    const int j0 = jt * kPpTJ, k0 = kt * 4 * kPpKT;
+    const int i0 = seg * ni_seg, i1 = min(nx, i0 + ni_seg);
+    const int j = j0 - 1 + row, k = k0 - 4 + 4 * ch;
+    const bool jk_in = j >= 0 && j < ny && k >= 0 && k < nz;
+    const bool inner = row >= 1 && row <= kPpTJ && ch >= 1 && ch <= kPpKT && j < ny && k < nz;
+    const int cnx = (nx + 1) / 2, cny = (ny + 1) / 2, cnz = (nz + 1) / 2;
+    const int Jb = j0 / 2 - 1, Kb = k0 / 2 - 3;  // coarse window origin
+    const int64_t pl = (int64_t)ny * nz, cpl = (int64_t)cny * cnz;
+    const int64_t off = (int64_t)j * nz + k;
+    auto ldz = [&](int q) {
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (jk_in && q >= 0 && q < nx) v = *reinterpret_cast<const float4*>(z_in + (int64_t)q * pl + off);
+        return v;
+    };
+    auto ldk = [&](int q) {
+        uint32_t v = 0;
+        if (jk_in && q >= 0 && q < nx) v = *reinterpret_cast<const uint32_t*>(code + (int64_t)q * pl + off);
+        return v;
+    };
+    auto load_coarse = [&](int m) {  // coarse plane m's window into its ring slot
+        if (m < 0 || m >= cnx) return;
+        float* E = sE[m % 3];
+        for (int t = tid; t < kPpEN; t += 256) {
+            const int a = t / kPpEK, c = t - a * kPpEK;
+            const int J = Jb + a, K = Kb + c;
+            E[t] = (J >= 0 && J < cny && K >= 0 && K < cnz) ? ec[(int64_t)m * cpl + (int64_t)J * cnz + K] : 0.0f;
+        }
+    };
+    {
+        const int m0 = (i0 - 1) >> 1;
+        load_coarse(m0 - 1);
+        load_coarse(m0);
+        load_coarse(m0 + 1);
+    }
+    float4 Zc4 = ldz(i0 - 1), Zn4 = ldz(i0), Zf4 = ldz(i0 + 1);  // z_in at planes q, q+1, q+2
+    uint32_t Kmm = 0, Km = 0, Kc = ldk(i0 - 1), Kn = ldk(i0), Kf = ldk(i0 + 1);
+    float Tmm[4] = {0.f, 0.f, 0.f, 0.f}, Tm[4] = {0.f, 0.f, 0.f, 0.f};
+    double da = 0.0, db = 0.0;
+    int s3 = 0;
+    for (int q = i0 - 1; q <= i1; ++q) {
+        const int par = (q - i0 + 1) & 1;
+        uint8_t* C = sC[s3] + 8;
+        *reinterpret_cast<uint32_t*>(C + row * W + 4 * ch) = Kc;
+        if (q & 1) load_coarse((q >> 1) + 1);
+        const float4 Zg4 = ldz(q + 3);  // planes q .. q+2 are held: three planes of loads in flight
+        const uint32_t Kg = ldk(q + 3);
+        // the sweep's pointwise inputs at plane q - 1
+        float rv[4] = {0.f, 0.f, 0.f, 0.f}, zo[4] = {0.f, 0.f, 0.f, 0.f};
+        const int p = q - 1;
+        if (inner && p >= i0) {
+            ldf_nt<4>(r, (int64_t)p * pl + off, rv);
+            if (z_old) ldf_nt<4>(z_old, (int64_t)p * pl + off, zo);
+        }
+        __syncthreads();
+        // --- prolongation at plane q (tile and ring) ------------------------
+        float Tc[4] = {Zc4.x, Zc4.y, Zc4.z, Zc4.w};
+        if (jk_in && q >= 0 && q < nx && Kc != 0u) {  // Kc == 0: four air cells
+            const int I = q >> 1;
+            int I2 = I + ((q & 1) ? 1 : -1);
+            I2 = I2 < 0 ? 0 : (I2 >= cnx ? cnx - 1 : I2);
+            const int J = j >> 1;
+            int J2 = J + ((j & 1) ? 1 : -1);
+            J2 = J2 < 0 ? 0 : (J2 >= cny ? cny - 1 : J2);
+            const float* EI = sE[I % 3];
+            const float* EI2 = sE[I2 % 3];
+            const int rJ = (J - Jb) * kPpEK, rJ2 = (J2 - Jb) * kPpEK;
+            const int kb = (k >> 1) - 1;
+            float cw[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                int kk = kb + u;
+                kk = kk < 0 ? 0 : (kk >= cnz ? cnz - 1 : kk);
+                const int cK = kk - Kb;
+                float acc = 0.0f;
+                acc += 0.5625f * EI[rJ + cK];
+                acc += 0.1875f * EI[rJ2 + cK];
+                acc += 0.1875f * EI2[rJ + cK];
+                acc += 0.0625f * EI2[rJ2 + cK];
+                cw[u] = acc;
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (byte_of(Kc, e) < kCodeFluid0) continue;
+                const int kq = k + e;
+                const int K = kq >> 1;
+                int K2 = K + ((kq & 1) ? 1 : -1);
+                K2 = K2 < 0 ? 0 : (K2 >= cnz ? cnz - 1 : K2);
+                const int sK = K - kb, sK2 = (K2 - kb) < 0 ? 0 : ((K2 - kb) >= 4 ? 3 : (K2 - kb));
+                Tc[e] += 0.75f * cw[sK] + 0.25f * cw[sK2];
+            }
+        }
+        float* VZ = sZ[par] + 4;
+        *reinterpret_cast<float4*>(VZ + row * W + 4 * ch) = make_float4(Tc[0], Tc[1], Tc[2], Tc[3]);
+        // --- post-sweep at plane p = q - 1 and the dot products ------------
+        if (inner && p >= i0) {
+            const float* PZ = sZ[par ^ 1] + 4;
+            const uint8_t* PC = sC[s3 == 0 ? 2 : s3 - 1] + 8;
+            const float4 d4 = *reinterpret_cast<const float4*>(PZ + (row - 1) * W + 4 * ch);
+            const float4 u4 = *reinterpret_cast<const float4*>(PZ + (row + 1) * W + 4 * ch);
+            const uint32_t kd = *reinterpret_cast<const uint32_t*>(PC + (row - 1) * W + 4 * ch);
+            const uint32_t ku = *reinterpret_cast<const uint32_t*>(PC + (row + 1) * W + 4 * ch);
+            const uint8_t cb = PC[row * W + 4 * ch - 1], cf = PC[row * W + 4 * ch + 4];
+            const float vb = PZ[row * W + 4 * ch - 1], vf = PZ[row * W + 4 * ch + 4];
+            const float vd[4] = {d4.x, d4.y, d4.z, d4.w}, vu[4] = {u4.x, u4.y, u4.z, u4.w};
+            float out[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const uint8_t cc = byte_of(Km, e);
+                out[e] = 0.0f;
+                if (cc >= kCodeFluid0) {
+                    const float ps = Tm[e];
+                    const uint8_t c_b = e > 0 ? byte_of(Km, e - 1) : cb;
+                    const float v_b = e > 0 ? Tm[e - 1] : vb;
+                    const uint8_t c_f = e < 3 ? byte_of(Km, e + 1) : cf;
+                    const float v_f = e < 3 ? Tm[e + 1] : vf;
+                    float sum = nbr_term(byte_of(Kmm, e), Tmm[e], ps) + nbr_term(byte_of(Kc, e), Tc[e], ps);
+                    sum = sum + nbr_term(byte_of(kd, e), vd[e], ps);
+                    sum = sum + nbr_term(byte_of(ku, e), vu[e], ps);
+                    sum = sum + nbr_term(c_b, v_b, ps);
+                    sum = sum + nbr_term(c_f, v_f, ps);
+                    out[e] = ps + winv(cc, w) * (rv[e] - (6.0f * ps - sum));
+                }
+                da += (double)rv[e] * (double)out[e];
+                if (z_old) db += (double)rv[e] * (double)zo[e];
+            }
+            *reinterpret_cast<float4*>(z_out + (int64_t)p * pl + off) = make_float4(out[0], out[1], out[2], out[3]);
+        }
+        Zc4 = Zn4; Zn4 = Zf4; Zf4 = Zg4;
+        Kmm = Km; Km = Kc; Kc = Kn; Kn = Kf; Kf = Kg;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) { Tmm[e] = Tm[e]; Tm[e] = Tc[e]; }
+        s3 = s3 == 2 ? 0 : s3 + 1;
+    }
+    double v[2] = {da, db};
+    write_partials<2>(v, part);
+}
+
 // z_out = z_in + P ec from level LC to its refinement LF (tiles TF, vector
 // width vf); z_out may be z_in.
 static int prolong_level(hipStream_t st, const Tiles& TF, int vf, const MLvl& LF, const MLvl& LC,
@@ -923,10 +1118,13 @@ int mg_setup(hipStream_t st, void* ws, int nx, int ny, int nz, const int* mat, i
     return hipSuccess;
 }
 
-// Partial count of the last sweep: one per fine tile (the V-cycle's fine
-// buffers all come from 256-byte aligned workspace, so the vector width
-// depends on nz only).
-unsigned mg_dots_groups(int nx, int ny, int nz) { return make_tiles(nx, ny, nz, vec_for(nz, {})).ntiles; }
+// Partial count of the last pass: one per workgroup of the single-pass
+// prolongation and post-sweep when nz % 4 == 0, else one per fine tile (the
+// V-cycle's fine buffers all come from 256-byte aligned workspace, so the
+// vector width depends on nz only).
+unsigned mg_dots_groups(int nx, int ny, int nz) {
+    return (nz & 3) == 0 ? pp_shape(nx, ny, nz).grid : make_tiles(nx, ny, nz, vec_for(nz, {})).ntiles;
+}
 
 #define MG_VEC(KERNEL, VEC, GRID, ST, ...)                                          \
     do {                                                                           \
@@ -1014,7 +1212,11 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
     const unsigned g2 = grid_for(W.L2.n);
     // fine: two pre-smoothing sweeps from zero, restriction of the residual
     const Pre2Coef K0 = pre2_chebyshev(kPreLo), K1 = K0;
-    int rc = pre_restrict(st, T0, v0, W.L0, W.L1, C1, code0, r, z, W.r1, done, 1, 7680, K0);
+    // With the single-pass prolongation and post-sweep, the pre-smoothed
+    // values go to the fine temporary (that pass reads them and writes z).
+    const bool fused_post = v0 == 4;
+    float* zpre = fused_post ? W.zt0 : z;
+    int rc = pre_restrict(st, T0, v0, W.L0, W.L1, C1, code0, r, zpre, W.r1, done, 1, 7680, K0);
     if (rc != hipSuccess) return rc;
     // level 1 (the single pass walks its i segments plane by plane: with the
     // few workgroups this level fills, that chain is slower than two kernels)
@@ -1061,6 +1263,13 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
     if (rc != hipSuccess) return rc;
     chebyshev(st, T1, v1, W.code1, W.r1, W.z1, W.zt1, kL1ChebSteps, kL1ChebLo, done);
     // fine: prolongation, then one sweep fused with r.z, r.z_old
+    if (fused_post) {
+        const PpShape P = pp_shape(nx, ny, nz);
+        if (P.grid != mg_dots_groups(nx, ny, nz)) return (int)hipErrorInvalidValue;
+        prolong_post_kernel<<<P.grid, 256, 0, st>>>(nx, ny, nz, P.ni, P.jtiles, P.ktiles, code0, W.zt0, W.z1, r, z_old,
+                                                    z, dots_part, kPostW, done);
+        return (int)hipGetLastError();
+    }
     rc = prolong_level(st, T0, v0, W.L0, W.L1, C1, code0, W.z1, z, W.zt0, done);
     if (rc != hipSuccess) return rc;
     if (T0.ntiles != mg_dots_groups(nx, ny, nz)) return (int)hipErrorInvalidValue;  // misaligned caller buffers
@@ -1077,6 +1286,43 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
 // children of coarse cell (I,J,K) (coarse dims (n+1)/2).  fused = 1 runs the
 // single-pass kernel and is an error unless nx, ny are even, nz % 4 == 0 and
 // the vectors are 16-byte aligned; fused = 0 runs the two-kernel path.
+// The V-cycle's last pass on the fine level (diagnostic entry, no counterpart
+// in the reference): zt = z_in + P ec on fluid cells (cell-centred trilinear
+// prolongation from the coarse grid ((n+1)/2 per axis); z_in elsewhere), then
+// z_out = zt + w D^-1 (r - A zt) on fluid cells (0 elsewhere), and per
+// workgroup the partials of r.z_out (part[0..G)) and r.z_old (part[G..2G),
+// 0 if z_old is null); *groups = G.  fused = 1 runs the single-pass kernel
+// (nz % 4 == 0 and 16-byte aligned vectors, otherwise an error), fused = 0
+// prolong + smooth_dots through tmp (nx*ny*nz floats).  z_out must not alias
+// z_in.
+extern "C" int bs_mg_prolong_post(const unsigned char* code, const float* z_in, const float* ec, const float* r,
+                                  const float* z_old, float* z_out, float* tmp, double* part, size_t part_len,
+                                  int* groups, int nx, int ny, int nz, int fused, double w, bs_stream_t stream) {
+    using namespace bs;
+    if (nx <= 0 || ny <= 0 || nz <= 0 || !groups) return (int)hipErrorInvalidValue;
+    hipStream_t st = (hipStream_t)stream;
+    const MLvl LF = mk(nx, ny, nz);
+    const MLvl LC = mk((nx + 1) / 2, (ny + 1) / 2, (nz + 1) / 2);
+    const int vf = vec_for(nz, {z_in, r, z_old, z_out, tmp, code});
+    if (fused) {
+        if (vf != 4) return (int)hipErrorInvalidValue;
+        const PpShape P = pp_shape(nx, ny, nz);
+        if (part_len < 2 * (size_t)P.grid) return (int)hipErrorInvalidValue;
+        *groups = (int)P.grid;
+        prolong_post_kernel<<<P.grid, 256, 0, st>>>(nx, ny, nz, P.ni, P.jtiles, P.ktiles, code, z_in, ec, r, z_old,
+                                                    z_out, part, (float)w, nullptr);
+        return (int)hipGetLastError();
+    }
+    const Tiles TF = make_tiles(nx, ny, nz, vf);
+    const Tiles TC = make_tiles(LC.nx, LC.ny, LC.nz, 1);
+    if (part_len < 2 * (size_t)TF.ntiles) return (int)hipErrorInvalidValue;
+    *groups = (int)TF.ntiles;
+    const int rc = prolong_level(st, TF, vf, LF, LC, TC, code, ec, z_in, tmp, nullptr);
+    if (rc != hipSuccess) return rc;
+    MG_VEC(smooth_dots_kernel, vf, TF.ntiles, st, TF, code, r, tmp, z_out, z_old, part, nullptr, (float)w);
+    return (int)hipGetLastError();
+}
+
 extern "C" int bs_mg_smooth_restrict(const unsigned char* code, const float* r, float* z, float* rc, int nx, int ny,
                                      int nz, int fused, double lo, bs_stream_t stream) {
     using namespace bs;

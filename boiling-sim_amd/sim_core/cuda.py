@@ -303,6 +303,18 @@ class SimCore:
                   _ptr("tmp_ptr", tmp_ptr), _usize("nx", nx), _usize("ny", ny), _usize("nz", nz), int(steps),
                   float(lo), _ptr("stream", stream))
 
+    def mg_prolong_post(self, code_ptr, z_in_ptr, ec_ptr, r_ptr, z_old_ptr, z_out_ptr, tmp_ptr, part_ptr, part_len,
+                        nx, ny, nz, fused=True, w=1.0, stream=0) -> int:
+        """The V-cycle's last fine-level pass (see bs_mg_prolong_post); returns
+        the number G of workgroup partials written per dot product."""
+        g = ctypes.c_int(0)
+        _lib.call("bs_mg_prolong_post", _ptr("code_ptr", code_ptr), _ptr("z_in_ptr", z_in_ptr),
+                  _ptr("ec_ptr", ec_ptr), _ptr("r_ptr", r_ptr), z_old_ptr or None, _ptr("z_out_ptr", z_out_ptr),
+                  _ptr("tmp_ptr", tmp_ptr), _ptr("part_ptr", part_ptr), int(part_len), ctypes.byref(g),
+                  _usize("nx", nx), _usize("ny", ny), _usize("nz", nz), 1 if fused else 0, float(w),
+                  _ptr("stream", stream))
+        return int(g.value)
+
     # ---- mixed-precision iterative refinement -------------------------------
     STORAGE = {"fp32": 0, "fp16": 1, "bf16": 2}
     # Residual reduction between replacements by storage type: 16-bit vectors

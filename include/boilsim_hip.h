@@ -272,6 +272,19 @@ int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx, int ny, i
  * Asynchronous on `stream`. */
 int bs_mg_smooth_restrict(const unsigned char* code, const float* r, float* z, float* rc, int nx, int ny, int nz,
                           int fused, double lo, bs_stream_t stream);
+/* The V-cycle's last pass on the fine level (diagnostic; no counterpart in the
+ * reference): zt = z_in + P ec on fluid cells (P the cell-centred trilinear
+ * prolongation, weights 3/4 and 1/4 per axis, from the coarse grid of
+ * ((nx+1)/2, (ny+1)/2, (nz+1)/2) cells; z_in elsewhere), then z_out = zt +
+ * w D^-1 (r - A zt) on fluid cells (0 elsewhere); part[0..G) receives per
+ * workgroup partials of r.z_out and part[G..2G) of r.z_old (z_old may be
+ * null), *groups = G.  fused = 1 runs the single-pass kernel the V-cycle uses
+ * (nz % 4 == 0, 16-byte aligned vectors; otherwise an error), fused = 0 the
+ * prolongation and the sweep as two kernels through tmp (nx*ny*nz floats).
+ * z_out must not alias z_in.  Asynchronous on `stream`. */
+int bs_mg_prolong_post(const unsigned char* code, const float* z_in, const float* ec, const float* r,
+                       const float* z_old, float* z_out, float* tmp, double* part, size_t part_len, int* groups,
+                       int nx, int ny, int nz, int fused, double w, bs_stream_t stream);
 /* `steps` (even, > 0) Chebyshev steps over [lo, 2] of D^-1 A on A x = r from
  * x, the result in x (tmp: device scratch of nx*ny*nz floats): the V-cycle's
  * post-smoother (4 steps, lo = 0.02) and coarsest-level polish (20 steps,

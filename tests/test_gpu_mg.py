@@ -1,7 +1,8 @@
-"""GPU tests of the V-cycle's first pass on one level (two damped-Jacobi
-sweeps from zero, restriction of the residual): the single-pass kernel
-against the two-kernel path and against a numpy restatement.  The V-cycle
-belongs to the learned preconditioner, which has no counterpart in the
+"""GPU tests of the V-cycle's single-pass kernels against their two-kernel
+paths and against numpy restatements: the first pass on one level (two
+smoothing steps from zero, restriction of the residual), the Chebyshev steps,
+and the fine level's last pass (prolongation, post-sweep, dot products).  The
+V-cycle belongs to the learned preconditioner, which has no counterpart in the
 reference; its operator is the pressure stencil of python/boilingsim/
 fluid.py:222-240 on class codes."""
 
@@ -43,8 +44,11 @@ def _apply(code, v):
 
 
 def _dinv(code):
-    fl = code >= 2
-    return np.where(fl, 1.0 / np.where(fl, 8.0 - code, 1.0), 0.0)
+    """D^-1 on fluid cells; a fluid cell walled in by solid on all six sides
+    has a zero row (its neighbours all mirror it), for which the kernels take
+    D = 1."""
+    dg = np.where(code >= 2, 8.0 - code, 0.0)
+    return np.where(dg > 0, 1.0 / np.where(dg > 0, dg, 1.0), np.where(code >= 2, 1.0, 0.0))
 
 
 def _reference(code, r, lo=0.0):
@@ -214,3 +218,99 @@ def test_chebyshev_rejects_bad_arguments(simcore, torch_gpu):
     for steps, lo in ((3, 0.1), (-2, 0.1), (4, 0.0), (4, 2.0)):
         with pytest.raises(RuntimeError):
             simcore.mg_chebyshev(code.data_ptr(), z.data_ptr(), z.data_ptr(), z.data_ptr(), *shape, steps, lo)
+
+
+def _prolong_reference(code, ec):
+    """Cell-centred trilinear prolongation (weights 3/4 to the parent coarse
+    cell, 1/4 to the neighbour on the child's side, clamped at the domain),
+    FP64, on fluid cells only."""
+    shape, cshape = code.shape, ec.shape
+
+    def axis(n, N):
+        i = np.arange(n)
+        return i // 2, np.clip(i // 2 + np.where(i & 1, 1, -1), 0, N - 1)
+
+    (I, I2), (J, J2), (K, K2) = (axis(n, N) for n, N in zip(shape, cshape))
+    e = ec.astype(np.float64)
+    out = np.zeros(shape)
+    for ii, wa in ((I, .75), (I2, .25)):
+        for jj, wb in ((J, .75), (J2, .25)):
+            for kk, wc in ((K, .75), (K2, .25)):
+                out += wa * wb * wc * e[ii][:, jj][:, :, kk]
+    return np.where(code >= 2, out, 0.0)
+
+
+# nz % 4 == 0 throughout (the single-pass kernel's condition): partial j and
+# k tiles, odd nx and ny, one-row grids, several i segments with a short last.
+PP_CASES = [
+    ("pot", (64, 64, 48)),
+    ("pot", (90, 30, 60)),
+    ("random", (40, 26, 112)),
+    ("random", (27, 15, 60)),
+    ("random", (33, 29, 116)),
+    ("random", (8, 12, 4)),
+    ("random", (98, 24, 16)),
+    ("random", (5, 1, 8)),
+]
+
+
+@pytest.mark.parametrize("with_old", [True, False])
+@pytest.mark.parametrize("kind,shape", PP_CASES)
+def test_prolong_post_matches_two_kernels(simcore, torch_gpu, kind, shape, with_old):
+    """The fine level's last pass (prolongation, undamped post-sweep, r.z and
+    r.z_old) in one kernel against the prolongation and sweep kernels."""
+    torch = torch_gpu
+    rng = np.random.default_rng(7 + sum(shape) + with_old)
+    code = _codes(_mask(kind, shape, rng))
+    fl = code >= 2
+    cshape = tuple((n + 1) // 2 for n in shape)
+    r = np.where(fl, rng.standard_normal(shape), 0.0).astype(np.float32)
+    z_in = np.where(fl, rng.standard_normal(shape), 0.0).astype(np.float32)
+    z_old = rng.standard_normal(shape).astype(np.float32)
+    ec = rng.standard_normal(cshape).astype(np.float32)
+    held = [dev(torch, a) for a in (code, z_in, ec, r, z_old)]  # held while the kernels run
+    cd, zid, ecd, rd, zod = held
+    n = int(np.prod(shape))
+    out = {}
+    for fused in (1, 0):
+        z = torch.full(shape, np.nan, dtype=torch.float32, device="cuda")
+        tmp = torch.full(shape, np.nan, dtype=torch.float32, device="cuda")
+        part = torch.full((2 * n + 64,), np.nan, dtype=torch.float64, device="cuda")
+        g = simcore.mg_prolong_post(cd.data_ptr(), zid.data_ptr(), ecd.data_ptr(), rd.data_ptr(),
+                                    zod.data_ptr() if with_old else 0, z.data_ptr(), tmp.data_ptr(), part.data_ptr(),
+                                    part.numel(), *shape, fused, 1.0)
+        torch.cuda.synchronize()
+        p = host(part)
+        out[fused] = (host(z), p[:g].sum(), p[g:2 * g].sum())
+    (zf, af, bf), (zu, au, bu) = out[1], out[0]
+    print(f"[{kind} {shape} z_old {with_old}] fluid {int(fl.sum())}, r.z {af:.6e} / {au:.6e}, "
+          f"r.z_old {bf:.6e} / {bu:.6e}")
+    assert np.isfinite(zf).all()
+    assert np.array_equal(zf.view(np.uint32), zu.view(np.uint32))
+    # FP64 restatement: zt = z_in + P ec, one undamped Jacobi sweep from zt
+    zt = z_in.astype(np.float64) + _prolong_reference(code, ec)
+    z64 = np.where(fl, zt + _dinv(code) * (r - _apply(code, zt)), 0.0)
+    np.testing.assert_allclose(zf, z64, rtol=0, atol=2e-6 * np.abs(z64).max())
+    # the partials sum FP64 products of the same FP32 values in another order
+    a64 = float((r.astype(np.float64) * zf.astype(np.float64)).sum())
+    b64 = float((r.astype(np.float64) * z_old.astype(np.float64)).sum()) if with_old else 0.0
+    scale_a = float(np.abs(r.astype(np.float64) * zf).sum())
+    scale_b = float(np.abs(r.astype(np.float64) * z_old).sum())
+    for got in (af, au):
+        assert abs(got - a64) <= 1e-11 * scale_a
+    for got in (bf, bu):
+        assert abs(got - b64) <= 1e-11 * scale_b
+
+
+def test_prolong_post_rejects_unsupported(simcore, torch_gpu):
+    torch = torch_gpu
+    part = torch.zeros(4096, dtype=torch.float64, device="cuda")
+    for shape, part_len in (((8, 8, 6), part.numel()), ((8, 8, 8), 1)):  # nz % 4 != 0; too few partial slots
+        code = dev(torch, _codes(np.zeros(shape, np.int32)))
+        z = torch.zeros(shape, dtype=torch.float32, device="cuda")
+        z2 = torch.zeros(shape, dtype=torch.float32, device="cuda")
+        ec = torch.zeros(tuple((n + 1) // 2 for n in shape), dtype=torch.float32, device="cuda")
+        with pytest.raises(RuntimeError):
+            simcore.mg_prolong_post(code.data_ptr(), z.data_ptr(), ec.data_ptr(), z.data_ptr(), 0, z2.data_ptr(),
+                                    z2.data_ptr(), part.data_ptr(), part_len, *shape, 1, 1.0)
+        torch.cuda.synchronize()
