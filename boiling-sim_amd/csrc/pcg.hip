@@ -58,6 +58,8 @@ struct PcgCtrl {
     int pad;
     double best_rr;
     double dots[2];  // flexible CG: r.z_new, r.z_old (reduced)
+    int fresh;       // flexible CG: the next direction is p = z (first iteration of a solve)
+    int pad2;
 };
 static_assert(sizeof(PcgCtrl) <= 256, "control block must fit 256 bytes");
 
@@ -421,15 +423,36 @@ __global__ __launch_bounds__(1024) void pap_reduce_kernel(const double* __restri
     ctrl->alpha = v[0] != 0.0 ? rz / v[0] : 0.0;
 }
 
-// r.z_new, r.z_old totals for the flexible-CG direction kernel.
+// r.z_new, r.z_old totals and the flexible-CG direction's scalars, after the
+// check of iteration it advanced ctrl->iter to it + 1: beta =
+// (r.z_new - r.z_old) / r.z (Polak-Ribiere), or, with init, no beta and the
+// next direction p = z.  A preconditioner that is not positive on this
+// residual (r.z_new <= 0) ends the flexible CG for the plain-PCG fallback;
+// so does reaching max_iter.
 __global__ __launch_bounds__(1024) void dots_reduce_kernel(const double* __restrict__ partZ, unsigned G,
-                                                           PcgCtrl* ctrl) {
+                                                           PcgCtrl* ctrl, int init) {
     if (ctrl->done) return;
     double v[2];
     reduce1024<2>(partZ, G, v);
     if (threadIdx.x != 0) return;
     ctrl->dots[0] = v[0];
     ctrl->dots[1] = v[1];
+    if (!(v[0] > 0.0)) {
+        ctrl->fallback = 1;
+        ctrl->done = 1;
+        return;
+    }
+    const int it1 = ctrl->iter;  // iterations completed
+    double beta = 0.0;
+    if (!init) {
+        const double rz_old = ctrl->rz[(it1 - 1) & 1];
+        beta = rz_old != 0.0 ? (v[0] - v[1]) / rz_old : 0.0;
+    }
+    ctrl->rz[it1 & 1] = v[0];
+    ctrl->rz_new = v[0];
+    ctrl->beta = beta;
+    ctrl->fresh = init;
+    if (!init && it1 >= ctrl->max_iter) ctrl->done = 1;
 }
 
 // alpha = r.z / p.Ap (0 if p.Ap == 0, axpy_device.cu:82-91); x += alpha p,
@@ -586,11 +609,22 @@ static int ensure_poll() {
         else KERNEL<1><<<(GRID), 256, 0, (ST)>>>(__VA_ARGS__);                     \
     } while (0)
 
+static uint32_t max_tiles(int nx, int ny, int nz);
+
+// Whether the i-marching SpMV applies: nz % 4 == 0 with aligned vectors, and
+// its partials fit the per-tile slots (any_size) or the grid has at least as
+// many workgroups as vector tiles (the plain PCG's choice).
+static bool roll_fits(const Tiles& T, int vec, const PcgBuffers& B, bool any_size) {
+    const unsigned g = roll_shape(T.nx, T.ny, T.nz).grid;
+    return B.partT && vec == 4 && g <= (any_size ? max_tiles(T.nx, T.ny, T.nz) : (unsigned)T.ntiles);
+}
+
 // Ap = A p and alpha = r.z / p.Ap into the control block (it < 0: the
 // iteration number is the device counter).
-static void enqueue_spmv(const Tiles& T, int vec, const PcgBuffers& B, int it, hipStream_t st) {
+static void enqueue_spmv(const Tiles& T, int vec, const PcgBuffers& B, int it, hipStream_t st,
+                         bool any_size = false) {
     const RollShape rs = roll_shape(T.nx, T.ny, T.nz);
-    if (B.partT && vec == 4 && rs.grid <= (unsigned)T.ntiles) {
+    if (roll_fits(T, vec, B, any_size)) {
         pcg_spmv_roll_kernel<<<rs.grid, 256, 0, st>>>(T.nx, T.ny, T.nz, rs.ni, rs.jtiles, rs.ktiles, B.code, B.p,
                                                       B.ap, B.partT, B.ctrl);
         pap_reduce_kernel<<<1, 1024, 0, st>>>(B.partT, rs.grid, B.ctrl, it);
@@ -951,37 +985,12 @@ __global__ __launch_bounds__(256) void fcg_dots_kernel(int64_t n, const float* _
     write_partials<2>(v, partZ);
 }
 
-// beta = (r.z_new - r.z_old) / r.z; p = z_new + beta p, after the check of
-// iteration it advanced ctrl->iter to it + 1.  init: p = z and r.z only.
+// p = z_new + beta p with beta from dots_reduce_kernel; init: p = z.
 template <int VEC>
 __global__ __launch_bounds__(256) void fcg_direction_kernel(Tiles T, const float* __restrict__ z,
-                                                            float* __restrict__ p, const double* __restrict__ partZ,
-                                                            unsigned G, PcgCtrl* ctrl, int init) {
+                                                            float* __restrict__ p, const PcgCtrl* ctrl, int init) {
     if (ctrl->done) return;
-    const int it1 = ctrl->iter;  // iterations completed
-    (void)partZ;
-    (void)G;
-    const double v[2] = {ctrl->dots[0], ctrl->dots[1]};
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    if (!(v[0] > 0.0)) {  // preconditioner not positive on this residual
-        if (lead) {
-            ctrl->fallback = 1;
-            ctrl->done = 1;
-        }
-        return;
-    }
-    double beta = 0.0;
-    if (!init) {
-        const double rz_old = ctrl->rz[(it1 - 1) & 1];
-        beta = rz_old != 0.0 ? (v[0] - v[1]) / rz_old : 0.0;
-    }
-    if (lead) {
-        ctrl->rz[it1 & 1] = v[0];
-        ctrl->rz_new = v[0];
-        ctrl->beta = beta;
-        if (!init && it1 >= ctrl->max_iter) ctrl->done = 1;
-    }
-    const float bf = (float)beta;
+    const float bf = (float)ctrl->beta;
     uint32_t t0, t1;
     tile_span(T.ntiles, t0, t1);
     const int64_t plane = (int64_t)T.ny * T.nz;
@@ -999,6 +1008,183 @@ __global__ __launch_bounds__(256) void fcg_direction_kernel(Tiles T, const float
             for (int e = 0; e < VEC; ++e) pv[e] = zv[e] + bf * pv[e];
             stf<VEC>(p, c0, pv);
         }
+    }
+}
+
+// The flexible-CG iteration's first pass (nz % 4 == 0, 16-byte aligned
+// vectors), pcg_spmv_roll_kernel's tiling and stencil: the direction
+// p_new = z + beta p_old (p_new = z when ctrl->fresh), Ap = A p_new with the
+// partials of p_new.Ap, and the previous iteration's deferred x += alpha
+// p_old (not when fresh).  Each float operation is the one of the direction,
+// SpMV and update kernels, so the iterates are bit-identical to theirs.
+// p_new goes to another buffer than p_old: the ring cells of a tile belong to
+// its neighbours, whose old values are still read there.
+__global__ __launch_bounds__(256) void fcg_spmv_dir_kernel(int nx, int ny, int nz, int ni_seg, int jtiles,
+                                                           int ktiles, const uint8_t* __restrict__ code,
+                                                           const float* __restrict__ z,
+                                                           const float* __restrict__ p_old,
+                                                           float* __restrict__ p_new, float* __restrict__ x,
+                                                           float* __restrict__ ap, double* __restrict__ part,
+                                                           const PcgCtrl* ctrl) {
+    if (ctrl->done) return;
+    const bool fresh = ctrl->fresh != 0;
+    const float bf = (float)ctrl->beta, af = (float)ctrl->alpha;
+    constexpr int CH = kRollKT + 2, SR = kRollTJ + 2, W = 4 * CH;
+    static_assert(CH * SR == 256, "one thread per chunk of the tile and its ring");
+    __shared__ __align__(16) float sV[2][SR * W + 8];
+    __shared__ __align__(16) uint8_t sC[2][SR * W + 16];
+    const int tid = threadIdx.x;
+    const int r = tid / CH, c = tid - (tid / CH) * CH;
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const int cols = jtiles * ktiles;
+    const int seg = (int)(b / cols), col = (int)(b % cols);
+    const int jt = col / ktiles, kt = col - (col / ktiles) * ktiles;
+    const int j0 = jt * kRollTJ, k0 = kt * 4 * kRollKT;
+    const int i0 = seg * ni_seg, i1 = min(nx, i0 + ni_seg);
+    const int j = j0 - 1 + r, k = k0 - 4 + 4 * c;
+    const bool jk_in = j >= 0 && j < ny && k >= 0 && k < nz;
+    const bool interior = r >= 1 && r <= kRollTJ && c >= 1 && c <= kRollKT && j < ny && k < nz;
+    const int64_t pl = (int64_t)ny * nz;
+    const int64_t off = (int64_t)j * nz + k;
+    // planes outside the grid and chunks outside (j,k) read as air (value 0)
+    auto ldv = [&](const float* f, int q) {
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (jk_in && q >= 0 && q < nx) v = *reinterpret_cast<const float4*>(f + (int64_t)q * pl + off);
+        return v;
+    };
+    auto ldk = [&](int q) {
+        uint32_t v = 0;
+        if (jk_in && q >= 0 && q < nx) v = *reinterpret_cast<const uint32_t*>(code + (int64_t)q * pl + off);
+        return v;
+    };
+    auto ldx = [&](int q) {
+        float v[4] = {0.f, 0.f, 0.f, 0.f};
+        if (interior && !fresh && q < i1) ldf_nt<4>(x, (int64_t)q * pl + off, v);
+        return make_float4(v[0], v[1], v[2], v[3]);
+    };
+    auto dir = [&](float4 zv, float4 pv) {
+        if (fresh) return zv;
+        return make_float4(zv.x + bf * pv.x, zv.y + bf * pv.y, zv.z + bf * pv.z, zv.w + bf * pv.w);
+    };
+    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    // raw z and p_old two planes ahead; directions at q - 1, q (q + 1 is
+    // formed at the top of iteration q, a plane after its loads were issued)
+    float4 Pc = fresh ? zero4 : ldv(p_old, i0);
+    float4 Vm = dir(ldv(z, i0 - 1), fresh ? zero4 : ldv(p_old, i0 - 1)), Vc = dir(ldv(z, i0), Pc);
+    float4 Zn = ldv(z, i0 + 1), Pn = fresh ? zero4 : ldv(p_old, i0 + 1);
+    float4 Xc = ldx(i0);
+    uint32_t Km = ldk(i0 - 1), Kc = ldk(i0), Kn = ldk(i0 + 1);
+    double pap = 0.0;
+    for (int q = i0; q < i1; ++q) {
+        float* V = sV[q & 1] + 4;
+        uint8_t* C = sC[q & 1] + 8;
+        *reinterpret_cast<float4*>(V + r * W + 4 * c) = Vc;
+        *reinterpret_cast<uint32_t*>(C + r * W + 4 * c) = Kc;
+        const float4 Zf = ldv(z, q + 2);
+        const float4 Pf = fresh ? zero4 : ldv(p_old, q + 2);
+        const uint32_t Kf = ldk(q + 2);
+        const float4 Xn = ldx(q + 1);
+        const float4 Vn = dir(Zn, Pn);
+        __syncthreads();
+        if (interior) {
+            const float4 d4 = *reinterpret_cast<const float4*>(V + (r - 1) * W + 4 * c);
+            const float4 u4 = *reinterpret_cast<const float4*>(V + (r + 1) * W + 4 * c);
+            const uint32_t kd = *reinterpret_cast<const uint32_t*>(C + (r - 1) * W + 4 * c);
+            const uint32_t ku = *reinterpret_cast<const uint32_t*>(C + (r + 1) * W + 4 * c);
+            const float vb = V[r * W + 4 * c - 1], vf = V[r * W + 4 * c + 4];
+            const uint8_t cb = C[r * W + 4 * c - 1], cf = C[r * W + 4 * c + 4];
+            const float vc[4] = {Vc.x, Vc.y, Vc.z, Vc.w}, vl[4] = {Vm.x, Vm.y, Vm.z, Vm.w};
+            const float vr[4] = {Vn.x, Vn.y, Vn.z, Vn.w}, vd[4] = {d4.x, d4.y, d4.z, d4.w};
+            const float vu[4] = {u4.x, u4.y, u4.z, u4.w};
+            float out[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const uint8_t cc = (Kc >> (8 * e)) & 0xFF;
+                if (cc < kCodeFluid0) { out[e] = 0.0f; continue; }
+                const float ps = vc[e];
+                const uint8_t c_b = e > 0 ? (uint8_t)((Kc >> (8 * (e - 1))) & 0xFF) : cb;
+                const float v_b = e > 0 ? vc[e - 1] : vb;
+                const uint8_t c_f = e < 3 ? (uint8_t)((Kc >> (8 * (e + 1))) & 0xFF) : cf;
+                const float v_f = e < 3 ? vc[e + 1] : vf;
+                float s = nbr_term((uint8_t)((Km >> (8 * e)) & 0xFF), vl[e], ps) +
+                          nbr_term((uint8_t)((Kn >> (8 * e)) & 0xFF), vr[e], ps);
+                s = s + nbr_term((uint8_t)((kd >> (8 * e)) & 0xFF), vd[e], ps);
+                s = s + nbr_term((uint8_t)((ku >> (8 * e)) & 0xFF), vu[e], ps);
+                s = s + nbr_term(c_b, v_b, ps);
+                s = s + nbr_term(c_f, v_f, ps);
+                out[e] = 6.0f * ps - s;
+                pap += (double)ps * (double)out[e];
+            }
+            const int64_t o = (int64_t)q * pl + off;
+            *reinterpret_cast<float4*>(ap + o) = make_float4(out[0], out[1], out[2], out[3]);
+            *reinterpret_cast<float4*>(p_new + o) = Vc;
+            if (!fresh) {
+                const float xv[4] = {Xc.x + af * Pc.x, Xc.y + af * Pc.y, Xc.z + af * Pc.z, Xc.w + af * Pc.w};
+                stf_nt<4>(x, o, xv);
+            }
+        }
+        Vm = Vc; Vc = Vn; Pc = Pn;
+        Zn = Zf; Pn = Pf; Xc = Xn;
+        Km = Kc; Kc = Kn; Kn = Kf;
+    }
+    double v[1] = {pap};
+    write_partials<1>(v, part);
+}
+
+// r -= alpha Ap (the update's residual half; x follows in the next
+// iteration's first pass); partials of r.r.
+template <int VEC>
+__global__ __launch_bounds__(256) void fcg_rupdate_kernel(Tiles T, const float* __restrict__ ap,
+                                                          float* __restrict__ r, double* __restrict__ partB,
+                                                          const PcgCtrl* ctrl) {
+    if (ctrl->done) return;
+    const float af = (float)ctrl->alpha;
+    uint32_t t0, t1;
+    tile_span(T.ntiles, t0, t1);
+    double rr = 0.0;
+    const int64_t plane = (int64_t)T.ny * T.nz;
+    for (uint32_t t = t0; t < t1; ++t) {
+        int i, j, k0;
+        if (!tile_cell<VEC>(T, t, i, j, k0)) continue;
+        const int64_t c0 = (int64_t)i * plane + (int64_t)j * T.nz + k0;
+        float av[VEC], rv[VEC];
+        ldf_nt<VEC>(ap, c0, av);
+        ldf<VEC>(r, c0, rv);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            rv[e] = rv[e] - af * av[e];
+            rr += (double)rv[e] * (double)rv[e];
+        }
+        stf<VEC>(r, c0, rv);
+    }
+    double v[1] = {rr};
+    write_partials<1>(v, partB);
+}
+
+// After the flexible CG stops: the last update's deferred x += alpha p (p of
+// iteration ctrl->iter - 1, in p_buf[ctrl->iter & 1]); nothing when no
+// iteration ran.
+template <int VEC>
+__global__ __launch_bounds__(256) void fcg_finish_kernel(Tiles T, const float* __restrict__ p0,
+                                                         const float* __restrict__ p1, float* __restrict__ x,
+                                                         const PcgCtrl* ctrl) {
+    const int it = ctrl->iter;
+    if (it <= 0) return;
+    const float* p = (it & 1) ? p1 : p0;
+    const float af = (float)ctrl->alpha;
+    uint32_t t0, t1;
+    tile_span(T.ntiles, t0, t1);
+    const int64_t plane = (int64_t)T.ny * T.nz;
+    for (uint32_t t = t0; t < t1; ++t) {
+        int i, j, k0;
+        if (!tile_cell<VEC>(T, t, i, j, k0)) continue;
+        const int64_t c0 = (int64_t)i * plane + (int64_t)j * T.nz + k0;
+        float pv[VEC], xv[VEC];
+        ldf<VEC>(p, c0, pv);
+        ldf<VEC>(x, c0, xv);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) xv[e] = xv[e] + af * pv[e];
+        stf<VEC>(x, c0, xv);
     }
 }
 
@@ -1020,6 +1206,8 @@ __global__ void fcg_prepare_kernel(PcgCtrl* ctrl, int it0) {
 struct FcgBuffers {
     PcgBuffers B;
     float* z[2];
+    float* p2;  // second direction buffer of the fused first pass (p = {B.p, p2})
+    bool fused;  // fcg_spmv_dir_kernel + fcg_rupdate_kernel instead of SpMV, update, direction
     double* partZ;
     void* pws;  // preconditioner workspace
     int kind;
@@ -1052,16 +1240,45 @@ static int fcg_precond(hipStream_t st, const FcgBuffers& F, const float* r, floa
 }
 
 // One flexible-CG iteration of parity `par` (z_new = z[1-par], z_old = z[par]).
+// Fused: the direction formed from z[par] and p[par] goes to p[1-par], and x
+// takes the previous iteration's step in the same pass (the last step is
+// applied by fcg_finish_kernel).  Otherwise SpMV, update (x and r) and
+// direction run as separate passes on one p.
 static int fcg_enqueue_iteration(const Tiles& T, int vec, const FcgBuffers& F, int par, hipStream_t st) {
     const PcgBuffers& B = F.B;
-    enqueue_spmv(T, vec, B, -1, st);
-    BS_VEC_LAUNCH(fcg_update_kernel, vec, B.nwg, st, T, B.p, B.ap, B.x, B.r, B.partA, B.partB, B.ctrl);
+    if (F.fused) {
+        const RollShape rs = roll_shape(T.nx, T.ny, T.nz);
+        float* pb[2] = {B.p, F.p2};
+        fcg_spmv_dir_kernel<<<rs.grid, 256, 0, st>>>(T.nx, T.ny, T.nz, rs.ni, rs.jtiles, rs.ktiles, B.code, F.z[par],
+                                                     pb[par], pb[par ^ 1], B.x, B.ap, B.partT, B.ctrl);
+        pap_reduce_kernel<<<1, 1024, 0, st>>>(B.partT, rs.grid, B.ctrl, -1);
+        BS_VEC_LAUNCH(fcg_rupdate_kernel, vec, B.nwg, st, T, B.ap, B.r, B.partB, B.ctrl);
+    } else {
+        enqueue_spmv(T, vec, B, -1, st, true);
+        BS_VEC_LAUNCH(fcg_update_kernel, vec, B.nwg, st, T, B.p, B.ap, B.x, B.r, B.partA, B.partB, B.ctrl);
+    }
     fcg_check_kernel<<<1, 256, 0, st>>>(B.partB, B.nwg, B.ctrl);
     const int rc = fcg_precond(st, F, B.r, F.z[par ^ 1], F.z[par]);
     if (rc != hipSuccess) return rc;
-    dots_reduce_kernel<<<1, 1024, 0, st>>>(F.partZ, F.partZ_G, B.ctrl);
-    BS_VEC_LAUNCH(fcg_direction_kernel, vec, B.nwg, st, T, F.z[par ^ 1], B.p, F.partZ, F.partZ_G, B.ctrl, 0);
+    dots_reduce_kernel<<<1, 1024, 0, st>>>(F.partZ, F.partZ_G, B.ctrl, 0);
+    if (!F.fused) BS_VEC_LAUNCH(fcg_direction_kernel, vec, B.nwg, st, T, F.z[par ^ 1], B.p, B.ctrl, 0);
     return (int)hipGetLastError();
+}
+
+// The first direction after the preconditioner's first application z[0]:
+// the fused first pass forms it itself (ctrl->fresh), otherwise p = z[0].
+static void fcg_start(const Tiles& T, int vec, const FcgBuffers& F, hipStream_t st) {
+    const PcgBuffers& B = F.B;
+    dots_reduce_kernel<<<1, 1024, 0, st>>>(F.partZ, F.partZ_G, B.ctrl, 1);
+    if (!F.fused) BS_VEC_LAUNCH(fcg_direction_kernel, vec, B.nwg, st, T, F.z[0], B.p, B.ctrl, 1);
+}
+
+// The fused first pass applies where the i-marching SpMV does;
+// BOILSIM_FCG_SPLIT=1 selects the separate passes (same iterates).
+static bool fcg_fused(const Tiles& T, int vec, const PcgBuffers& B) {
+    const char* e = getenv("BOILSIM_FCG_SPLIT");
+    if (e && e[0] == '1') return false;
+    return roll_fits(T, vec, B, true);
 }
 
 // Instantiated graphs of an (even, odd) iteration pair, keyed by everything
@@ -1073,7 +1290,7 @@ struct PairGraphKey {
     const float* params;
     const void* packed;
     double n_fluid;
-    int nx, ny, nz, kind, vec;
+    int nx, ny, nz, kind, vec, fused;
     unsigned nwg;
     bool operator==(const PairGraphKey& o) const { return memcmp(this, &o, sizeof(*this)) == 0; }
 };
@@ -1095,6 +1312,7 @@ static int pair_graph(const Tiles& T, int vec, const FcgBuffers& F, hipGraphExec
     key.packed = F.packed;
     key.n_fluid = F.n_fluid;
     key.nx = F.nx; key.ny = F.ny; key.nz = F.nz; key.kind = F.kind; key.vec = vec; key.nwg = F.B.nwg;
+    key.fused = F.fused ? 1 : 0;
     for (auto& e : g_graphs)
         if (e.used && e.key == key) { *out = e.exec; return hipSuccess; }
     if (!g_capture_stream) BS_RETURN_IF_ERR(hipStreamCreateWithFlags(&g_capture_stream, hipStreamNonBlocking));
@@ -1179,6 +1397,10 @@ static int fcg_iterate(const Tiles& T, int vec, const FcgBuffers& F, int max_ite
         rr_prev = rr_last;
         rr_last = h.rr;
     }
+    if (rc == hipSuccess && F.fused) {
+        BS_VEC_LAUNCH(fcg_finish_kernel, vec, B.nwg, st, T, B.p, F.p2, B.x, B.ctrl);
+        rc = (int)hipGetLastError();
+    }
     if (rc == hipSuccess) {
         rc = hipMemcpyAsync(g_poll.final, B.ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, st);
         if (rc == hipSuccess) rc = hipStreamSynchronize(st);
@@ -1205,7 +1427,7 @@ static size_t precond_ws_bytes(int nx, int ny, int nz, int kind) {
 extern "C" size_t bs_fcg_workspace_bytes(int nx, int ny, int nz, int precond) {
     if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
     const size_t n = (size_t)nx * ny * nz;
-    return bs_pcg_workspace_bytes(nx, ny, nz) + 2 * align256(n * 4) + align256(partz_slots(nx, ny, nz) * sizeof(double)) +
+    return bs_pcg_workspace_bytes(nx, ny, nz) + 3 * align256(n * 4) + align256(partz_slots(nx, ny, nz) * sizeof(double)) +
            align256(precond_ws_bytes(nx, ny, nz, precond));
 }
 
@@ -1231,6 +1453,7 @@ extern "C" int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx
     char* c = (char*)workspace + bs_pcg_workspace_bytes(nx, ny, nz);
     F.z[0] = (float*)c; c += align256(n * 4);
     F.z[1] = (float*)c; c += align256(n * 4);
+    F.p2 = (float*)c; c += align256(n * 4);
     F.partZ = (double*)c; c += align256(partz_slots(nx, ny, nz) * sizeof(double));
     F.pws = c;
     F.kind = precond;
@@ -1239,11 +1462,12 @@ extern "C" int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx
     F.nx = nx; F.ny = ny; F.nz = nz;
     F.partZ_G = precond == 1 ? kUNetPartWg : mg_dots_groups(nx, ny, nz);
     B.x = x;
-    const int vec = vec_for(nz, {x, div_u, mat, B.b, B.r, B.p, B.ap, B.code, F.z[0], F.z[1]});
+    const int vec = vec_for(nz, {x, div_u, mat, B.b, B.r, B.p, B.ap, B.code, F.z[0], F.z[1], F.p2});
     const Tiles T = make_tiles(nx, ny, nz, vec);
     B.nwg = pick_nwg(T, kMaxWg);
     B.partB = B.partA + B.nwg;
     B.partS = B.partA + 3 * B.nwg;
+    F.fused = fcg_fused(T, vec, B);
     const float bscale = -rho * dx * dx / dt;
     pcg_ctrl_init_kernel<<<1, 64, 0, st>>>(B.ctrl, tol * tol, max_iter, 1);
     BS_VEC_LAUNCH(pcg_setup_kernel, vec, B.nwg, st, T, mat, div_u, bscale, mat_fluid, mat_air,
@@ -1268,8 +1492,7 @@ extern "C" int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx
     fcg_prepare_kernel<<<1, 64, 0, st>>>(B.ctrl, 0);
     int rc = fcg_precond(st, F, B.r, F.z[0], nullptr);
     if (rc != hipSuccess) return rc;
-    dots_reduce_kernel<<<1, 1024, 0, st>>>(F.partZ, F.partZ_G, B.ctrl);
-    BS_VEC_LAUNCH(fcg_direction_kernel, vec, B.nwg, st, T, F.z[0], B.p, F.partZ, F.partZ_G, B.ctrl, 1);
+    fcg_start(T, vec, F, st);
     BS_RETURN_IF_ERR(hipGetLastError());
     rc = fcg_iterate(T, vec, F, max_iter, st);
     if (rc != hipSuccess) return rc;
@@ -1744,6 +1967,7 @@ extern "C" int bs_fcg_refine_solve(double* x, float* x32, const float* div_u, co
     char* c = (char*)workspace + bs_pcg_workspace_bytes(nx, ny, nz);
     F.z[0] = (float*)c; c += align256(n * 4);
     F.z[1] = (float*)c; c += align256(n * 4);
+    F.p2 = (float*)c; c += align256(n * 4);
     F.partZ = (double*)c; c += align256(partz_slots(nx, ny, nz) * sizeof(double));
     F.pws = c;
     F.kind = precond;
@@ -1756,11 +1980,12 @@ extern "C" int bs_fcg_refine_solve(double* x, float* x32, const float* div_u, co
                           ny, nz, dx, dt, rho, mat_fluid, mat_air, warm_start, st);
     if (rc != hipSuccess) return rc;
     B.x = R.rs;  // the inner solution e
-    const int vec = vec_for(nz, {B.x, B.b, B.r, B.p, B.ap, B.code, F.z[0], F.z[1]});
+    const int vec = vec_for(nz, {B.x, B.b, B.r, B.p, B.ap, B.code, F.z[0], F.z[1], F.p2});
     const Tiles T = make_tiles(nx, ny, nz, vec);
     B.nwg = pick_nwg(T, kMaxWg);
     B.partB = B.partA + B.nwg;
     B.partS = B.partA + 3 * B.nwg;
+    F.fused = fcg_fused(T, vec, B);
     if (precond == 1) {
         RefCtrl h;
         BS_RETURN_IF_ERR(hipMemcpyAsync(&h, R.rc, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -1788,8 +2013,7 @@ extern "C" int bs_fcg_refine_solve(double* x, float* x32, const float* div_u, co
         fcg_prepare_kernel<<<1, 64, 0, st>>>(B.ctrl, 0);
         rc = fcg_precond(st, F, B.r, F.z[0], nullptr);
         if (rc != hipSuccess) return rc;
-        dots_reduce_kernel<<<1, 1024, 0, st>>>(F.partZ, F.partZ_G, B.ctrl);
-        BS_VEC_LAUNCH(fcg_direction_kernel, vec, B.nwg, st, T, F.z[0], B.p, F.partZ, F.partZ_G, B.ctrl, 1);
+        fcg_start(T, vec, F, st);
         BS_RETURN_IF_ERR(hipGetLastError());
         rc = fcg_iterate(T, vec, F, max_inner, st);
         if (rc != hipSuccess) return rc;

@@ -154,9 +154,11 @@ def test_projection_driver(simcore, torch_gpu, monkeypatch, solver):
 
 def test_graph_replay_matches_eager(simcore, torch_gpu, monkeypatch):
     """The flexible CG replays a captured HIP graph of two iterations; running
-    the same solve with eager launches, and the graph path again after eager
-    work and synchronisations in between, gives bit-identical solutions and
-    statistics (the reductions are deterministic)."""
+    the same solve with eager launches, with the iteration's first pass split
+    into SpMV, update and direction kernels (x updated at once instead of an
+    iteration later), and the graph path again after eager work and
+    synchronisations in between, gives bit-identical solutions and statistics
+    (the same float operations; the reductions are deterministic)."""
     torch = torch_gpu
     from boilsim.unet import UNet
 
@@ -164,11 +166,12 @@ def test_graph_replay_matches_eager(simcore, torch_gpu, monkeypatch):
     cfg, mat, div, dt, rho = _pot("small")
     mat_d, div_d = dev(torch, mat), dev(torch, div)
     results = []
-    for mode in ("graph", "eager", "graph", "graph"):
-        if mode == "eager":
-            monkeypatch.setenv("BOILSIM_FCG_EAGER", "1")
-        else:
-            monkeypatch.delenv("BOILSIM_FCG_EAGER", raising=False)
+    for mode in ("graph", "eager", "split", "split-eager", "graph", "graph"):
+        for var, on in (("BOILSIM_FCG_EAGER", "eager" in mode), ("BOILSIM_FCG_SPLIT", "split" in mode)):
+            if on:
+                monkeypatch.setenv(var, "1")
+            else:
+                monkeypatch.delenv(var, raising=False)
         x, st = _fcg(simcore, torch, net, cfg, mat_d, div_d, 2)
         junk = torch.randn(1 << 20, device="cuda").sum().item()  # eager work between replays
         results.append((x, st, junk))
