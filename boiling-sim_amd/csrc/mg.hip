@@ -674,12 +674,21 @@ static bool pr_fits(const MLvl& L, int vec) { return vec == 4 && (L.nx & 1) == 0
 
 __device__ __forceinline__ uint8_t byte_of(uint32_t w, int e) { return (uint8_t)((w >> (8 * e)) & 0xFFu); }
 
+// FOLD: the flexible CG's residual update runs in the same pass: the kernel
+// reads r_old and Ap, works on r = r_old - alpha Ap (alpha = *alpha_p, the
+// update kernel's float operations), writes r to r_out on the tile (r_out
+// must not alias r_old: the ring cells belong to neighbouring tiles, which
+// still read their old values) and one partial of r.r per workgroup.
+template <bool FOLD>
 __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int nz, int ni_seg, int jtiles,
                                                             int ktiles, const uint8_t* __restrict__ code,
                                                             const float* __restrict__ r, float* __restrict__ z,
                                                             float* __restrict__ rc, const int* __restrict__ done,
-                                                            Pre2Coef K) {
+                                                            Pre2Coef K, const float* __restrict__ ap,
+                                                            const double* __restrict__ alpha_p,
+                                                            float* __restrict__ r_out, double* __restrict__ rr_part) {
     if (done && *done) return;
+    const float af = FOLD ? (float)*alpha_p : 0.0f;
     constexpr int CH = kPrKT + 2, SR = kPrTJ + 4, W = 4 * CH;
     static_assert(CH * SR == 256, "one thread per chunk of the tile and its ring");
     __shared__ __align__(16) float sR[2][SR * W + 8];
@@ -710,12 +719,24 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
         if (jk_in && q >= 0 && q < nx) v = *reinterpret_cast<const uint32_t*>(code + (int64_t)q * pl + off);
         return v;
     };
+    auto lda = [&](int q) {
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (FOLD && jk_in && q >= 0 && q < nx) v = *reinterpret_cast<const float4*>(ap + (int64_t)q * pl + off);
+        return v;
+    };
+    auto fold = [&](float4 rv, float4 av) {
+        if (!FOLD) return rv;
+        return make_float4(rv.x - af * av.x, rv.y - af * av.y, rv.z - af * av.z, rv.w - af * av.w);
+    };
     const int cnx = nx >> 1, cny = ny >> 1, cnz = nz >> 1;
     (void)cnx;
-    float4 Rm = ldv(i0 - 2), Rc = ldv(i0 - 1), Rn = ldv(i0), Rf = ldv(i0 + 1);
+    // r at planes q-1, q, q+1 (updated when FOLD); raw r and Ap at q+2, q+3
+    float4 Rm = fold(ldv(i0 - 2), lda(i0 - 2)), Rc = fold(ldv(i0 - 1), lda(i0 - 1)), Rn = fold(ldv(i0), lda(i0));
+    float4 Rf = ldv(i0 + 1), Af = lda(i0 + 1);
     uint32_t Kmm = ldk(i0 - 3), Km = ldk(i0 - 2), Kc = ldk(i0 - 1), Kn = ldk(i0), Kf = ldk(i0 + 1);
     float Zmm[4] = {0.f, 0.f, 0.f, 0.f}, Zm[4] = {0.f, 0.f, 0.f, 0.f};
     float acc[2] = {0.f, 0.f};
+    double rr = 0.0;
     int s3 = 0;  // codes slot of plane q (planes q-1, q-2 in the two before it)
     for (int q = i0 - 1; q <= i1; ++q) {
         const int par = (q - i0 + 1) & 1;
@@ -725,7 +746,15 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
         *reinterpret_cast<float4*>(VR + row * W + 4 * ch) = Rc;
         *reinterpret_cast<uint32_t*>(C + row * W + 4 * ch) = Kc;
         const float4 Rg = ldv(q + 3);  // two planes of loads in flight
+        const float4 Ag = lda(q + 3);
         const uint32_t Kg = ldk(q + 3);
+        if (FOLD && inner && q >= i0 && q < i1) {
+            *reinterpret_cast<float4*>(r_out + (int64_t)q * pl + off) = Rc;
+            rr += (double)Rc.x * (double)Rc.x;
+            rr += (double)Rc.y * (double)Rc.y;
+            rr += (double)Rc.z * (double)Rc.z;
+            rr += (double)Rc.w * (double)Rc.w;
+        }
         __syncthreads();
         // --- smoother at plane q (tile and its one-cell ring) ---------------
         float Zc[4] = {0.f, 0.f, 0.f, 0.f};
@@ -817,11 +846,15 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
                 acc[0] = acc[1] = 0.0f;
             }
         }
-        Rm = Rc; Rc = Rn; Rn = Rf; Rf = Rg;
+        Rm = Rc; Rc = Rn; Rn = fold(Rf, Af); Rf = Rg; Af = Ag;
         Kmm = Km; Km = Kc; Kc = Kn; Kn = Kf; Kf = Kg;
 #pragma unroll
         for (int e = 0; e < 4; ++e) { Zmm[e] = Zm[e]; Zm[e] = Zc[e]; }
         s3 = s3 == 2 ? 0 : s3 + 1;
+    }
+    if (FOLD) {
+        double v[1] = {rr};
+        write_partials<1>(v, rr_part);
     }
 }
 
@@ -1168,11 +1201,19 @@ static void restrict_level(hipStream_t st, const Tiles& TF, int vf, const MLvl& 
 // the restriction kernels.
 static int pre_restrict(hipStream_t st, const Tiles& TF, int vf, const MLvl& LF, const MLvl& LC, const Tiles& TC1,
                         const uint8_t* code, const float* r, float* z, float* rc, const int* done, int fused,
-                        int target_wg, Pre2Coef K) {
+                        int target_wg, Pre2Coef K, const MgFold* fold = nullptr) {
     if (fused && pr_fits(LF, vf)) {
         const PrShape S = pr_shape(LF.nx, LF.ny, LF.nz, target_wg);
-        pre2_restrict_kernel<<<S.grid, 256, 0, st>>>(LF.nx, LF.ny, LF.nz, S.ni, S.jtiles, S.ktiles, code, r, z, rc,
-                                                     done, K);
+        if (fold)
+            pre2_restrict_kernel<true><<<S.grid, 256, 0, st>>>(LF.nx, LF.ny, LF.nz, S.ni, S.jtiles, S.ktiles, code,
+                                                               r, z, rc, done, K, fold->ap, fold->alpha, fold->r_out,
+                                                               fold->rr_part);
+        else
+            pre2_restrict_kernel<false><<<S.grid, 256, 0, st>>>(LF.nx, LF.ny, LF.nz, S.ni, S.jtiles, S.ktiles, code,
+                                                                r, z, rc, done, K, nullptr, nullptr, nullptr,
+                                                                nullptr);
+    } else if (fold) {
+        return (int)hipErrorInvalidValue;
     } else {
         MG_VEC(pre2_kernel, vf, TF.ntiles, st, TF, code, r, z, done, K);
         restrict_level(st, TF, vf, LF, LC, TC1, code, r, z, rc, done);
@@ -1199,11 +1240,17 @@ static void chebyshev(hipStream_t st, const Tiles& T, int vec, const uint8_t* co
     }
 }
 
+unsigned mg_fold_groups(int nx, int ny, int nz) {
+    const MLvl L = mk(nx, ny, nz);
+    return pr_fits(L, 4) ? pr_shape(nx, ny, nz, 7680).grid : 0u;
+}
+
 int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* code0, const float* r, float* z,
              const float* z_old, double* dots_part, int coarse_kind, const float* params, const void* packed,
-             double n_fluid_coarse, const int* done) {
+             double n_fluid_coarse, const int* done, const MgFold* fold) {
     MgWs W = carve_mg(ws, nx, ny, nz);
-    const int v0 = vec_for(W.L0.nz, {r, z, z_old, code0, W.zt0});
+    const int v0 = fold ? vec_for(W.L0.nz, {r, z, z_old, code0, W.zt0, fold->ap, fold->r_out})
+                        : vec_for(W.L0.nz, {r, z, z_old, code0, W.zt0});
     const int v1 = vec_for(W.L1.nz, {W.r1, W.z1, W.zt1, W.code1});
     const Tiles T0 = make_tiles(W.L0.nx, W.L0.ny, W.L0.nz, v0);
     const Tiles T1 = make_tiles(W.L1.nx, W.L1.ny, W.L1.nz, v1);
@@ -1216,8 +1263,12 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
     // values go to the fine temporary (that pass reads them and writes z).
     const bool fused_post = v0 == 4;
     float* zpre = fused_post ? W.zt0 : z;
-    int rc = pre_restrict(st, T0, v0, W.L0, W.L1, C1, code0, r, zpre, W.r1, done, 1, 7680, K0);
+    int rc = pre_restrict(st, T0, v0, W.L0, W.L1, C1, code0, r, zpre, W.r1, done, 1, 7680, K0, fold);
     if (rc != hipSuccess) return rc;
+    if (fold) {
+        r = fold->r_out;  // the rest of the cycle works on the updated residual
+        if (fold->after) fold->after(st, fold->after_arg);
+    }
     // level 1 (the single pass walks its i segments plane by plane: with the
     // few workgroups this level fills, that chain is slower than two kernels)
     rc = pre_restrict(st, T1, v1, W.L1, W.L2, C2, W.code1, W.r1, W.z1, W.r2, done, 0, 0, K1);

@@ -1207,7 +1207,10 @@ struct FcgBuffers {
     PcgBuffers B;
     float* z[2];
     float* p2;  // second direction buffer of the fused first pass (p = {B.p, p2})
+    float* r2;  // second residual buffer of the folded update (r = {B.r, r2})
     bool fused;  // fcg_spmv_dir_kernel + fcg_rupdate_kernel instead of SpMV, update, direction
+    bool fold;   // (fused) the residual update runs in the V-cycle's first pass instead
+    unsigned fold_G;  // r.r partials of that pass
     double* partZ;
     void* pws;  // preconditioner workspace
     int kind;
@@ -1218,8 +1221,18 @@ struct FcgBuffers {
     int nx, ny, nz;
 };
 
-// z = M(r) (+ partials of r.z and r.z_old into partZ).
-static int fcg_precond(hipStream_t st, const FcgBuffers& F, const float* r, float* z, const float* z_old) {
+// The convergence check after the folded update (mg_apply calls it right
+// after its first fine pass).
+static void fcg_fold_check(hipStream_t st, void* arg) {
+    const FcgBuffers& F = *static_cast<const FcgBuffers*>(arg);
+    fcg_check_kernel<<<1, 256, 0, st>>>(F.B.partT, F.fold_G, F.B.ctrl);
+}
+
+// z = M(r) (+ partials of r.z and r.z_old into partZ).  With r_out (V-cycle
+// only), r is the residual before the update r_out = r - alpha Ap, which the
+// cycle's first pass performs (followed by the convergence check).
+static int fcg_precond(hipStream_t st, const FcgBuffers& F, const float* r, float* z, const float* z_old,
+                       float* r_out = nullptr) {
     const PcgBuffers& B = F.B;
     const int* done = &B.ctrl->done;
     const int64_t n = (int64_t)F.nx * F.ny * F.nz;
@@ -1234,9 +1247,18 @@ static int fcg_precond(hipStream_t st, const FcgBuffers& F, const float* r, floa
     }
     const int coarse = F.kind == 2 ? 1 : (F.kind == 4 ? 2 : 0);
     (void)n;
+    MgFold fold;
+    if (r_out) {
+        fold.ap = B.ap;
+        fold.alpha = &B.ctrl->alpha;
+        fold.r_out = r_out;
+        fold.rr_part = B.partT;
+        fold.after = fcg_fold_check;
+        fold.after_arg = const_cast<FcgBuffers*>(&F);
+    }
     // The V-cycle's last sweep also writes the r.z and r.z_old partials.
     return mg_apply(st, F.pws, F.nx, F.ny, F.nz, B.code, r, z, z_old, F.partZ, coarse, F.params, F.packed,
-                    F.n_fluid, done);
+                    F.n_fluid, done, r_out ? &fold : nullptr);
 }
 
 // One flexible-CG iteration of parity `par` (z_new = z[1-par], z_old = z[par]).
@@ -1252,6 +1274,14 @@ static int fcg_enqueue_iteration(const Tiles& T, int vec, const FcgBuffers& F, i
         fcg_spmv_dir_kernel<<<rs.grid, 256, 0, st>>>(T.nx, T.ny, T.nz, rs.ni, rs.jtiles, rs.ktiles, B.code, F.z[par],
                                                      pb[par], pb[par ^ 1], B.x, B.ap, B.partT, B.ctrl);
         pap_reduce_kernel<<<1, 1024, 0, st>>>(B.partT, rs.grid, B.ctrl, -1);
+        if (F.fold) {
+            // r[par] -> r[1-par] in the V-cycle's first pass, then the check
+            float* rb[2] = {B.r, F.r2};
+            const int rc = fcg_precond(st, F, rb[par], F.z[par ^ 1], F.z[par], rb[par ^ 1]);
+            if (rc != hipSuccess) return rc;
+            dots_reduce_kernel<<<1, 1024, 0, st>>>(F.partZ, F.partZ_G, B.ctrl, 0);
+            return (int)hipGetLastError();
+        }
         BS_VEC_LAUNCH(fcg_rupdate_kernel, vec, B.nwg, st, T, B.ap, B.r, B.partB, B.ctrl);
     } else {
         enqueue_spmv(T, vec, B, -1, st, true);
@@ -1281,6 +1311,14 @@ static bool fcg_fused(const Tiles& T, int vec, const PcgBuffers& B) {
     return roll_fits(T, vec, B, true);
 }
 
+// The folded residual update: fused first pass, V-cycle preconditioner whose
+// first fine pass can take it, and room for its partials in partT.
+static bool fcg_fold(const FcgBuffers& F) {
+    if (!F.fused || F.kind < 2) return false;
+    const unsigned g = mg_fold_groups(F.nx, F.ny, F.nz);
+    return g > 0 && g <= max_tiles(F.nx, F.ny, F.nz);
+}
+
 // Instantiated graphs of an (even, odd) iteration pair, keyed by everything
 // the captured launches depend on.  The ~60 launches of one iteration (most
 // of them small coarse-level kernels) become one graph launch.
@@ -1290,7 +1328,7 @@ struct PairGraphKey {
     const float* params;
     const void* packed;
     double n_fluid;
-    int nx, ny, nz, kind, vec, fused;
+    int nx, ny, nz, kind, vec, fused, fold;
     unsigned nwg;
     bool operator==(const PairGraphKey& o) const { return memcmp(this, &o, sizeof(*this)) == 0; }
 };
@@ -1313,6 +1351,7 @@ static int pair_graph(const Tiles& T, int vec, const FcgBuffers& F, hipGraphExec
     key.n_fluid = F.n_fluid;
     key.nx = F.nx; key.ny = F.ny; key.nz = F.nz; key.kind = F.kind; key.vec = vec; key.nwg = F.B.nwg;
     key.fused = F.fused ? 1 : 0;
+    key.fold = F.fold ? 1 : 0;
     for (auto& e : g_graphs)
         if (e.used && e.key == key) { *out = e.exec; return hipSuccess; }
     if (!g_capture_stream) BS_RETURN_IF_ERR(hipStreamCreateWithFlags(&g_capture_stream, hipStreamNonBlocking));
@@ -1427,7 +1466,7 @@ static size_t precond_ws_bytes(int nx, int ny, int nz, int kind) {
 extern "C" size_t bs_fcg_workspace_bytes(int nx, int ny, int nz, int precond) {
     if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
     const size_t n = (size_t)nx * ny * nz;
-    return bs_pcg_workspace_bytes(nx, ny, nz) + 3 * align256(n * 4) + align256(partz_slots(nx, ny, nz) * sizeof(double)) +
+    return bs_pcg_workspace_bytes(nx, ny, nz) + 4 * align256(n * 4) + align256(partz_slots(nx, ny, nz) * sizeof(double)) +
            align256(precond_ws_bytes(nx, ny, nz, precond));
 }
 
@@ -1454,6 +1493,7 @@ extern "C" int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx
     F.z[0] = (float*)c; c += align256(n * 4);
     F.z[1] = (float*)c; c += align256(n * 4);
     F.p2 = (float*)c; c += align256(n * 4);
+    F.r2 = (float*)c; c += align256(n * 4);
     F.partZ = (double*)c; c += align256(partz_slots(nx, ny, nz) * sizeof(double));
     F.pws = c;
     F.kind = precond;
@@ -1462,12 +1502,14 @@ extern "C" int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx
     F.nx = nx; F.ny = ny; F.nz = nz;
     F.partZ_G = precond == 1 ? kUNetPartWg : mg_dots_groups(nx, ny, nz);
     B.x = x;
-    const int vec = vec_for(nz, {x, div_u, mat, B.b, B.r, B.p, B.ap, B.code, F.z[0], F.z[1], F.p2});
+    const int vec = vec_for(nz, {x, div_u, mat, B.b, B.r, B.p, B.ap, B.code, F.z[0], F.z[1], F.p2, F.r2});
     const Tiles T = make_tiles(nx, ny, nz, vec);
     B.nwg = pick_nwg(T, kMaxWg);
     B.partB = B.partA + B.nwg;
     B.partS = B.partA + 3 * B.nwg;
     F.fused = fcg_fused(T, vec, B);
+    F.fold = fcg_fold(F);
+    F.fold_G = F.fold ? mg_fold_groups(nx, ny, nz) : 0u;
     const float bscale = -rho * dx * dx / dt;
     pcg_ctrl_init_kernel<<<1, 64, 0, st>>>(B.ctrl, tol * tol, max_iter, 1);
     BS_VEC_LAUNCH(pcg_setup_kernel, vec, B.nwg, st, T, mat, div_u, bscale, mat_fluid, mat_air,
@@ -1968,6 +2010,7 @@ extern "C" int bs_fcg_refine_solve(double* x, float* x32, const float* div_u, co
     F.z[0] = (float*)c; c += align256(n * 4);
     F.z[1] = (float*)c; c += align256(n * 4);
     F.p2 = (float*)c; c += align256(n * 4);
+    F.r2 = (float*)c; c += align256(n * 4);
     F.partZ = (double*)c; c += align256(partz_slots(nx, ny, nz) * sizeof(double));
     F.pws = c;
     F.kind = precond;
@@ -1980,12 +2023,14 @@ extern "C" int bs_fcg_refine_solve(double* x, float* x32, const float* div_u, co
                           ny, nz, dx, dt, rho, mat_fluid, mat_air, warm_start, st);
     if (rc != hipSuccess) return rc;
     B.x = R.rs;  // the inner solution e
-    const int vec = vec_for(nz, {B.x, B.b, B.r, B.p, B.ap, B.code, F.z[0], F.z[1], F.p2});
+    const int vec = vec_for(nz, {B.x, B.b, B.r, B.p, B.ap, B.code, F.z[0], F.z[1], F.p2, F.r2});
     const Tiles T = make_tiles(nx, ny, nz, vec);
     B.nwg = pick_nwg(T, kMaxWg);
     B.partB = B.partA + B.nwg;
     B.partS = B.partA + 3 * B.nwg;
     F.fused = fcg_fused(T, vec, B);
+    F.fold = fcg_fold(F);
+    F.fold_G = F.fold ? mg_fold_groups(nx, ny, nz) : 0u;
     if (precond == 1) {
         RefCtrl h;
         BS_RETURN_IF_ERR(hipMemcpyAsync(&h, R.rc, sizeof(h), hipMemcpyDeviceToHost, st));

@@ -29,11 +29,28 @@ size_t mg_ws_bytes(int nx, int ny, int nz);
 // level through *n_fluid_coarse (synchronises once).
 int mg_setup(hipStream_t st, void* ws, int nx, int ny, int nz, const int* mat, int mf, int ma, const uint8_t* fcode,
              double* n_fluid_coarse);
+// The flexible CG's residual update folded into the V-cycle's first fine
+// pass: the pass reads r_old (mg_apply's r) and Ap, works on and writes
+// r_out = r_old - (*alpha) Ap (r_out must not alias r_old) and one r.r
+// partial per workgroup into rr_part (mg_fold_groups of them); `after` is
+// enqueued right after that pass (the convergence check), and the rest of
+// the cycle uses r_out.
+struct MgFold {
+    const float* ap;
+    const double* alpha;
+    float* r_out;
+    double* rr_part;
+    void (*after)(hipStream_t, void*);
+    void* after_arg;
+};
+// Workgroups (r.r partials) of the folded first pass; 0 when the fine level
+// cannot take it (nx, ny even and nz % 4 == 0 are needed).
+unsigned mg_fold_groups(int nx, int ny, int nz);
 // z = V-cycle(r); also writes partials of r.z and r.z_old (z_old may be null)
 // into dots_part, mg_dots_groups(nx,ny,nz) workgroups of 2 values.
 int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* code0, const float* r, float* z,
              const float* z_old, double* dots_part, int coarse_kind, const float* params, const void* packed,
-             double n_fluid_coarse, const int* done);
+             double n_fluid_coarse, const int* done, const MgFold* fold = nullptr);
 unsigned mg_dots_groups(int nx, int ny, int nz);
 
 // e *= (r.e)/(e.A e) (0 when not positive): the A-norm-optimal multiple of a

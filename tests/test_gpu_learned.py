@@ -154,11 +154,14 @@ def test_projection_driver(simcore, torch_gpu, monkeypatch, solver):
 
 def test_graph_replay_matches_eager(simcore, torch_gpu, monkeypatch):
     """The flexible CG replays a captured HIP graph of two iterations; running
-    the same solve with eager launches, with the iteration's first pass split
-    into SpMV, update and direction kernels (x updated at once instead of an
-    iteration later), and the graph path again after eager work and
-    synchronisations in between, gives bit-identical solutions and statistics
-    (the same float operations; the reductions are deterministic)."""
+    the same solve with eager launches, with the iteration's passes split
+    (SpMV, update and direction kernels, x updated at once instead of an
+    iteration later, the residual update outside the V-cycle's first pass),
+    and the graph path again after eager work and synchronisations in between,
+    gives bit-identical solutions and statistics (the same float operations;
+    the reductions are deterministic).  Only r.r is summed per workgroup of
+    another pass when the update is folded, so the recursive residual agrees
+    to rounding between split and folded solves."""
     torch = torch_gpu
     from boilsim.unet import UNet
 
@@ -176,9 +179,12 @@ def test_graph_replay_matches_eager(simcore, torch_gpu, monkeypatch):
         junk = torch.randn(1 << 20, device="cuda").sum().item()  # eager work between replays
         results.append((x, st, junk))
     x0, st0, _ = results[0]
-    for x, st, _ in results[1:]:
-        assert np.array_equal(x.view(np.uint32), x0.view(np.uint32))
-        assert st == st0
+    for (x, st, _), mode in zip(results[1:], ("eager", "split", "split-eager", "graph", "graph")):
+        assert np.array_equal(x.view(np.uint32), x0.view(np.uint32)), mode
+        if "split" in mode:
+            assert st["relres_recursive"] == pytest.approx(st0["relres_recursive"], rel=1e-9)
+            st = dict(st, relres_recursive=st0["relres_recursive"])
+        assert st == st0, mode
 
 
 def test_plume_projection_reaches_tolerance(simcore, torch_gpu, monkeypatch):
