@@ -152,7 +152,8 @@ def test_projection_driver(simcore, torch_gpu, monkeypatch, solver):
         assert rel2 < (1.05e-6 if solver != "cg-rust" else 1e-5)
 
 
-def test_graph_replay_matches_eager(simcore, torch_gpu, monkeypatch):
+@pytest.mark.parametrize("shape", [None, (63, 62, 48)])
+def test_graph_replay_matches_eager(simcore, torch_gpu, monkeypatch, shape):
     """The flexible CG replays a captured HIP graph of two iterations; running
     the same solve with eager launches, with the iteration's passes split
     (SpMV, update and direction kernels, x updated at once instead of an
@@ -161,12 +162,23 @@ def test_graph_replay_matches_eager(simcore, torch_gpu, monkeypatch):
     gives bit-identical solutions and statistics (the same float operations;
     the reductions are deterministic).  Only r.r is summed per workgroup of
     another pass when the update is folded, so the recursive residual agrees
-    to rounding between split and folded solves."""
+    to rounding between split and folded solves.  On the 64x64x48 pot every
+    pass is fused and the update folded; at 63x62x48 (odd nx) the V-cycle's
+    first pass cannot take the update, so the fused first pass runs with the
+    separate residual update."""
     torch = torch_gpu
+    from boilsim.fixtures import CONFIGS, DT, RHO, GridConfig, pot_mask, seeded_velocities
     from boilsim.unet import UNet
 
     net = UNet.load()
-    cfg, mat, div, dt, rho = _pot("small")
+    if shape is None:
+        cfg, mat, div, dt, rho = _pot("small")
+    else:
+        cfg = GridConfig("odd", *shape, CONFIGS["small"].dx)
+        mat = pot_mask(*shape)
+        ux, uy, uz = seeded_velocities(*shape, seed=11, plume=False)
+        div = oracle.divergence(ux, uy, uz, mat, cfg.dx)
+        dt, rho = DT, RHO
     mat_d, div_d = dev(torch, mat), dev(torch, div)
     results = []
     for mode in ("graph", "eager", "split", "split-eager", "graph", "graph"):
@@ -185,6 +197,10 @@ def test_graph_replay_matches_eager(simcore, torch_gpu, monkeypatch):
             assert st["relres_recursive"] == pytest.approx(st0["relres_recursive"], rel=1e-9)
             st = dict(st, relres_recursive=st0["relres_recursive"])
         assert st == st0, mode
+    # and the solve itself is right (the oracle's FP64 residual)
+    b = oracle.rhs_f64(div, mat, cfg.dx, dt, rho)
+    rel2, _ = oracle.residual_f64(x0, b, mat)
+    assert rel2 < 1.05e-6
 
 
 def test_plume_projection_reaches_tolerance(simcore, torch_gpu, monkeypatch):
