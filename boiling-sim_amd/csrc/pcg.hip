@@ -1329,6 +1329,7 @@ struct PairGraphKey {
     const void* packed;
     double n_fluid;
     int nx, ny, nz, kind, vec, fused, fold;
+    int unit;  // 0: the (even, odd) pair; 1 + par: the single iteration of parity par; 3: the start
     unsigned nwg;
     bool operator==(const PairGraphKey& o) const { return memcmp(this, &o, sizeof(*this)) == 0; }
 };
@@ -1337,11 +1338,11 @@ struct PairGraphEntry {
     hipGraphExec_t exec;
     bool used;
 };
-static PairGraphEntry g_graphs[8];
+static PairGraphEntry g_graphs[24];
 static int g_graph_next = 0;
 static hipStream_t g_capture_stream = nullptr;
 
-static int pair_graph(const Tiles& T, int vec, const FcgBuffers& F, hipGraphExec_t* out) {
+static int pair_graph(const Tiles& T, int vec, const FcgBuffers& F, hipGraphExec_t* out, int unit = 0) {
     PairGraphKey key;
     memset(&key, 0, sizeof(key));
     key.ws = F.B.code;
@@ -1352,12 +1353,22 @@ static int pair_graph(const Tiles& T, int vec, const FcgBuffers& F, hipGraphExec
     key.nx = F.nx; key.ny = F.ny; key.nz = F.nz; key.kind = F.kind; key.vec = vec; key.nwg = F.B.nwg;
     key.fused = F.fused ? 1 : 0;
     key.fold = F.fold ? 1 : 0;
+    key.unit = unit;
     for (auto& e : g_graphs)
         if (e.used && e.key == key) { *out = e.exec; return hipSuccess; }
     if (!g_capture_stream) BS_RETURN_IF_ERR(hipStreamCreateWithFlags(&g_capture_stream, hipStreamNonBlocking));
     BS_RETURN_IF_ERR(hipStreamBeginCapture(g_capture_stream, hipStreamCaptureModeRelaxed));
-    int rc = fcg_enqueue_iteration(T, vec, F, 0, g_capture_stream);
-    if (rc == hipSuccess) rc = fcg_enqueue_iteration(T, vec, F, 1, g_capture_stream);
+    int rc = hipSuccess;
+    if (unit == 3) {
+        rc = fcg_precond(g_capture_stream, F, F.B.r, F.z[0], nullptr);
+        if (rc == hipSuccess) {
+            fcg_start(T, vec, F, g_capture_stream);
+            rc = (int)hipGetLastError();
+        }
+    } else {
+        rc = fcg_enqueue_iteration(T, vec, F, unit == 0 ? 0 : unit - 1, g_capture_stream);
+        if (rc == hipSuccess && unit == 0) rc = fcg_enqueue_iteration(T, vec, F, 1, g_capture_stream);
+    }
     hipGraph_t graph = nullptr;
     const hipError_t ec = hipStreamEndCapture(g_capture_stream, &graph);
     if (rc != hipSuccess) { if (graph) (void)hipGraphDestroy(graph); return rc; }
@@ -1367,7 +1378,7 @@ static int pair_graph(const Tiles& T, int vec, const FcgBuffers& F, hipGraphExec
     (void)hipGraphDestroy(graph);
     BS_RETURN_IF_ERR(ei);
     PairGraphEntry& slot = g_graphs[g_graph_next];
-    g_graph_next = (g_graph_next + 1) % 8;
+    g_graph_next = (g_graph_next + 1) % 24;
     if (slot.used) (void)hipGraphExecDestroy(slot.exec);
     slot.key = key;
     slot.exec = exec;
@@ -1376,53 +1387,90 @@ static int pair_graph(const Tiles& T, int vec, const FcgBuffers& F, hipGraphExec
     return hipSuccess;
 }
 
+// The first preconditioner application z[0] = M(r) and the first direction's
+// scalars (one graph launch instead of the cycle's ~55 kernel launches).
+static int fcg_begin(const Tiles& T, int vec, const FcgBuffers& F, hipStream_t st) {
+    const char* eager_env = getenv("BOILSIM_FCG_EAGER");
+    if (eager_env && eager_env[0] == '1') {
+        const int rc = fcg_precond(st, F, F.B.r, F.z[0], nullptr);
+        if (rc != hipSuccess) return rc;
+        fcg_start(T, vec, F, st);
+        return (int)hipGetLastError();
+    }
+    hipGraphExec_t g = nullptr;
+    const int rc = pair_graph(T, vec, F, &g, 3);
+    if (rc != hipSuccess) return rc;
+    return (int)hipGraphLaunch(g, st);
+}
+
 // Runs iterations from the device counter (even on entry) in graph-launched
-// pairs until done, polling the flag one batch behind.
+// units until done, polling the flag one unit behind.
 static int fcg_iterate(const Tiles& T, int vec, const FcgBuffers& F, int max_iter, hipStream_t st) {
     const PcgBuffers& B = F.B;
     BS_RETURN_IF_ERR((hipError_t)ensure_poll());
     // BOILSIM_FCG_EAGER=1 launches the kernels directly instead of replaying
-    // the captured graph (same results; used to check the graph path).
+    // the captured graphs (same results; used to check the graph path).
     const char* eager_env = getenv("BOILSIM_FCG_EAGER");
     const bool eager = eager_env && eager_env[0] == '1';
-    hipGraphExec_t exec = nullptr;
-    int rc = eager ? hipSuccess : pair_graph(T, vec, F, &exec);
+    hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};  // pair, single of parity 0, single of parity 1
+    int rc = hipSuccess;
+    for (int u = 0; u < 3 && !eager && rc == hipSuccess; ++u) rc = pair_graph(T, vec, F, &exec[u], u);
     if (rc != hipSuccess) return rc;
     hipEvent_t ev[2];
     BS_RETURN_IF_ERR(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     BS_RETURN_IF_ERR(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-    // Iteration pairs are queued two deep, each followed by a copy of the
-    // control block into a pinned slot, and polled oldest first.  A pair is
-    // milliseconds of GPU work, so the host stays ahead; once the residual's
-    // reduction per pair predicts that the pairs in flight reach the stopping
-    // test, no further pair is queued until that is confirmed, so at most the
-    // rest of the converging pair runs as no-ops (a wrong prediction costs
-    // one host round trip of GPU idle time).
+    // Units (an iteration pair, or a single iteration) are queued two deep,
+    // each followed by a copy of the control block into a pinned slot, and
+    // polled oldest first.  A unit is milliseconds of GPU work, so the host
+    // stays ahead.  From the residual's reduction per iteration between the
+    // last two polls: once the units in flight are predicted to reach the
+    // stopping test, no further unit is queued until that is confirmed;
+    // within two iterations of it the units are single iterations (so an odd
+    // count does not run a pair's second half as no-ops), and a single
+    // iteration also restores the even parity pairs start from.  A wrong
+    // prediction costs one host round trip of GPU idle time or one more
+    // unit.
     int it = 0, head = 0, inflight = 0;
+    int unit_its[2] = {0, 0};
     double rr_prev = -1.0, rr_last = -1.0, stop = -1.0;
+    int it_prev = 0, it_last = 0;
     bool finished = false;
     while (!finished && rc == hipSuccess) {
         int want = 2;
-        if (rr_prev > 0.0 && rr_last > 0.0 && inflight > 0) {
-            const double q = rr_last / rr_prev;  // per pair
-            double pred = rr_last;
-            for (int k = 0; k < inflight; ++k) pred *= q;
-            if (q < 1.0 && pred < stop) want = inflight;
-        }
+        double q1 = 1.0;  // per iteration
+        int its_inflight = 0;
+        for (int k = 0; k < inflight; ++k) its_inflight += unit_its[(head + k) & 1];
+        if (rr_prev > 0.0 && rr_last > 0.0 && it_last > it_prev) q1 = pow(rr_last / rr_prev, 1.0 / (it_last - it_prev));
+        const bool rated = rr_prev > 0.0 && rr_last > 0.0 && it_last > it_prev && q1 < 1.0;
+        if (rated && inflight > 0 && rr_last * pow(q1, its_inflight) < stop) want = inflight;
         while (inflight < want && it < max_iter && rc == hipSuccess) {
             const int slot = (head + inflight) & 1;
+            int n = 2;
+            if (it & 1) {
+                n = 1;  // back to even parity
+            } else if (rated) {
+                // singles once two more iterations are predicted to suffice:
+                // a pair could then end with a no-op half (an iteration's
+                // launches), a superfluous single costs only its poll copy
+                double pred = rr_last;
+                for (int k = 0; k < its_inflight + 2; ++k) pred *= q1;
+                if (pred < stop) n = 1;
+            }
+            if (it + n > max_iter) n = 1;
             if (eager) {
-                rc = fcg_enqueue_iteration(T, vec, F, 0, st);
-                if (rc == hipSuccess) rc = fcg_enqueue_iteration(T, vec, F, 1, st);
+                rc = fcg_enqueue_iteration(T, vec, F, it & 1, st);
+                if (rc == hipSuccess && n == 2) rc = fcg_enqueue_iteration(T, vec, F, 1, st);
             } else {
-                rc = hipGraphLaunch(exec, st);
+                rc = hipGraphLaunch(n == 2 ? exec[0] : exec[1 + (it & 1)], st);
             }
             if (rc == hipSuccess)
                 rc = hipMemcpyAsync(g_poll.ctl + slot, B.ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, st);
             if (rc == hipSuccess) rc = hipEventRecord(ev[slot], st);
             if (rc == hipSuccess) {
+                unit_its[slot] = n;
                 ++inflight;
-                it += 2;
+                its_inflight += n;
+                it += n;
             }
         }
         if (rc != hipSuccess || inflight == 0) break;
@@ -1434,7 +1482,9 @@ static int fcg_iterate(const Tiles& T, int vec, const FcgBuffers& F, int max_ite
         if (h.done) finished = true;
         stop = h.stop;
         rr_prev = rr_last;
+        it_prev = it_last;
         rr_last = h.rr;
+        it_last = h.iter;
     }
     if (rc == hipSuccess && F.fused) {
         BS_VEC_LAUNCH(fcg_finish_kernel, vec, B.nwg, st, T, B.p, F.p2, B.x, B.ctrl);
@@ -1532,10 +1582,8 @@ extern "C" int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx
         if (rc != hipSuccess) return rc;
     }
     fcg_prepare_kernel<<<1, 64, 0, st>>>(B.ctrl, 0);
-    int rc = fcg_precond(st, F, B.r, F.z[0], nullptr);
+    int rc = fcg_begin(T, vec, F, st);
     if (rc != hipSuccess) return rc;
-    fcg_start(T, vec, F, st);
-    BS_RETURN_IF_ERR(hipGetLastError());
     rc = fcg_iterate(T, vec, F, max_iter, st);
     if (rc != hipSuccess) return rc;
     PcgCtrl h = *g_poll.final;
@@ -2056,10 +2104,8 @@ extern "C" int bs_fcg_refine_solve(double* x, float* x32, const float* div_u, co
         BS_VECS_LAUNCH(refine_inner_start_kernel, float, vec, B.nwg, st, T, B.code, R.rs, R.rc, B.r, B.p, B.partS);
         pcg_setup_finalize<<<1, 256, 0, st>>>(B.partS, B.nwg, B.ctrl, 1);
         fcg_prepare_kernel<<<1, 64, 0, st>>>(B.ctrl, 0);
-        rc = fcg_precond(st, F, B.r, F.z[0], nullptr);
+        rc = fcg_begin(T, vec, F, st);
         if (rc != hipSuccess) return rc;
-        fcg_start(T, vec, F, st);
-        BS_RETURN_IF_ERR(hipGetLastError());
         rc = fcg_iterate(T, vec, F, max_inner, st);
         if (rc != hipSuccess) return rc;
         its += g_poll.final->iter;
