@@ -48,9 +48,9 @@ B_PCG_ITER = 47.0  # spmv 4+1+4, update 4+4+(4+4)+(4+4)+1, direction 4+1+(4+4)
 # Fine-level algorithmic bytes per cell of one learned flexible-CG iteration:
 # direction + spmv + deferred x step 25 (code, z, p_old, x; p, Ap, x),
 # residual update + pre-smoothing + residual + restriction 17.5 (code, r_old,
-# Ap; r, z, rc/8 x 4), prolongation + post-sweep + dots 17.5 (code, z_in,
-# coarse e/8 x 4, r, z_old; z).
-B_LEARNED_FINE = 60.0
+# Ap; r, z, rc/8 x 4), prolongation + post-sweep + r.z 13.5 (code, z_in,
+# coarse e/8 x 4, r; z).
+B_LEARNED_FINE = 56.0
 # HBM bytes per cell one two-sweep launch of jacobi_pair_kernel actually
 # moves at 440x440x280, measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (one counter per run) and calibrated by kernels of known traffic

@@ -58,6 +58,7 @@ struct PcgCtrl {
     int pad;
     double best_rr;
     double dots[2];  // flexible CG: r.z_new, r.z_old (reduced)
+    double apz;      // flexible CG, fused first pass: Ap.z of the iteration (z its preconditioned residual)
     int fresh;       // flexible CG: the next direction is p = z (first iteration of a solve)
     int pad2;
 };
@@ -429,8 +430,12 @@ __global__ __launch_bounds__(1024) void pap_reduce_kernel(const double* __restri
 // next direction p = z.  A preconditioner that is not positive on this
 // residual (r.z_new <= 0) ends the flexible CG for the plain-PCG fallback;
 // so does reaching max_iter.
+// apz_mode: r.z_old is not in the partials but follows from the iteration's
+// quantities, r_{k+1}.z_k = r_k.z_k - alpha_k (A p_k).z_k (the fused first
+// pass accumulates (A p_k).z_k while it holds both), which spares the last
+// preconditioner pass the z_old stream.
 __global__ __launch_bounds__(1024) void dots_reduce_kernel(const double* __restrict__ partZ, unsigned G,
-                                                           PcgCtrl* ctrl, int init) {
+                                                           PcgCtrl* ctrl, int init, int apz_mode = 0) {
     if (ctrl->done) return;
     double v[2];
     reduce1024<2>(partZ, G, v);
@@ -446,7 +451,9 @@ __global__ __launch_bounds__(1024) void dots_reduce_kernel(const double* __restr
     double beta = 0.0;
     if (!init) {
         const double rz_old = ctrl->rz[(it1 - 1) & 1];
-        beta = rz_old != 0.0 ? (v[0] - v[1]) / rz_old : 0.0;
+        const double rzo = apz_mode ? rz_old - ctrl->alpha * ctrl->apz : v[1];
+        if (apz_mode) ctrl->dots[1] = rzo;
+        beta = rz_old != 0.0 ? (v[0] - rzo) / rz_old : 0.0;
     }
     ctrl->rz[it1 & 1] = v[0];
     ctrl->rz_new = v[0];
@@ -1070,11 +1077,12 @@ __global__ __launch_bounds__(256) void fcg_spmv_dir_kernel(int nx, int ny, int n
     // raw z and p_old two planes ahead; directions at q - 1, q (q + 1 is
     // formed at the top of iteration q, a plane after its loads were issued)
     float4 Pc = fresh ? zero4 : ldv(p_old, i0);
-    float4 Vm = dir(ldv(z, i0 - 1), fresh ? zero4 : ldv(p_old, i0 - 1)), Vc = dir(ldv(z, i0), Pc);
+    float4 Zc = ldv(z, i0);
+    float4 Vm = dir(ldv(z, i0 - 1), fresh ? zero4 : ldv(p_old, i0 - 1)), Vc = dir(Zc, Pc);
     float4 Zn = ldv(z, i0 + 1), Pn = fresh ? zero4 : ldv(p_old, i0 + 1);
     float4 Xc = ldx(i0);
     uint32_t Km = ldk(i0 - 1), Kc = ldk(i0), Kn = ldk(i0 + 1);
-    double pap = 0.0;
+    double pap = 0.0, apz = 0.0;
     for (int q = i0; q < i1; ++q) {
         float* V = sV[q & 1] + 4;
         uint8_t* C = sC[q & 1] + 8;
@@ -1115,6 +1123,10 @@ __global__ __launch_bounds__(256) void fcg_spmv_dir_kernel(int nx, int ny, int n
                 out[e] = 6.0f * ps - s;
                 pap += (double)ps * (double)out[e];
             }
+            apz += (double)out[0] * (double)Zc.x;
+            apz += (double)out[1] * (double)Zc.y;
+            apz += (double)out[2] * (double)Zc.z;
+            apz += (double)out[3] * (double)Zc.w;
             const int64_t o = (int64_t)q * pl + off;
             *reinterpret_cast<float4*>(ap + o) = make_float4(out[0], out[1], out[2], out[3]);
             *reinterpret_cast<float4*>(p_new + o) = Vc;
@@ -1124,11 +1136,24 @@ __global__ __launch_bounds__(256) void fcg_spmv_dir_kernel(int nx, int ny, int n
             }
         }
         Vm = Vc; Vc = Vn; Pc = Pn;
-        Zn = Zf; Pn = Pf; Xc = Xn;
+        Zc = Zn; Zn = Zf; Pn = Pf; Xc = Xn;
         Km = Kc; Kc = Kn; Kn = Kf;
     }
-    double v[1] = {pap};
-    write_partials<1>(v, part);
+    double v[2] = {pap, apz};
+    write_partials<2>(v, part);
+}
+
+// p.Ap, alpha = r.z / p.Ap and Ap.z totals after the fused first pass.
+__global__ __launch_bounds__(1024) void fcg_pap_reduce_kernel(const double* __restrict__ part, unsigned G,
+                                                              PcgCtrl* ctrl) {
+    if (ctrl->done) return;
+    double v[2];
+    reduce1024<2>(part, G, v);
+    if (threadIdx.x != 0) return;
+    const double rz = ctrl->rz[ctrl->iter & 1];
+    ctrl->pAp = v[0];
+    ctrl->alpha = v[0] != 0.0 ? rz / v[0] : 0.0;
+    ctrl->apz = v[1];
 }
 
 // r -= alpha Ap (the update's residual half; x follows in the next
@@ -1273,13 +1298,13 @@ static int fcg_enqueue_iteration(const Tiles& T, int vec, const FcgBuffers& F, i
         float* pb[2] = {B.p, F.p2};
         fcg_spmv_dir_kernel<<<rs.grid, 256, 0, st>>>(T.nx, T.ny, T.nz, rs.ni, rs.jtiles, rs.ktiles, B.code, F.z[par],
                                                      pb[par], pb[par ^ 1], B.x, B.ap, B.partT, B.ctrl);
-        pap_reduce_kernel<<<1, 1024, 0, st>>>(B.partT, rs.grid, B.ctrl, -1);
+        fcg_pap_reduce_kernel<<<1, 1024, 0, st>>>(B.partT, rs.grid, B.ctrl);
         if (F.fold) {
             // r[par] -> r[1-par] in the V-cycle's first pass, then the check
             float* rb[2] = {B.r, F.r2};
-            const int rc = fcg_precond(st, F, rb[par], F.z[par ^ 1], F.z[par], rb[par ^ 1]);
+            const int rc = fcg_precond(st, F, rb[par], F.z[par ^ 1], nullptr, rb[par ^ 1]);
             if (rc != hipSuccess) return rc;
-            dots_reduce_kernel<<<1, 1024, 0, st>>>(F.partZ, F.partZ_G, B.ctrl, 0);
+            dots_reduce_kernel<<<1, 1024, 0, st>>>(F.partZ, F.partZ_G, B.ctrl, 0, 1);
             return (int)hipGetLastError();
         }
         BS_VEC_LAUNCH(fcg_rupdate_kernel, vec, B.nwg, st, T, B.ap, B.r, B.partB, B.ctrl);
@@ -1288,9 +1313,10 @@ static int fcg_enqueue_iteration(const Tiles& T, int vec, const FcgBuffers& F, i
         BS_VEC_LAUNCH(fcg_update_kernel, vec, B.nwg, st, T, B.p, B.ap, B.x, B.r, B.partA, B.partB, B.ctrl);
     }
     fcg_check_kernel<<<1, 256, 0, st>>>(B.partB, B.nwg, B.ctrl);
-    const int rc = fcg_precond(st, F, B.r, F.z[par ^ 1], F.z[par]);
+    // fused: r.z_old from the first pass's Ap.z instead of a z_old stream
+    const int rc = fcg_precond(st, F, B.r, F.z[par ^ 1], F.fused ? nullptr : F.z[par]);
     if (rc != hipSuccess) return rc;
-    dots_reduce_kernel<<<1, 1024, 0, st>>>(F.partZ, F.partZ_G, B.ctrl, 0);
+    dots_reduce_kernel<<<1, 1024, 0, st>>>(F.partZ, F.partZ_G, B.ctrl, 0, F.fused ? 1 : 0);
     if (!F.fused) BS_VEC_LAUNCH(fcg_direction_kernel, vec, B.nwg, st, T, F.z[par ^ 1], B.p, B.ctrl, 0);
     return (int)hipGetLastError();
 }
@@ -1304,11 +1330,13 @@ static void fcg_start(const Tiles& T, int vec, const FcgBuffers& F, hipStream_t 
 }
 
 // The fused first pass applies where the i-marching SpMV does;
-// BOILSIM_FCG_SPLIT=1 selects the separate passes (same iterates).
+// BOILSIM_FCG_SPLIT=1 selects the separate passes (the same iterates up to
+// the rounding of r.z_old, which the fused path forms from Ap.z).
 static bool fcg_fused(const Tiles& T, int vec, const PcgBuffers& B) {
     const char* e = getenv("BOILSIM_FCG_SPLIT");
     if (e && e[0] == '1') return false;
-    return roll_fits(T, vec, B, true);
+    // two partials (p.Ap, Ap.z) per workgroup go to the per-tile slots
+    return roll_fits(T, vec, B, true) && 2 * roll_shape(T.nx, T.ny, T.nz).grid <= max_tiles(T.nx, T.ny, T.nz);
 }
 
 // The folded residual update: fused first pass, V-cycle preconditioner whose

@@ -159,10 +159,12 @@ def test_graph_replay_matches_eager(simcore, torch_gpu, monkeypatch, shape):
     (SpMV, update and direction kernels, x updated at once instead of an
     iteration later, the residual update outside the V-cycle's first pass),
     and the graph path again after eager work and synchronisations in between,
-    gives bit-identical solutions and statistics (the same float operations;
-    the reductions are deterministic).  Only r.r is summed per workgroup of
-    another pass when the update is folded, so the recursive residual agrees
-    to rounding between split and folded solves.  On the 64x64x48 pot every
+    gives bit-identical solutions and statistics among the fused runs (the
+    same float operations; the reductions are deterministic).  The split
+    passes form r.z_old from z_old itself, the fused ones as r.z - alpha Ap.z
+    (the same quantity in exact arithmetic) and sum r.r per workgroup of
+    another pass: their solves agree to rounding, with the same iteration
+    count.  On the 64x64x48 pot every
     pass is fused and the update folded; at 63x62x48 (odd nx) the V-cycle's
     first pass cannot take the update, so the fused first pass runs with the
     separate residual update."""
@@ -191,11 +193,16 @@ def test_graph_replay_matches_eager(simcore, torch_gpu, monkeypatch, shape):
         junk = torch.randn(1 << 20, device="cuda").sum().item()  # eager work between replays
         results.append((x, st, junk))
     x0, st0, _ = results[0]
+    xs = results[2][0]
     for (x, st, _), mode in zip(results[1:], ("eager", "split", "split-eager", "graph", "graph")):
-        assert np.array_equal(x.view(np.uint32), x0.view(np.uint32)), mode
         if "split" in mode:
-            assert st["relres_recursive"] == pytest.approx(st0["relres_recursive"], rel=1e-9)
-            st = dict(st, relres_recursive=st0["relres_recursive"])
+            assert np.array_equal(x.view(np.uint32), xs.view(np.uint32)), mode
+            assert np.linalg.norm(x - x0) <= 1e-5 * np.linalg.norm(x0), mode
+            for key in ("iterations", "converged", "fallback", "pcg_iterations"):
+                assert st[key] == st0[key], (mode, key)
+            assert st["relres_true"] == pytest.approx(st0["relres_true"], rel=1e-2)
+            continue
+        assert np.array_equal(x.view(np.uint32), x0.view(np.uint32)), mode
         assert st == st0, mode
     # and the solve itself is right (the oracle's FP64 residual)
     b = oracle.rhs_f64(div, mat, cfg.dx, dt, rho)
