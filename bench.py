@@ -322,7 +322,7 @@ def main():
     gux, guy, guz = ux0.clone(), uy0.clone(), uz0.clone()
     p = torch.zeros(cfg.shape, dtype=torch.float32, device="cuda")
     div = torch.zeros_like(p)
-    bits = torch.zeros(1, dtype=torch.int32, device="cuda")
+    bits = torch.zeros(2, dtype=torch.int32, device="cuda")  # max|div|, max|u| as float bits
     net = UNet.load()
     ws_pcg = torch.empty(core.pcg_workspace_bytes(nx, ny, nz), dtype=torch.uint8, device="cuda")
     ws_f = {k: torch.empty(core.fcg_workspace_bytes(nx, ny, nz, k), dtype=torch.uint8, device="cuda")
@@ -344,8 +344,8 @@ def main():
                                 ptr(net.params), ptr(net.packed), ptr(ws_f[k]), ws_f[k].numel())
         es1.record()
         core.subtract_pressure_gradient(ptr(gux), ptr(guy), ptr(guz), ptr(p), ptr(gmat), nx, ny, nz, cfg.dx, DT, RHO)
-        core.compute_divergence(ptr(div), ptr(gux), ptr(guy), ptr(guz), ptr(gmat), nx, ny, nz, cfg.dx)
-        core.max_abs(ptr(div), n, ptr(bits))
+        # diagnostic divergence with max|div| (and max|u|) in one pass, as pressure_projection
+        core.divergence_max(ptr(div), ptr(gux), ptr(guy), ptr(guz), ptr(gmat), nx, ny, nz, cfg.dx, ptr(bits))
         return st
 
     def leg(solver):

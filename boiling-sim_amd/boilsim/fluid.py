@@ -244,9 +244,9 @@ def pressure_projection(grid: Grid, ws: FluidWorkspace, cfg: ScenarioConfig, dt:
         core.subtract_pressure_gradient(grid.ux.data_ptr(), grid.uy.data_ptr(), grid.uz.data_ptr(),
                                         grid.p.data_ptr(), grid.mat.data_ptr(), nx, ny, nz, dx, dt, rho,
                                         MAT_FLUID)
-        core.compute_divergence(ws.div_u.data_ptr(), grid.ux.data_ptr(), grid.uy.data_ptr(),
-                                grid.uz.data_ptr(), grid.mat.data_ptr(), nx, ny, nz, dx, MAT_FLUID)
-        core.max_abs(ws.div_u.data_ptr(), grid.nx * grid.ny * grid.nz, ws.max_bits.data_ptr())
+        # the diagnostic divergence and its max|div| in one pass (max|u| too)
+        core.divergence_max(ws.div_u.data_ptr(), grid.ux.data_ptr(), grid.uy.data_ptr(), grid.uz.data_ptr(),
+                            grid.mat.data_ptr(), nx, ny, nz, dx, ws.max_bits.data_ptr(), MAT_FLUID)
     return float(_bits_to_floats(ws.max_bits)[0])
 
 
