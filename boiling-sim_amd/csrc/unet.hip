@@ -140,11 +140,15 @@ inline ConvTiles conv_tiles(const Lvl& L) {
                      (uint32_t)((L.nz + kTK - 1) / kTK)};
 }
 
-template <int CIN, int COUT>
+// UPIN: `in` lives on the 2x-coarser grid (ny, nz = cny, cnz) and the
+// convolution reads its nearest-neighbour upsampling, in[i/2][j/2][k/2], as
+// up_kernel would have written it (the upsampled tensor never goes to
+// memory).
+template <int CIN, int COUT, bool UPIN = false>
 __global__ __launch_bounds__(256) void conv_mfma_kernel(
     Lvl L, ConvTiles T, const __bf16* __restrict__ in, const bf16x8* __restrict__ wfrag,
     const float* __restrict__ bias, int relu, __bf16* __restrict__ out, const __bf16* __restrict__ skip,
-    __bf16* __restrict__ out_sum, const int* __restrict__ done) {
+    __bf16* __restrict__ out_sum, const int* __restrict__ done, int cny = 0, int cnz = 0) {
     if (done && *done) return;
     static_assert(CIN % 8 == 0, "channels come in groups of 8");
     constexpr int KS = ksteps_of(CIN);
@@ -167,8 +171,11 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(
         const int hj = hr % kHJ, hi = hr / kHJ;
         const int gi = i0 - 1 + hi, gj = j0 - 1 + hj, gk = k0 - 1 + hk;
         bf16x8 val = bf16x8{};
-        if ((unsigned)gi < (unsigned)L.nx && (unsigned)gj < (unsigned)L.ny && (unsigned)gk < (unsigned)L.nz)
-            val = *reinterpret_cast<const bf16x8*>(in + ((int64_t)gi * plane + (int64_t)gj * L.nz + gk) * CIN + q * 8);
+        if ((unsigned)gi < (unsigned)L.nx && (unsigned)gj < (unsigned)L.ny && (unsigned)gk < (unsigned)L.nz) {
+            const int64_t src = UPIN ? ((int64_t)(gi >> 1) * cny + (gj >> 1)) * cnz + (gk >> 1)
+                                     : (int64_t)gi * plane + (int64_t)gj * L.nz + gk;
+            val = *reinterpret_cast<const bf16x8*>(in + src * CIN + q * 8);
+        }
         tile[q * kHalo + h] = val;
     }
     __syncthreads();
@@ -780,14 +787,19 @@ static unsigned elem_grid(int64_t n) {
     return (unsigned)(b < 1 ? 1 : b);
 }
 
+// up_from: `in` is on that (2x-coarser) level and is read upsampled.
 template <int CIN, int COUT>
 static void conv_fwd(hipStream_t st, const Lvl& L, const __bf16* in, const __bf16* packed, int l,
                      const float* params, int relu, __bf16* out, const __bf16* skip, __bf16* out_sum,
-                     const int* done) {
+                     const int* done, const Lvl* up_from = nullptr) {
     const ConvTiles T = conv_tiles(L);
-    conv_mfma_kernel<CIN, COUT><<<T.ti * T.tj * T.tk, 256, 0, st>>>(
-        L, T, in, reinterpret_cast<const bf16x8*>(packed + frag_off(l, 0)), params + conv_b_off(l), relu, out,
-        skip, out_sum, done);
+    const bf16x8* wf = reinterpret_cast<const bf16x8*>(packed + frag_off(l, 0));
+    if (up_from)
+        conv_mfma_kernel<CIN, COUT, true><<<T.ti * T.tj * T.tk, 256, 0, st>>>(
+            L, T, in, wf, params + conv_b_off(l), relu, out, skip, out_sum, done, up_from->ny, up_from->nz);
+    else
+        conv_mfma_kernel<CIN, COUT><<<T.ti * T.tj * T.tk, 256, 0, st>>>(L, T, in, wf, params + conv_b_off(l),
+                                                                        relu, out, skip, out_sum, done);
 }
 
 // Input gradient of conv l: the forward kernel on g with the flipped weights.
@@ -830,10 +842,17 @@ static int unet_forward(hipStream_t st, UNetWs& W, const float* params, const __
     pool_kernel<16><<<elem_grid(W.L3.n * 2), 256, 0, st>>>(W.L2, W.L3, W.h2, W.p2, done);
     conv_fwd<16, 32>(st, W.L3, W.p2, packed, 2, params, 1, W.h3a, nullptr, nullptr, done);
     conv_fwd<32, 32>(st, W.L3, W.h3a, packed, 3, params, 1, W.h3, nullptr, nullptr, done);
-    up_kernel<32><<<elem_grid(W.L2.n * 4), 256, 0, st>>>(W.L2, W.L3, W.h3, W.up3, done);
-    conv_fwd<32, 16>(st, W.L2, W.up3, packed, 4, params, 1, training ? W.u2 : nullptr, W.h2, W.s2, done);
-    up_kernel<16><<<elem_grid(W.L1.n * 2), 256, 0, st>>>(W.L1, W.L2, W.s2, W.up2, done);
-    conv_fwd<16, 8>(st, W.L1, W.up2, packed, 5, params, 1, training ? W.u1 : nullptr, W.h1, W.s1, done);
+    // The decoder convolutions read their coarser input upsampled on the fly;
+    // training keeps the upsampled tensors for the backward pass.
+    if (training) {
+        up_kernel<32><<<elem_grid(W.L2.n * 4), 256, 0, st>>>(W.L2, W.L3, W.h3, W.up3, done);
+        conv_fwd<32, 16>(st, W.L2, W.up3, packed, 4, params, 1, W.u2, W.h2, W.s2, done);
+        up_kernel<16><<<elem_grid(W.L1.n * 2), 256, 0, st>>>(W.L1, W.L2, W.s2, W.up2, done);
+        conv_fwd<16, 8>(st, W.L1, W.up2, packed, 5, params, 1, W.u1, W.h1, W.s1, done);
+    } else {
+        conv_fwd<32, 16>(st, W.L2, W.h3, packed, 4, params, 1, nullptr, W.h2, W.s2, done, &W.L3);
+        conv_fwd<16, 8>(st, W.L1, W.s2, packed, 5, params, 1, nullptr, W.h1, W.s1, done, &W.L2);
+    }
     conv_fwd<8, 8>(st, W.L1, W.s1, packed, 6, params, 1, W.d1, nullptr, nullptr, done);
     conv_out_kernel<<<rz_partials ? (unsigned)kPartWg : eg1, 256, 0, st>>>(
         W.L1, W.d1, params + conv_w_off(7), params + conv_b_off(7), W.rms, code, W.y, z,

@@ -1,5 +1,7 @@
 """GPU parity of the U-Net kernels (MFMA implicit-GEMM convolutions, pooling,
-upsampling, loss gradient, weight gradients, Adam) against the FP64 oracle.
+upsampling, loss gradient, weight gradients, Adam) against the FP64 oracle,
+and of the inference forward pass (upsampling read inside the convolutions)
+against the training one.
 
 The oracle rounds activations to BF16 where the kernels store BF16, so the
 forward pass is compared at a tolerance set by FP32-vs-FP64 accumulation and
@@ -66,6 +68,34 @@ def test_forward_backward_match_oracle(torch_gpu, shape, seed):
         err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
         print(f"  conv{li + 1} ({ci}->{co}) grad rel L2 {err:.2e}")
         assert err < 3e-2, (li, err)
+
+
+@pytest.mark.parametrize("shape,seed", [((20, 18, 14), 4), ((11, 13, 9), 5), ((110, 110, 70), 6)])
+def test_inference_forward_matches_training_forward(torch_gpu, shape, seed):
+    """Inference reads the decoder's upsampled inputs on the fly inside the
+    convolutions' staging; training writes them with up_kernel (its backward
+    needs them).  Nearest-neighbour upsampling copies values, so the two
+    forward passes agree bit for bit; with the oracle comparison of the
+    training forward above this pins the inference path too."""
+    torch = torch_gpu
+    from boilsim.unet import UNet, init_params
+
+    code, r, _ = _case(shape, seed)
+    params = init_params(seed)
+    params[U.num_params() - 1] = 0.05
+    net = UNet(params)
+    n_fluid = int((code >= 2).sum())
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    r_d, code_d = dev(r), dev(code)
+    out = {}
+    for training in (True, False):
+        z_d = torch.full(shape, np.nan, dtype=torch.float32, device="cuda")
+        ws = torch.zeros(UNet.workspace_bytes(shape, training=training), dtype=torch.uint8, device="cuda")
+        net.forward(r_d, code_d, n_fluid, z_d, ws, training=training)
+        torch.cuda.synchronize()
+        out[training] = z_d.cpu().numpy()
+    assert np.isfinite(out[False]).all()
+    assert np.array_equal(out[True].view(np.uint32), out[False].view(np.uint32))
 
 
 def test_adam_step_matches_numpy(torch_gpu):
