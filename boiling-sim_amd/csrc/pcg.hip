@@ -1347,10 +1347,12 @@ static bool fcg_fold(const FcgBuffers& F) {
     return g > 0 && g <= max_tiles(F.nx, F.ny, F.nz);
 }
 
-// Instantiated graphs of an (even, odd) iteration pair, keyed by everything
-// the captured launches depend on.  The ~60 launches of one iteration (most
-// of them small coarse-level kernels) become one graph launch.
-struct PairGraphKey {
+// Instantiated graphs of the flexible CG's launch units (an (even, odd)
+// iteration pair, a single iteration of either parity, the start), keyed by
+// everything the captured launches depend on.  The ~55 launches of one
+// iteration (most of them small coarse-level kernels) become one graph
+// launch.
+struct UnitGraphKey {
     const void* ws;
     const float* x;
     const float* params;
@@ -1359,19 +1361,19 @@ struct PairGraphKey {
     int nx, ny, nz, kind, vec, fused, fold;
     int unit;  // 0: the (even, odd) pair; 1 + par: the single iteration of parity par; 3: the start
     unsigned nwg;
-    bool operator==(const PairGraphKey& o) const { return memcmp(this, &o, sizeof(*this)) == 0; }
+    bool operator==(const UnitGraphKey& o) const { return memcmp(this, &o, sizeof(*this)) == 0; }
 };
-struct PairGraphEntry {
-    PairGraphKey key;
+struct UnitGraphEntry {
+    UnitGraphKey key;
     hipGraphExec_t exec;
     bool used;
 };
-static PairGraphEntry g_graphs[24];
+static UnitGraphEntry g_graphs[24];
 static int g_graph_next = 0;
 static hipStream_t g_capture_stream = nullptr;
 
-static int pair_graph(const Tiles& T, int vec, const FcgBuffers& F, hipGraphExec_t* out, int unit = 0) {
-    PairGraphKey key;
+static int unit_graph(const Tiles& T, int vec, const FcgBuffers& F, hipGraphExec_t* out, int unit = 0) {
+    UnitGraphKey key;
     memset(&key, 0, sizeof(key));
     key.ws = F.B.code;
     key.x = F.B.x;
@@ -1405,7 +1407,7 @@ static int pair_graph(const Tiles& T, int vec, const FcgBuffers& F, hipGraphExec
     const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     BS_RETURN_IF_ERR(ei);
-    PairGraphEntry& slot = g_graphs[g_graph_next];
+    UnitGraphEntry& slot = g_graphs[g_graph_next];
     g_graph_next = (g_graph_next + 1) % 24;
     if (slot.used) (void)hipGraphExecDestroy(slot.exec);
     slot.key = key;
@@ -1426,7 +1428,7 @@ static int fcg_begin(const Tiles& T, int vec, const FcgBuffers& F, hipStream_t s
         return (int)hipGetLastError();
     }
     hipGraphExec_t g = nullptr;
-    const int rc = pair_graph(T, vec, F, &g, 3);
+    const int rc = unit_graph(T, vec, F, &g, 3);
     if (rc != hipSuccess) return rc;
     return (int)hipGraphLaunch(g, st);
 }
@@ -1442,7 +1444,7 @@ static int fcg_iterate(const Tiles& T, int vec, const FcgBuffers& F, int max_ite
     const bool eager = eager_env && eager_env[0] == '1';
     hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};  // pair, single of parity 0, single of parity 1
     int rc = hipSuccess;
-    for (int u = 0; u < 3 && !eager && rc == hipSuccess; ++u) rc = pair_graph(T, vec, F, &exec[u], u);
+    for (int u = 0; u < 3 && !eager && rc == hipSuccess; ++u) rc = unit_graph(T, vec, F, &exec[u], u);
     if (rc != hipSuccess) return rc;
     hipEvent_t ev[2];
     BS_RETURN_IF_ERR(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
