@@ -133,7 +133,11 @@ __global__ __launch_bounds__(256) void codes_from_mat_kernel(MLvl L, const int* 
         if (k < L.nz - 1) ns += solid(c + 1);
         code[c] = (uint8_t)(kCodeFluid0 + ns);
     }
-    if (nfluid) atomicAdd(nfluid, cnt);
+    if (nfluid) {  // one atomic per workgroup (per-thread ones on one address serialise)
+        double v[1] = {(double)cnt};
+        block_sums<1>(v);
+        if (threadIdx.x == 0) atomicAdd(nfluid, (unsigned int)v[0]);
+    }
 }
 
 // z = w D^-1 r (first sweep from z = 0).
@@ -1142,7 +1146,8 @@ int mg_setup(hipStream_t st, void* ws, int nx, int ny, int nz, const int* mat, i
     coarse_class_kernel<<<grid_for(W.L1.n), 256, 0, st>>>(W.L0, W.L1, fcode, W.mat1);
     codes_from_mat_kernel<<<grid_for(W.L1.n), 256, 0, st>>>(W.L1, W.mat1, 0, 2, W.code1, nullptr);
     coarse_class_kernel<<<grid_for(W.L2.n), 256, 0, st>>>(W.L1, W.L2, W.code1, W.mat2);
-    codes_from_mat_kernel<<<grid_for(W.L2.n), 256, 0, st>>>(W.L2, W.mat2, 0, 2, W.code2, W.nfl);
+    // the fluid count's atomics: 256 workgroups, one each
+    codes_from_mat_kernel<<<256, 256, 0, st>>>(W.L2, W.mat2, 0, 2, W.code2, W.nfl);
     BS_RETURN_IF_ERR(hipGetLastError());
     unsigned int h = 0;
     BS_RETURN_IF_ERR(hipMemcpyAsync(&h, W.nfl, sizeof(h), hipMemcpyDeviceToHost, st));

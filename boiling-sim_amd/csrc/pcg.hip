@@ -1217,7 +1217,9 @@ __global__ __launch_bounds__(256) void count_fluid_kernel(int64_t n, const uint8
     unsigned cnt = 0;
     const int64_t stride = (int64_t)gridDim.x * 256;
     for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += stride) cnt += code[t] >= kCodeFluid0;
-    atomicAdd((unsigned int*)out, cnt);
+    double v[1] = {(double)cnt};  // one atomic per workgroup
+    block_sums<1>(v);
+    if (threadIdx.x == 0) atomicAdd((unsigned int*)out, (unsigned int)v[0]);
 }
 
 __global__ void fcg_prepare_kernel(PcgCtrl* ctrl, int it0) {
