@@ -111,6 +111,37 @@ __global__ __launch_bounds__(256) void coarse_class_kernel(MLvl F, MLvl C, const
     }
 }
 
+// The same for fine nz % 4 == 0 (and 4-byte aligned codes): a thread forms
+// two coarse cells along k from four 4-byte rows of fine codes.
+__global__ __launch_bounds__(256) void coarse_class2_kernel(MLvl F, MLvl C, const uint8_t* __restrict__ fcode,
+                                                            int* __restrict__ cmat) {
+    const int64_t half = C.n >> 1;  // C.nz is even
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t h = (int64_t)blockIdx.x * 256 + threadIdx.x; h < half; h += stride) {
+        const int64_t cc = 2 * h;
+        int I, J, K;
+        ijk(C, (uint32_t)cc, I, J, K);
+        uint32_t any_fl = 0u, any_air = 0u;  // byte e: fine k = 2K + e
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) {
+                const int i = 2 * I + a, j = 2 * J + b;
+                if (i >= F.nx || j >= F.ny) continue;
+                const uint32_t w = *reinterpret_cast<const uint32_t*>(fcode + ((int64_t)i * F.ny + j) * F.nz + 2 * K);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const uint32_t q = (w >> (8 * e)) & 0xFFu;
+                    any_fl |= (q >= kCodeFluid0 ? 1u : 0u) << e;
+                    any_air |= (q == kCodeAir ? 1u : 0u) << e;
+                }
+            }
+#pragma unroll
+        for (int o = 0; o < 2; ++o) {
+            const uint32_t m = 3u << (2 * o);  // fine k = 2(K+o), 2(K+o)+1
+            cmat[cc + o] = (any_fl & m) ? 0 : ((any_air & m) ? 2 : 1);
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void codes_from_mat_kernel(MLvl L, const int* __restrict__ mat, int mf,
                                                              int ma, uint8_t* __restrict__ code,
                                                              unsigned int* __restrict__ nfluid) {
@@ -1143,7 +1174,10 @@ int mg_setup(hipStream_t st, void* ws, int nx, int ny, int nz, const int* mat, i
         codes_from_mat_kernel<<<grid_for(W.L0.n), 256, 0, st>>>(W.L0, mat, mf, ma, (uint8_t*)W.zt0, nullptr);
         fcode = (const uint8_t*)W.zt0;
     }
-    coarse_class_kernel<<<grid_for(W.L1.n), 256, 0, st>>>(W.L0, W.L1, fcode, W.mat1);
+    if ((W.L0.nz & 3) == 0 && ((uintptr_t)fcode & 3) == 0)
+        coarse_class2_kernel<<<grid_for(W.L1.n / 2), 256, 0, st>>>(W.L0, W.L1, fcode, W.mat1);
+    else
+        coarse_class_kernel<<<grid_for(W.L1.n), 256, 0, st>>>(W.L0, W.L1, fcode, W.mat1);
     codes_from_mat_kernel<<<grid_for(W.L1.n), 256, 0, st>>>(W.L1, W.mat1, 0, 2, W.code1, nullptr);
     coarse_class_kernel<<<grid_for(W.L2.n), 256, 0, st>>>(W.L1, W.L2, W.code1, W.mat2);
     // the fluid count's atomics: 256 workgroups, one each

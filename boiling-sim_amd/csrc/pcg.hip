@@ -78,8 +78,9 @@ struct PcgBuffers {
 __device__ __forceinline__ int is_solid_m(int m, int mf, int ma) { return (m != mf && m != ma) ? 1 : 0; }
 
 // ---------------------------------------------------------------------------
-// Set-up: class codes, b, and (cold start) x = 0, r = b, p = z = M^-1 r;
-// partials of b.b and r.z.  Reads the int32 mask once (neighbours via cache).
+// Set-up: class codes, b, and (cold start) x = 0, r = b, p = z = M^-1 r (p
+// may be null); partials of b.b and r.z.  Reads the int32 mask once
+// (neighbours via cache).
 // ---------------------------------------------------------------------------
 template <int VEC>
 __global__ __launch_bounds__(256) void pcg_setup_kernel(
@@ -134,7 +135,7 @@ __global__ __launch_bounds__(256) void pcg_setup_kernel(
         if (cold) {
             stf<VEC>(x, c0, zero);
             stf<VEC>(r, c0, bv);
-            stf<VEC>(p, c0, zv);
+            if (p) stf<VEC>(p, c0, zv);  // null: the caller forms its own first direction
         }
     }
     double v[2] = {bb, rz};
@@ -1595,7 +1596,7 @@ extern "C" int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx
     const float bscale = -rho * dx * dx / dt;
     pcg_ctrl_init_kernel<<<1, 64, 0, st>>>(B.ctrl, tol * tol, max_iter, 1);
     BS_VEC_LAUNCH(pcg_setup_kernel, vec, B.nwg, st, T, mat, div_u, bscale, mat_fluid, mat_air,
-                  warm_start ? 0 : 1, B.code, B.b, B.x, B.r, B.p, B.partS);
+                  warm_start ? 0 : 1, B.code, B.b, B.x, B.r, nullptr, B.partS);
     pcg_setup_finalize<<<1, 256, 0, st>>>(B.partS, B.nwg, B.ctrl, warm_start ? 0 : 1);
     if (warm_start) {
         BS_VEC_LAUNCH(pcg_residual_kernel, vec, B.nwg, st, T, B.code, B.b, B.x, B.r, B.p, B.partS, 1);
