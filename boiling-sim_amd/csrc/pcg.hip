@@ -142,9 +142,9 @@ __global__ __launch_bounds__(256) void pcg_setup_kernel(
     write_partials<2>(v, partS);
 }
 
-// r = b - A x formed in FP64 (stored rounded to FP32); partials of r.r and
-// r.z; with set_direction also p = z = M^-1 r (warm start / residual
-// replacement).
+// r = b - A x formed in FP64 (stored rounded to FP32 unless r is null, for
+// the test of the true residual alone); partials of r.r and r.z; with
+// set_direction also p = z = M^-1 r (warm start / residual replacement).
 template <int VEC>
 __global__ __launch_bounds__(256) void pcg_residual_kernel(
     Tiles T, const uint8_t* __restrict__ code, const float* __restrict__ b, const float* __restrict__ x,
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(256) void pcg_residual_kernel(
             rr += r64 * r64;
             rz += (double)rv[e] * (double)zv[e];
         }
-        stf<VEC>(r, c0, rv);
+        if (r) stf<VEC>(r, c0, rv);
         if (set_direction) stf<VEC>(p, c0, zv);
     }
     double v[2] = {rr, rz};
@@ -874,7 +874,7 @@ extern "C" int bs_pcg_solve(float* x, const float* div_u, const int* mat, int nx
         int rc = pcg_iterate(T, vec, B, it0, max_iter, st);
         if (rc != hipSuccess) return rc;
         // True residual of the current x (r and p untouched).
-        BS_VEC_LAUNCH(pcg_residual_kernel, vec, B.nwg, st, T, B.code, B.b, B.x, B.ap, B.p, B.partS, 0);
+        BS_VEC_LAUNCH(pcg_residual_kernel, vec, B.nwg, st, T, B.code, B.b, B.x, nullptr, B.p, B.partS, 0);
         pcg_residual_finalize<<<1, 256, 0, st>>>(B.partS, B.nwg, B.ctrl, 0, 0);
         BS_RETURN_IF_ERR(hipGetLastError());
         BS_RETURN_IF_ERR(hipMemcpyAsync(g_poll.final, B.ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, st));
@@ -1625,7 +1625,7 @@ extern "C" int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx
     // True residual; plain PCG (residual replacement) while it is above tol.
     int restarts = 0, it0 = h.iter;
     for (;;) {
-        BS_VEC_LAUNCH(pcg_residual_kernel, vec, B.nwg, st, T, B.code, B.b, B.x, B.ap, B.p, B.partS, 0);
+        BS_VEC_LAUNCH(pcg_residual_kernel, vec, B.nwg, st, T, B.code, B.b, B.x, nullptr, B.p, B.partS, 0);
         pcg_residual_finalize<<<1, 256, 0, st>>>(B.partS, B.nwg, B.ctrl, 0, 0);
         BS_RETURN_IF_ERR(hipGetLastError());
         BS_RETURN_IF_ERR(hipMemcpyAsync(g_poll.final, B.ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, st));
