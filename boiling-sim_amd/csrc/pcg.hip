@@ -35,6 +35,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
+
 namespace bs {
 
 // Device control block of one solve (lives in caller-provided memory).
@@ -835,23 +837,37 @@ static void carve(void* ws, size_t n, PcgBuffers& B) {
     B.partT = (double*)c;
 }
 
-// Solves A x = b (b from div_u as above) to ||b - A x||_2 <= tol ||b||_2 on
-// the true residual: CG runs until the recursive residual meets tol, then the
-// true residual is formed in one FP64 stencil pass; while it is above tol the
-// solve restarts from x with r = b - A x (residual replacement), at most
-// max_restarts times.  warm_start != 0 uses x's contents as the initial guess.
-extern "C" int bs_pcg_solve(float* x, const float* div_u, const int* mat, int nx, int ny, int nz,
-                            float dx, float dt, float rho, int mat_fluid, int mat_air, double tol,
-                            int max_iter, int warm_start, int max_restarts, void* workspace,
-                            size_t workspace_bytes, hipStream_t st, bs_solve_stats* stats) {
-    if (stats) memset(stats, 0, sizeof(*stats));
-    if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
-    if (workspace_bytes < bs_pcg_workspace_bytes(nx, ny, nz)) return (int)hipErrorInvalidValue;
-    BS_RETURN_IF_ERR((hipError_t)ensure_poll());
-    const size_t n = (size_t)nx * ny * nz;
-    PcgBuffers B;
-    carve(workspace, n, B);
-    B.x = x;
+namespace bs {
+// Queues PCG iterations like pcg_iterate, but waits for each batch with a
+// plain copy of the done flag into host memory the caller owns (no pinned
+// block, no events: nothing is allocated); the final control block lands in
+// *h.
+static int pcg_iterate_sync(const Tiles& T, int vec, const PcgBuffers& B, int it0, int max_iter,
+                            hipStream_t st, PcgCtrl* h) {
+    int it = it0, batch = 8;
+    while (it < max_iter) {
+        const int nb = (max_iter - it) < batch ? (max_iter - it) : batch;
+        for (int q = 0; q < nb; ++q, ++it) enqueue_pcg_iteration<float>(T, vec, B, it, st);
+        BS_RETURN_IF_ERR(hipGetLastError());
+        int done = 0;
+        BS_RETURN_IF_ERR(hipMemcpyAsync(&done, (const char*)B.ctrl + offsetof(PcgCtrl, done), sizeof(int),
+                                        hipMemcpyDeviceToHost, st));
+        BS_RETURN_IF_ERR(hipStreamSynchronize(st));
+        if (done) break;
+        if (batch < 32) batch *= 2;
+    }
+    BS_RETURN_IF_ERR(hipMemcpyAsync(h, B.ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, st));
+    return (int)hipStreamSynchronize(st);
+}
+
+// The body of bs_pcg_solve over carved buffers (B.x set).  pinned: poll the
+// done flag one batch behind through the pinned host block; otherwise through
+// pcg_iterate_sync into host memory on the stack.
+static int pcg_solve_body(PcgBuffers& B, const float* div_u, const int* mat, int nx, int ny, int nz,
+                          float dx, float dt, float rho, int mat_fluid, int mat_air, double tol, int max_iter,
+                          int warm_start, int max_restarts, hipStream_t st, bs_solve_stats* stats,
+                          bool pinned) {
+    float* x = B.x;
     const int vec = vec_for(nz, {x, div_u, mat, B.b, B.r, B.p, B.ap, B.code});
     const Tiles T = make_tiles(nx, ny, nz, vec);
     B.nwg = pick_nwg(T, kMaxWg);
@@ -869,17 +885,19 @@ extern "C" int bs_pcg_solve(float* x, const float* div_u, const int* mat, int nx
     }
     BS_RETURN_IF_ERR(hipGetLastError());
     int restarts = 0, conv = 0, it0 = 0;
-    PcgCtrl h;
+    PcgCtrl h, hstage;
+    PcgCtrl* hc = pinned ? g_poll.final : &hstage;
     for (;;) {
-        int rc = pcg_iterate(T, vec, B, it0, max_iter, st);
+        int rc = pinned ? pcg_iterate(T, vec, B, it0, max_iter, st)
+                        : pcg_iterate_sync(T, vec, B, it0, max_iter, st, &hstage);
         if (rc != hipSuccess) return rc;
         // True residual of the current x (r and p untouched).
         BS_VEC_LAUNCH(pcg_residual_kernel, vec, B.nwg, st, T, B.code, B.b, B.x, nullptr, B.p, B.partS, 0);
         pcg_residual_finalize<<<1, 256, 0, st>>>(B.partS, B.nwg, B.ctrl, 0, 0);
         BS_RETURN_IF_ERR(hipGetLastError());
-        BS_RETURN_IF_ERR(hipMemcpyAsync(g_poll.final, B.ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, st));
+        BS_RETURN_IF_ERR(hipMemcpyAsync(hc, B.ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, st));
         BS_RETURN_IF_ERR(hipStreamSynchronize(st));
-        h = *g_poll.final;
+        h = *hc;
         const double rel_true = h.bsq > 0.0 ? sqrt(h.rr_true / h.bsq) : 0.0;
         if (stats) {
             stats->iterations = h.iter;
@@ -901,6 +919,82 @@ extern "C" int bs_pcg_solve(float* x, const float* div_u, const int* mat, int nx
         stats->relres_recursive = h.bsq > 0.0 ? sqrt(h.rr / h.bsq) : 0.0;
     }
     return hipSuccess;
+}
+}  // namespace bs
+
+// Solves A x = b (b from div_u as above) to ||b - A x||_2 <= tol ||b||_2 on
+// the true residual: CG runs until the recursive residual meets tol, then the
+// true residual is formed in one FP64 stencil pass; while it is above tol the
+// solve restarts from x with r = b - A x (residual replacement), at most
+// max_restarts times.  warm_start != 0 uses x's contents as the initial guess.
+extern "C" int bs_pcg_solve(float* x, const float* div_u, const int* mat, int nx, int ny, int nz,
+                            float dx, float dt, float rho, int mat_fluid, int mat_air, double tol,
+                            int max_iter, int warm_start, int max_restarts, void* workspace,
+                            size_t workspace_bytes, hipStream_t st, bs_solve_stats* stats) {
+    if (stats) memset(stats, 0, sizeof(*stats));
+    if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
+    if (workspace_bytes < bs_pcg_workspace_bytes(nx, ny, nz)) return (int)hipErrorInvalidValue;
+    BS_RETURN_IF_ERR((hipError_t)ensure_poll());
+    PcgBuffers B;
+    carve(workspace, (size_t)nx * ny * nz, B);
+    B.x = x;
+    return pcg_solve_body(B, div_u, mat, nx, ny, nz, dx, dt, rho, mat_fluid, mat_air, tol, max_iter, warm_start,
+                          max_restarts, st, stats, true);
+}
+
+// ---------------------------------------------------------------------------
+// bs_pcg_solve for callers that cannot allocate: its workspace is `scratch`,
+// a fixed device array of kScratchBytes that every device holds from the
+// library's load until the process ends.  Host code never passes `scratch`
+// itself (in host code the name is a host-side stand-in, not a device
+// address): each device's address of it is resolved once with
+// hipGetSymbolAddress, with that device current, and kept in g_scratch_ptr;
+// the kernels get carved pointers into that address.  One array per device
+// serves every call on it, so calls that overlap on one device race.  The
+// kernels write every buffer they read before reading it (the control block
+// by pcg_ctrl_init_kernel, codes/b/r/p by the set-up pass, partials by the
+// kernel before their consumer), so the previous call's contents never leak.
+// ---------------------------------------------------------------------------
+namespace bs {
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kScratchDevices = 64;
+static void* g_scratch_ptr[kScratchDevices];
+static std::mutex g_scratch_mu;
+
+// The current device's address of `scratch`, resolved on first use.
+int scratch_setup(void** out) {
+    int dev = 0;
+    BS_RETURN_IF_ERR(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchDevices) return (int)hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        BS_RETURN_IF_ERR(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = p;
+    }
+    if (out) *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+}  // namespace bs
+
+extern "C" size_t bs_pcg_noalloc_scratch_bytes(void) { return kScratchBytes; }
+
+extern "C" int bs_pcg_solve_noalloc(float* x, const float* div_u, const int* mat, int nx, int ny, int nz,
+                                    float dx, float dt, float rho, int mat_fluid, int mat_air, double tol,
+                                    int max_iter, int warm_start, int max_restarts, hipStream_t st,
+                                    bs_solve_stats* stats) {
+    if (stats) memset(stats, 0, sizeof(*stats));
+    if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
+    if (bs_pcg_workspace_bytes(nx, ny, nz) > kScratchBytes) return (int)hipErrorInvalidValue;
+    void* ws = nullptr;
+    BS_RETURN_IF_ERR((hipError_t)scratch_setup(&ws));
+    PcgBuffers B;
+    carve(ws, (size_t)nx * ny * nz, B);
+    B.x = x;
+    return pcg_solve_body(B, div_u, mat, nx, ny, nz, dx, dt, rho, mat_fluid, mat_air, tol, max_iter, warm_start,
+                          max_restarts, st, stats, false);
 }
 
 // ===========================================================================

@@ -59,4 +59,9 @@ size_t step_ws_bytes();
 void scale_to_optimal(hipStream_t st, int nx, int ny, int nz, const uint8_t* code, const float* r, float* e,
                       double* part, const int* done);
 
+// Resolves (once per device, with it current) the current device's address
+// of the fixed scratch array of bs_pcg_solve_noalloc and returns it through
+// *out when out is not null.
+int scratch_setup(void** out);
+
 }  // namespace bs

@@ -2,6 +2,7 @@
 // error strings and device queries for the Python shim.
 #include "boilsim_hip.h"
 #include "common.hpp"
+#include "precond_api.hpp"
 
 #include <stdio.h>
 #include <string.h>
@@ -32,6 +33,8 @@ extern "C" int bs_device_open(int device_id, char* buf, int buflen) {
     BS_RETURN_IF_ERR(hipGetDeviceCount(&count));
     if (device_id < 0 || device_id >= count) return (int)hipErrorInvalidDevice;
     BS_RETURN_IF_ERR(hipSetDevice(device_id));
+    // The no-allocation solver's scratch address on this device.
+    BS_RETURN_IF_ERR((hipError_t)bs::scratch_setup(nullptr));
     hipDeviceProp_t prop;
     BS_RETURN_IF_ERR(hipGetDeviceProperties(&prop, device_id));
     if (buf && buflen > 0)

@@ -103,6 +103,8 @@ SIGNATURES = {
     "bs_pressure_solve_pcg_launch": [VP, VP, VP, I, I, I, F, F, F, I, I, F, I] + [VP] * 13 + [POINTER(c_int)],
     "bs_pcg_workspace_bytes": [I, I, I],
     "bs_pcg_solve": [VP, VP, VP, I, I, I, F, F, F, I, I, c_double, I, I, I, VP, c_size_t, S, c_void_p],
+    "bs_pcg_noalloc_scratch_bytes": [],
+    "bs_pcg_solve_noalloc": [VP, VP, VP, I, I, I, F, F, F, I, I, c_double, I, I, I, S, c_void_p],
     "bs_laplacian_spmv_launch": [VP, VP, VP, I, I, I, I, I],
     "bs_diag_inverse_apply_launch": [VP, VP, VP, I, I, I, I, I],
     "bs_dot_launch": [VP, VP, I, VP, VP],
@@ -138,6 +140,7 @@ SIGNATURES = {
 _RESTYPES = {
     "bs_error_string": c_char_p,
     "bs_pcg_workspace_bytes": c_size_t,
+    "bs_pcg_noalloc_scratch_bytes": c_size_t,
     "bs_unet_num_params": c_size_t,
     "bs_unet_packed_bytes": c_size_t,
     "bs_unet_workspace_bytes": c_size_t,

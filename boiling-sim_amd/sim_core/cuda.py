@@ -261,6 +261,32 @@ class SimCore:
         )
         return st.as_dict()
 
+    @staticmethod
+    def pcg_noalloc_scratch_bytes() -> int:
+        return int(_lib.load().bs_pcg_noalloc_scratch_bytes())
+
+    def pcg_solve_noalloc(self, x_ptr, div_u_ptr, mat_ptr, nx, ny, nz, dx, dt, rho, tol, max_iter,
+                          warm_start=False, max_restarts=4, mat_fluid=0, mat_air=2, stream=0) -> dict:
+        """``pcg_solve`` without a workspace and without any allocation: the
+        library's fixed per-device scratch array is the workspace.  Grids whose
+        workspace exceeds it raise ValueError; calls that overlap on one device
+        race (they share that array)."""
+        need = self.pcg_workspace_bytes(nx, ny, nz)
+        have = self.pcg_noalloc_scratch_bytes()
+        if need > have:
+            raise ValueError(f"pcg_solve_noalloc: a {nx}x{ny}x{nz} grid needs {need} bytes of workspace; "
+                             f"the fixed scratch array holds {have} bytes (use pcg_solve with a workspace)")
+        st = _lib.SolveStats()
+        _lib.call(
+            "bs_pcg_solve_noalloc",
+            _ptr("x_ptr", x_ptr), _ptr("div_u_ptr", div_u_ptr), _ptr("mat_ptr", mat_ptr),
+            _usize("nx", nx), _usize("ny", ny), _usize("nz", nz),
+            float(dx), float(dt), float(rho), _i32("mat_fluid", mat_fluid), _i32("mat_air", mat_air),
+            float(tol), _usize("max_iter", max_iter), 1 if warm_start else 0, int(max_restarts),
+            _ptr("stream", stream), ctypes.byref(st),
+        )
+        return st.as_dict()
+
     # -- flexible CG with the learned preconditioner (new) ------------------------
     @staticmethod
     def fcg_workspace_bytes(nx, ny, nz, precond) -> int:

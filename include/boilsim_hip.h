@@ -180,6 +180,24 @@ int bs_pcg_solve(float* x, const float* div_u, const int* mat, int nx, int ny, i
                  float dt, float rho, int mat_fluid, int mat_air, double tol, int max_iter,
                  int warm_start, int max_restarts, void* workspace, size_t workspace_bytes,
                  bs_stream_t stream, bs_solve_stats* stats);
+/* Bytes of the fixed device scratch array of bs_pcg_solve_noalloc (64 MiB). */
+size_t bs_pcg_noalloc_scratch_bytes(void);
+/* bs_pcg_solve without a workspace argument and without allocating anything:
+ * no hipMalloc/hipFree, no pinned host memory, no events or graphs.  It has no
+ * counterpart in the reference; it is for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its workspace is one fixed array per device of
+ * bs_pcg_noalloc_scratch_bytes() bytes, held from the library's load until
+ * the process ends, so grids with bs_pcg_workspace_bytes(nx,ny,nz) above that
+ * (about 3.9 M cells) are rejected with hipErrorInvalidValue before any work
+ * is queued.  Every call on a device shares that device's array: calls that
+ * overlap on one device (several host threads, or streams that run
+ * concurrently) race.  Same arguments and results as bs_pcg_solve otherwise;
+ * it blocks on a plain device-to-host copy of the done flag after each batch
+ * of iterations. */
+int bs_pcg_solve_noalloc(float* x, const float* div_u, const int* mat, int nx, int ny, int nz, float dx,
+                         float dt, float rho, int mat_fluid, int mat_air, double tol, int max_iter,
+                         int warm_start, int max_restarts, bs_stream_t stream, bs_solve_stats* stats);
 
 /* ---- the reference PCG's building blocks, standalone ---------------------- */
 
