@@ -160,7 +160,10 @@ def compare(state: dict, ref: dict, fluid_value: int = 0, limits: dict | None = 
     max_abs, max_rel, bits_equal, the limit (None where the protocol sets
     none) and whether it passes.  Pressure is compared over fluid cells
     (mat == fluid_value) when both states hold `mat`; other fields over all
-    elements.  report["passed"] is True when every limited field passes."""
+    elements.  A field of `ref` that `state` lacks, or holds with another
+    shape or dtype, fails (its entry carries "error" instead of the metrics).
+    report["passed"] is True when at least one field was compared and every
+    field passes."""
     limits = PROTOCOL_LIMITS if limits is None else limits
     fluid = None
     if "mat" in state and "mat" in ref:
@@ -169,14 +172,23 @@ def compare(state: dict, ref: dict, fluid_value: int = 0, limits: dict | None = 
         fluid = np.asarray(ref["mat"]) == fluid_value
     report = {}
     for name in ref:
-        if name == "mat" or name not in state:
+        if name == "mat":
             continue
-        r = field_error(state[name], ref[name], fluid if name == "p" else None)
         lim = limits.get(name)
+        if name not in state:
+            report[name] = {"error": "missing from the state", "limit": lim, "passes": False}
+            continue
+        a, b = np.asarray(state[name]), np.asarray(ref[name])
+        if a.shape != b.shape or a.dtype != b.dtype:
+            report[name] = {"error": f"{a.dtype}{a.shape} against the reference's {b.dtype}{b.shape}",
+                            "limit": lim, "passes": False}
+            continue
+        r = field_error(a, b, fluid if name == "p" else None)
         r["limit"] = lim
         r["passes"] = True if lim is None else r["max_rel"] < lim
         report[name] = r
-    report["passed"] = all(v["passes"] for k, v in report.items() if isinstance(v, dict))
+    entries = [v for v in report.values() if isinstance(v, dict)]
+    report["passed"] = bool(entries) and all(v["passes"] for v in entries)
     return report
 
 

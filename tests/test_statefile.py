@@ -86,6 +86,29 @@ def test_protocol_comparison():
         statefile.compare({**ref, "mat": mat + 1}, ref)
 
 
+def test_protocol_comparison_missing_or_mismatched_fields():
+    """A state that lacks a reference field, or holds it with another shape or
+    dtype, fails the protocol; an empty state fails too."""
+    rng = np.random.default_rng(2)
+    mat = rng.choice([0, 1, 2], size=(5, 4, 6)).astype(np.int32)
+    ref = {"mat": mat, "p": rng.standard_normal((5, 4, 6)).astype(np.float32),
+           "T": (300 + rng.random((5, 4, 6))).astype(np.float32),
+           "ux": rng.standard_normal((6, 4, 6)).astype(np.float32)}
+    for drop in ("p", "T", "ux"):
+        st = {k: v.copy() for k, v in ref.items() if k != drop}
+        rep = statefile.compare(st, ref)
+        assert not rep["passed"] and not rep[drop]["passes"] and "missing" in rep[drop]["error"]
+    assert not statefile.compare({}, ref)["passed"]
+    assert not statefile.compare({"mat": mat}, ref)["passed"]
+    assert not statefile.compare(ref, {"mat": mat})["passed"]  # nothing compared
+    wrong_shape = {**ref, "T": ref["T"][:, :, :5].copy()}
+    rep = statefile.compare(wrong_shape, ref)
+    assert not rep["passed"] and "(5, 4, 5)" in rep["T"]["error"]
+    wrong_dtype = {**ref, "p": ref["p"].astype(np.float64)}
+    rep = statefile.compare(wrong_dtype, ref)
+    assert not rep["passed"] and "float64" in rep["p"]["error"]
+
+
 def test_golden_fixture_reproduced_by_oracle():
     """The committed 5-step state equals what the oracle computes today."""
     import sys
