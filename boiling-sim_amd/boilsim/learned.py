@@ -166,6 +166,20 @@ def evaluate_loss(net: UNet, code, n_fluid: int, rs, es) -> float:
 
 _FCG_WS = {"key": None, "buf": None}
 _REFINE_WS = {"key": None, "buf": None, "x64": None}
+_DEFAULT_NET = {"net": None}
+
+# Refinement steps of one lpcg-hip projection at most (each cold-starts the
+# inner solver on the new residual).
+MAX_REFINE_STEPS = 20
+
+
+def default_net() -> UNet:
+    """The shipped network, loaded (read, copied to the device and packed)
+    once per process, so every projection reuses the same parameter buffers
+    (and the flexible CG's captured graphs, which are keyed on them)."""
+    if _DEFAULT_NET["net"] is None:
+        _DEFAULT_NET["net"] = UNet.load()
+    return _DEFAULT_NET["net"]
 
 
 def learned_pcg_solve(core, grid, ws, dt, rho, tol, max_iter, net: UNet | None = None,
@@ -179,11 +193,14 @@ def learned_pcg_solve(core, grid, ws, dt, rho, tol, max_iter, net: UNet | None =
     unconverged solve continues from its iterate in FP64 iterative refinement
     around the same learned solver (bs_fcg_refine_solve) and grid.p receives
     the refined solution rounded to FP32; the stats then carry
-    ``refined: True`` and the refinement's true residual."""
+    ``refined: True`` and the refinement's true residual.  ``max_iter``
+    bounds the whole solve: the refinement's inner iterations get what the
+    FP32 solve left of it (and at most MAX_REFINE_STEPS steps); ``capped``
+    says a cap ended an unconverged solve."""
     import torch
 
     if net is None:
-        net = UNet.load()
+        net = default_net()
     shape = grid.shape
     key = (shape, precond)
     if _FCG_WS["key"] != key:
@@ -194,7 +211,10 @@ def learned_pcg_solve(core, grid, ws, dt, rho, tol, max_iter, net: UNet | None =
     st = core.fcg_solve(grid.p.data_ptr(), ws.div_u.data_ptr(), grid.mat.data_ptr(), *shape, grid.dx, dt, rho,
                         tol, max_iter, precond, net.params.data_ptr(), net.packed.data_ptr(), buf.data_ptr(),
                         buf.numel(), warm_start=warm_start)
-    if st["converged"] or not refine:
+    used = st["iterations"] + st["pcg_iterations"]
+    if st["converged"] or not refine or used >= max_iter:
+        if not st["converged"]:
+            st["capped"] = used >= max_iter
         return st
     if _REFINE_WS["key"] != key:
         nb = core.fcg_refine_workspace_bytes(*shape, precond)
@@ -205,11 +225,15 @@ def learned_pcg_solve(core, grid, ws, dt, rho, tol, max_iter, net: UNet | None =
     x64.copy_(grid.p)
     rs = core.fcg_refine_solve(x64.data_ptr(), ws.div_u.data_ptr(), grid.mat.data_ptr(), *shape, grid.dx, dt, rho,
                                tol, precond, net.params.data_ptr(), net.packed.data_ptr(), rbuf.data_ptr(),
-                               rbuf.numel(), warm_start=True, x32_ptr=grid.p.data_ptr())
+                               rbuf.numel(), max_outer=MAX_REFINE_STEPS, max_inner=max_iter - used,
+                               warm_start=True, x32_ptr=grid.p.data_ptr())
     out = dict(st)
     out.update({"refined": True, "fp32_iterations": st["iterations"], "fp32_relres_true": st["relres_true"],
                 "iterations": st["iterations"] + rs["inner_iterations"], "refine_steps": rs["outer"],
-                "converged": rs["converged"], "relres_true": rs["relres_true"]})
+                "refine_fallback": rs["fallback"], "converged": rs["converged"], "relres_true": rs["relres_true"]})
+    if not rs["converged"]:
+        out["capped"] = ("max_iter" if used + rs["inner_iterations"] >= max_iter else
+                         "refine_steps" if rs["outer"] >= MAX_REFINE_STEPS else False)
     return out
 
 

@@ -441,6 +441,52 @@ static inline float rhs_coef(float dx, float rho) {
     return b;
 }
 
+// ---------------------------------------------------------------------------
+// Diagnostic: div6 against the IEEE quotient x / 6.0f for all 2^32 float bit
+// patterns on this device (262144 workgroups x 256 threads x 64 patterns).
+// counts[0] += patterns where they differ (NaN = NaN), counts[1] += patterns
+// on the fma path, counts[2] += patterns where the one-term x * RN(1/6)
+// differs (the check's control).  Reads no memory but the counters.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kDiv6PerThread = 64;
+constexpr uint32_t kDiv6Groups = (uint32_t)((1ull << 32) / (256ull * kDiv6PerThread));
+
+__device__ __forceinline__ bool same_float(float a, float b) {
+    if (__builtin_isnan(a) && __builtin_isnan(b)) return true;
+    return __float_as_uint(a) == __float_as_uint(b);
+}
+
+__global__ __launch_bounds__(256) void div6_check_kernel(unsigned long long* counts) {
+    const float zh = 0x1.555556p-3f;
+    const uint32_t base = (blockIdx.x * 256u + threadIdx.x) * kDiv6PerThread;
+    unsigned c[3] = {0u, 0u, 0u};
+    for (uint32_t k = 0; k < kDiv6PerThread; ++k) {
+        const float x = __uint_as_float(base + k);
+        const float q = x / 6.0f;
+        const float ax = fabsf(x);
+        c[0] += same_float(div6(x), q) ? 0u : 1u;
+        c[1] += (ax >= 0x1p-96f && ax <= 0x1.fffffep127f) ? 1u : 0u;
+        c[2] += same_float(x * zh, q) ? 0u : 1u;
+    }
+    __shared__ unsigned red[3][4];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        unsigned v = c[q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const unsigned long long v = (unsigned long long)red[threadIdx.x][0] + red[threadIdx.x][1] +
+                                     red[threadIdx.x][2] + red[threadIdx.x][3];
+        if (v) atomicAdd(counts + threadIdx.x, v);
+    }
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
 }  // namespace bs
 
 using namespace bs;
@@ -542,4 +588,12 @@ extern "C" int bs_pressure_solve_launch(float* p, float* p_tmp, const float* div
                                         int n_iter, int mat_fluid, int mat_air) {
     return bs_jacobi_solve_launch(p, p_tmp, div_u, mat, nx, ny, nz, dx, dt, rho, n_iter, mat_fluid,
                                   mat_air, 0, nullptr);
+}
+
+// Device check of the exact division by 6 (no counterpart in the reference;
+// see div6_check_kernel).  counts: three device uint64s, zeroed here.
+extern "C" int bs_div6_check(unsigned long long* counts, hipStream_t stream) {
+    bs::zero_u64_kernel<<<1, 64, 0, stream>>>(counts, 3);
+    bs::div6_check_kernel<<<bs::kDiv6Groups, 256, 0, stream>>>(counts);
+    return (int)hipGetLastError();
 }

@@ -2045,24 +2045,35 @@ static int refine_residual(const PcgBuffers& B, const RefineBufs& R, const doubl
     return (int)hipStreamSynchronize(st);
 }
 
-}  // namespace bs
+// One inner step of Jacobi-PCG in the refinement, vectors r, p, Ap stored as
+// S: either a cold start on the scaled residual (cold), or the recursive
+// residual replaced by the true one with the search direction kept and
+// rescaled, so the Krylov process continues instead of restarting and the
+// refinement costs no extra iterations.  Runs until the recursive residual
+// has dropped by f or the iteration counter `it` (continued across steps)
+// reaches max_inner.
+template <typename S>
+static int refine_pcg_step(const Tiles& Ti, int vi, const PcgBuffers& B, const RefineBufs& R, bool cold, double f,
+                           int& it, int max_inner, hipStream_t st) {
+    S* r = reinterpret_cast<S*>(B.r);
+    S* p = reinterpret_cast<S*>(B.p);
+    if (cold) {
+        pcg_ctrl_init_kernel<<<1, 64, 0, st>>>(B.ctrl, f * f, max_inner, 2);
+        BS_VECS_LAUNCH(refine_inner_start_kernel, S, vi, B.nwg, st, Ti, B.code, R.rs, R.rc, r, p, B.partS);
+        pcg_setup_finalize<<<1, 256, 0, st>>>(B.partS, B.nwg, B.ctrl, 1);
+    } else {
+        BS_VECS_LAUNCH(refine_replace_kernel, S, vi, B.nwg, st, Ti, B.code, R.rs, R.rc, r, p, B.partB);
+        refine_resume_kernel<<<1, 64, 0, st>>>(B.ctrl, R.rc, it - 1, f * f);
+        BS_VECS_LAUNCH(pcg_direction_kernel, S, vi, B.nwg, st, Ti, B.code, r, p, B.partB, B.ctrl, it - 1);
+    }
+    BS_RETURN_IF_ERR(hipGetLastError());
+    const int rc = pcg_iterate_t<S>(Ti, vi, B, it, max_inner, st);
+    if (rc != hipSuccess) return rc;
+    it = g_poll.final->iter;
+    return hipSuccess;
+}
 
-// Launch of KERNEL<vi, S> for the storage type of the call (SR casts a float
-// buffer to that type).
-#define REFINE_S_LAUNCH(KERNEL, GRID, ...)                                                  \
-    do {                                                                                   \
-        if (storage == 0) {                                                                \
-            using SType = float;                                                           \
-            BS_VECS_LAUNCH(KERNEL, SType, vi, GRID, st, __VA_ARGS__);                      \
-        } else if (storage == 1) {                                                         \
-            using SType = _Float16;                                                        \
-            BS_VECS_LAUNCH(KERNEL, SType, vi, GRID, st, __VA_ARGS__);                      \
-        } else {                                                                           \
-            using SType = __bf16;                                                          \
-            BS_VECS_LAUNCH(KERNEL, SType, vi, GRID, st, __VA_ARGS__);                      \
-        }                                                                                  \
-    } while (0)
-#define SR(buf) reinterpret_cast<SType*>(buf)
+}  // namespace bs
 
 extern "C" size_t bs_refine_workspace_bytes(int nx, int ny, int nz) {
     if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
@@ -2113,26 +2124,12 @@ extern "C" int bs_refine_solve(double* x, float* x32, const float* div_u, const 
         if (bb == 0.0 || rel <= tol) { conv = 1; break; }
         if (outer >= max_outer || it >= max_inner) break;
         const double f = segment_target(inner_tol, tol, rel);
-        if (outer == 0) {
-            // Cold start of the inner PCG on the scaled residual.
-            pcg_ctrl_init_kernel<<<1, 64, 0, st>>>(B.ctrl, f * f, max_inner, 2);
-            REFINE_S_LAUNCH(refine_inner_start_kernel, B.nwg, Ti, B.code, rs, rc, SR(B.r), SR(B.p), B.partS);
-            pcg_setup_finalize<<<1, 256, 0, st>>>(B.partS, B.nwg, B.ctrl, 1);
-        } else {
-            // Replace the recursive residual by the true one and keep the
-            // search direction: the Krylov process continues instead of
-            // restarting, so the refinement costs no extra iterations.
-            REFINE_S_LAUNCH(refine_replace_kernel, B.nwg, Ti, B.code, rs, rc, SR(B.r), SR(B.p), B.partB);
-            refine_resume_kernel<<<1, 64, 0, st>>>(B.ctrl, rc, it - 1, f * f);
-            REFINE_S_LAUNCH(pcg_direction_kernel, B.nwg, Ti, B.code, SR(B.r), SR(B.p), B.partB, B.ctrl, it - 1);
-        }
-        BS_RETURN_IF_ERR(hipGetLastError());
+        const bool cold = outer == 0;
         ++outer;
-        if (storage == 0) rcode = pcg_iterate_t<float>(Ti, vi, B, it, max_inner, st);
-        else if (storage == 1) rcode = pcg_iterate_t<_Float16>(Ti, vi, B, it, max_inner, st);
-        else rcode = pcg_iterate_t<__bf16>(Ti, vi, B, it, max_inner, st);
+        if (storage == 0) rcode = refine_pcg_step<float>(Ti, vi, B, R, cold, f, it, max_inner, st);
+        else if (storage == 1) rcode = refine_pcg_step<_Float16>(Ti, vi, B, R, cold, f, it, max_inner, st);
+        else rcode = refine_pcg_step<__bf16>(Ti, vi, B, R, cold, f, it, max_inner, st);
         if (rcode != hipSuccess) return rcode;
-        it = g_poll.final->iter;
         inner_rel = g_poll.final->rr > 0.0 && h.n_fluid ? sqrt(g_poll.final->rr / (double)h.n_fluid) : 0.0;
         refine_accumulate_kernel<<<stream_grid(n), 256, 0, st>>>(n, rs, rc, x);
     }
@@ -2215,7 +2212,12 @@ extern "C" int bs_fcg_refine_solve(double* x, float* x32, const float* div_u, co
         rc = mg_setup(st, F.pws, nx, ny, nz, mat, mat_fluid, mat_air, B.code, &F.n_fluid);
         if (rc != hipSuccess) return rc;
     }
-    int outer = 0, its = 0, conv = 0;
+    // After an inner flexible CG stalls (fallback), the remaining steps run
+    // the plain Jacobi-PCG inner solve of bs_refine_solve (FP32 vectors), as
+    // bs_fcg_solve hands a stalled solve over to plain PCG.
+    const int vp = refine_vec(nz, 0);
+    const Tiles Tp = make_tiles(nx, ny, nz, vp);
+    int outer = 0, its = 0, conv = 0, fell_back = 0, pcg_it = 0, pcg_steps = 0;
     double rel = 0.0, bb = 0.0, inner_rel = 0.0;
     for (;;) {
         RefCtrl h;
@@ -2224,19 +2226,38 @@ extern "C" int bs_fcg_refine_solve(double* x, float* x32, const float* div_u, co
         bb = h.bb;
         rel = bb > 0.0 ? sqrt(h.rr / bb) : 0.0;
         if (bb == 0.0 || rel <= tol) { conv = 1; break; }
-        if (outer >= max_outer) break;
+        if (outer >= max_outer || its >= max_inner) break;
         ++outer;
         const double f = segment_target(inner_tol, tol, rel);
-        pcg_ctrl_init_kernel<<<1, 64, 0, st>>>(B.ctrl, f * f, max_inner, 1);
-        BS_VECS_LAUNCH(refine_inner_start_kernel, float, vec, B.nwg, st, T, B.code, R.rs, R.rc, B.r, B.p, B.partS);
-        pcg_setup_finalize<<<1, 256, 0, st>>>(B.partS, B.nwg, B.ctrl, 1);
-        fcg_prepare_kernel<<<1, 64, 0, st>>>(B.ctrl, 0);
-        rc = fcg_begin(T, vec, F, st);
-        if (rc != hipSuccess) return rc;
-        rc = fcg_iterate(T, vec, F, max_inner, st);
-        if (rc != hipSuccess) return rc;
-        its += g_poll.final->iter;
-        inner_rel = g_poll.final->bsq > 0.0 ? sqrt(g_poll.final->rr / g_poll.final->bsq) : 0.0;
+        if (!fell_back) {
+            pcg_ctrl_init_kernel<<<1, 64, 0, st>>>(B.ctrl, f * f, max_inner - its, 1);
+            BS_VECS_LAUNCH(refine_inner_start_kernel, float, vec, B.nwg, st, T, B.code, R.rs, R.rc, B.r, B.p,
+                           B.partS);
+            pcg_setup_finalize<<<1, 256, 0, st>>>(B.partS, B.nwg, B.ctrl, 1);
+            fcg_prepare_kernel<<<1, 64, 0, st>>>(B.ctrl, 0);
+            rc = fcg_begin(T, vec, F, st);
+            if (rc != hipSuccess) return rc;
+            rc = fcg_iterate(T, vec, F, max_inner - its, st);
+            if (rc != hipSuccess) return rc;
+            const PcgCtrl& g = *g_poll.final;
+            its += g.iter;
+            inner_rel = g.bsq > 0.0 ? sqrt(g.rr / g.bsq) : 0.0;
+            if (g.fallback && !g.converged) {
+                fell_back = 1;
+                B.nwg = pick_nwg(Tp, kMaxWg);
+                B.partB = B.partA + B.nwg;
+                B.partS = B.partA + 3 * B.nwg;
+            }
+        } else {
+            // The plain PCG's own counter continues across its steps; its cap
+            // is what the total budget leaves.
+            const int before = pcg_it;
+            rc = refine_pcg_step<float>(Tp, vp, B, R, pcg_steps == 0, f, pcg_it, pcg_it + (max_inner - its), st);
+            if (rc != hipSuccess) return rc;
+            ++pcg_steps;
+            its += pcg_it - before;
+            inner_rel = g_poll.final->rr > 0.0 && h.n_fluid ? sqrt(g_poll.final->rr / (double)h.n_fluid) : 0.0;
+        }
         refine_accumulate_kernel<<<stream_grid((int64_t)n), 256, 0, st>>>((int64_t)n, R.rs, R.rc, x);
     }
     if (x32) refine_round_kernel<<<stream_grid((int64_t)n), 256, 0, st>>>((int64_t)n, x, x32);
@@ -2250,6 +2271,8 @@ extern "C" int bs_fcg_refine_solve(double* x, float* x32, const float* div_u, co
         stats->relres_true = rel;
         stats->bnorm = sqrt(bb);
         stats->relres_inner_last = inner_rel;
+        stats->fallback = fell_back;
+        stats->pcg_iterations = pcg_it;
     }
     return hipSuccess;
 }

@@ -46,4 +46,4 @@ extern "C" int bs_device_open(int device_id, char* buf, int buflen) {
 
 extern "C" int bs_device_synchronize(void) { return (int)hipDeviceSynchronize(); }
 
-extern "C" int bs_abi_version(void) { return 1; }
+extern "C" int bs_abi_version(void) { return 2; }

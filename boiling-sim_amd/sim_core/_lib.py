@@ -66,6 +66,8 @@ class RefineStats(Structure):
         ("relres_true", c_double),
         ("bnorm", c_double),
         ("relres_inner_last", c_double),
+        ("fallback", c_int),
+        ("pcg_iterations", c_int),
     ]
 
     def as_dict(self) -> dict:
@@ -88,6 +90,7 @@ SIGNATURES = {
     "bs_jacobi_pressure_launch_stream": [VP, VP, VP, VP, I, I, I, F, F, F, I, I, S],
     "bs_pressure_solve_launch": [VP, VP, VP, VP, I, I, I, F, F, F, I, I, I],
     "bs_jacobi_solve_launch": [VP, VP, VP, VP, I, I, I, F, F, F, I, I, I, I, S],
+    "bs_div6_check": [VP, S],
     "bs_divergence_launch": [VP, VP, VP, VP, VP, I, I, I, F, I, S],
     "bs_subtract_gradient_launch": [VP, VP, VP, VP, VP, I, I, I, F, F, F, I, S],
     "bs_max_abs_launch": [VP, c_longlong, VP, S],

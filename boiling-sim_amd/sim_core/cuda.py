@@ -383,11 +383,13 @@ class SimCore:
 
     def fcg_refine_solve(self, x64_ptr, div_u_ptr, mat_ptr, nx, ny, nz, dx, dt, rho, tol, precond,
                          unet_params_ptr, unet_packed_ptr, workspace_ptr, workspace_bytes, inner_tol=1e-3,
-                         max_outer=20, max_inner=2000, warm_start=False, x32_ptr=0, mat_fluid=0, mat_air=2,
+                         max_outer=20, max_inner=20000, warm_start=False, x32_ptr=0, mat_fluid=0, mat_air=2,
                          stream=0) -> dict:
         """FP64 x solving A x = b to a true relative residual ``tol`` by
         iterative refinement around the learned flexible CG (``fcg_solve``'s
-        preconditioner kinds)."""
+        preconditioner kinds), at most ``max_inner`` inner iterations in all;
+        a stalled inner flexible CG hands the remaining steps to plain
+        Jacobi-PCG (``fallback``, ``pcg_iterations`` in the stats)."""
         st = _lib.RefineStats()
         _lib.call(
             "bs_fcg_refine_solve",

@@ -65,6 +65,14 @@ int bs_jacobi_solve_launch(float* p, float* p_tmp, const float* div_u, const int
                            int ny, int nz, float dx, float dt, float rho, int n_iter, int mat_fluid,
                            int mat_air, int zero_init, bs_stream_t stream);
 
+/* Diagnostic (no counterpart in the reference): the Jacobi kernels' exact
+ * division by 6, fma(x, RN(1/6), x * RN(1/6 - RN(1/6))) for finite
+ * |x| >= 2^-96 and the IEEE divide otherwise, against x / 6.0f for all 2^32
+ * float bit patterns on the device.  counts (three device uint64s) receives
+ * the patterns where they differ, the patterns on the fma path, and the
+ * patterns where the one-term x * RN(1/6) differs.  Asynchronous. */
+int bs_div6_check(unsigned long long* counts, bs_stream_t stream);
+
 /* ---- projection kernels (the Warp side of pressure_projection) ------------ */
 
 /* compute_divergence (fluid.py:201-219). */
@@ -322,6 +330,8 @@ typedef struct {
     double relres_true;     /* ||b - A x||_2 / ||b||_2 of the FP64 iterate */
     double bnorm;           /* ||b||_2 */
     double relres_inner_last; /* recursive relative residual of the last inner solve */
+    int fallback;           /* bs_fcg_refine_solve: an inner flexible CG stalled and plain PCG took over */
+    int pcg_iterations;     /* bs_fcg_refine_solve: inner plain-PCG iterations after that */
 } bs_refine_stats;
 
 size_t bs_refine_workspace_bytes(int nx, int ny, int nz);
@@ -338,8 +348,11 @@ int bs_refine_solve(double* x, float* x32, const float* div_u, const int* mat, i
 size_t bs_fcg_refine_workspace_bytes(int nx, int ny, int nz, int precond);
 /* The same refinement with bs_fcg_solve's flexible CG (preconditioner
  * `precond`, FP32 vectors) as the inner solver, cold-started on each scaled
- * residual and run until its recursive residual drops by inner_tol (at most
- * max_inner iterations per step). */
+ * residual and run until its recursive residual drops by inner_tol; at most
+ * max_inner inner iterations in all, as in bs_refine_solve.  When an inner
+ * flexible CG stalls (bs_fcg_solve's test), the remaining steps run
+ * bs_refine_solve's plain Jacobi-PCG inner solve (FP32 vectors) instead, as
+ * bs_fcg_solve hands a stalled solve over to plain PCG. */
 int bs_fcg_refine_solve(double* x, float* x32, const float* div_u, const int* mat, int nx, int ny, int nz,
                         float dx, float dt, float rho, int mat_fluid, int mat_air, double tol, int max_outer,
                         double inner_tol, int max_inner, int warm_start, int precond, const float* unet_params,

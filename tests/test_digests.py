@@ -31,3 +31,23 @@ def test_config1_oracle_digests():
     assert c1.pop("max_div_after") == GOLD["config1"]["max_div_after"]
     for k, v in c1.items():
         assert make_digests.sha(v) == GOLD["config1"]["sha256"][k], k
+
+
+@pytest.mark.parametrize("name", ["dev", "prod"])
+def test_config2_oracle_digests(name):
+    """Config 2 at 110x110x70 and 440x440x280: the seed-7 velocities, the
+    oracle's divergence and its 200 sweeps (p, p_tmp) still hash to the
+    committed digests the GPU suite checks the device against."""
+    c2 = make_digests.config2(name)
+    for k, v in c2.items():
+        assert make_digests.sha(v) == GOLD["config2"][name]["sha256"][k], k
+
+
+def test_reference_gate_oracle_digests():
+    """The reference's 200-step gate case (single_carrot.yaml at dx = 4 mm,
+    test_pressure_parity.py:159-224): mask, velocities and the oracle's
+    projection still hash to the committed digests."""
+    g = make_digests.reference_gate()
+    assert g["max_div_after"] == GOLD["reference_gate"]["max_div_after"]
+    for k, v in g["sha256"].items():
+        assert v == GOLD["reference_gate"]["sha256"][k], k

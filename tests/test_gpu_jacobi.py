@@ -239,3 +239,56 @@ def test_config1_matches_committed_digests(simcore, torch_gpu):
         assert sha(host(t)) == gold["sha256"][k], k
     assert sha(host(div)) == gold["sha256"]["div_after"]
     assert float(host(mb).view(np.float32)[0]) == gold["max_div_after"]
+
+
+@pytest.mark.parametrize("name", ["dev", "prod"])
+def test_config2_matches_committed_digests(simcore, torch_gpu, name):
+    """Config 2 on the device at 110x110x70 and 440x440x280 (seed-7 velocities
+    with the plume): the divergence and 200 sweeps (p and p_tmp, the
+    two-sweeps-per-pass path) hash to the committed digests of the oracle's
+    outputs, bit for bit, at the full production size."""
+    import hashlib
+    import json
+    import os
+
+    torch = torch_gpu
+    from boilsim.fixtures import CONFIGS, DT, RHO, pot_mask, seeded_velocities
+
+    gold = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                       "config_digests.json")))["config2"][name]
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).view(np.uint8).tobytes()).hexdigest()
+    cfg = CONFIGS[name]
+    print(f"[config2] {name} {cfg.shape}", flush=True)
+    mat = pot_mask(*cfg.shape)
+    ux, uy, uz = seeded_velocities(*cfg.shape, seed=gold["seed"], plume=True)
+    assert sha(uz) == gold["sha256"]["uz"]
+    d = [dev(torch, a) for a in (ux, uy, uz)]
+    del ux, uy, uz
+    md = dev(torch, mat)
+    div = torch.empty(cfg.shape, device="cuda")
+    simcore.compute_divergence(div.data_ptr(), *(t.data_ptr() for t in d), md.data_ptr(), *cfg.shape, cfg.dx)
+    p = torch.zeros(cfg.shape, device="cuda")
+    pt = torch.zeros(cfg.shape, device="cuda")
+    simcore.pressure_solve(p.data_ptr(), pt.data_ptr(), div.data_ptr(), md.data_ptr(), *cfg.shape, cfg.dx, DT, RHO,
+                           gold["sweeps"])
+    torch.cuda.synchronize()
+    assert sha(host(div)) == gold["sha256"]["div"]
+    assert sha(host(p)) == gold["sha256"]["p"]
+    assert sha(host(pt)) == gold["sha256"]["p_tmp"]
+
+
+def test_div6_exhaustive_on_device(simcore, torch_gpu):
+    """The exact division by 6 of the Jacobi kernels equals the device's IEEE
+    x / 6.0f for all 2^32 float bit patterns (bs_div6_check); the one-term
+    control x * RN(1/6) differs somewhere, so the check can fail."""
+    from sim_core import _lib
+
+    torch = torch_gpu
+    counts = torch.full((3,), 7, dtype=torch.int64, device="cuda")
+    _lib.call("bs_div6_check", counts.data_ptr(), None)
+    torch.cuda.synchronize()
+    mism, fma_path, one_term = (int(v) for v in host(counts))
+    print(f"div6 on device: {mism} mismatches, {fma_path} on the fma path, one-term form differs on {one_term}")
+    assert mism == 0
+    assert fma_path == 7 * 2**29
+    assert one_term > 0

@@ -58,14 +58,14 @@ def test_learned_fcg_reaches_true_tolerance(simcore, torch_gpu, name):
 
 
 def test_network_gain_at_production_grid(simcore, torch_gpu):
-    """At 440x440x280 the U-Net coarse solve cuts the V-cycle's iterations and
-    time against Jacobi sweeps on the coarse grid (the same method without
-    the network), and both beat plain PCG."""
+    """At 440x440x280 the U-Net coarse solve cuts the V-cycle's iterations
+    against a Jacobi sweep + Chebyshev polish on the coarse grid (the same
+    method without the network); the learned solve and plain PCG both reach
+    a true 1e-6 by the oracle's FP64 residual.  (Timings are bench.py's.)"""
     torch = torch_gpu
-    import time
 
-    from boilsim.fixtures import CONFIGS, pot_mask, seeded_velocities
-    from boilsim.unet import UNet
+    from boilsim.fixtures import CONFIGS, DT, RHO, pot_mask, seeded_velocities
+    from boilsim.learned import default_net
 
     cfg = CONFIGS["prod"]
     mat = pot_mask(*cfg.shape)
@@ -75,27 +75,28 @@ def test_network_gain_at_production_grid(simcore, torch_gpu):
     gux, guy, guz = dev(torch, ux), dev(torch, uy), dev(torch, uz)
     simcore.compute_divergence(div_d.data_ptr(), gux.data_ptr(), guy.data_ptr(), guz.data_ptr(), mat_d.data_ptr(),
                                *cfg.shape, cfg.dx)
-    net = UNet.load()
+    torch.cuda.synchronize()
+    b = oracle.rhs_f64(host(div_d), mat, cfg.dx, DT, RHO)
+    net = default_net()
     res = {}
     for k in (2, 3):
-        _fcg(simcore, torch, net, cfg, mat_d, div_d, k)  # warm
-        t = time.perf_counter()
-        _, st = _fcg(simcore, torch, net, cfg, mat_d, div_d, k)
-        res[k] = (st, time.perf_counter() - t)
+        print(f"[440] flexible CG kind {k}", flush=True)
+        x, st = _fcg(simcore, torch, net, cfg, mat_d, div_d, k)
+        rel2, _ = oracle.residual_f64(x, b, mat)
+        print(f"  {st}; oracle relres {rel2:.3e}", flush=True)
+        assert st["converged"] == 1 and st["fallback"] == 0 and rel2 <= 1.05e-6
+        res[k] = st
+    print("[440] plain PCG", flush=True)
     wsp = torch.empty(simcore.pcg_workspace_bytes(*cfg.shape), dtype=torch.uint8, device="cuda")
     x = torch.zeros(cfg.shape, dtype=torch.float32, device="cuda")
-    t = time.perf_counter()
-    sp = simcore.pcg_solve(x.data_ptr(), div_d.data_ptr(), mat_d.data_ptr(), *cfg.shape, cfg.dx, 1e-3, 958.0, 1e-6,
+    sp = simcore.pcg_solve(x.data_ptr(), div_d.data_ptr(), mat_d.data_ptr(), *cfg.shape, cfg.dx, DT, RHO, 1e-6,
                            20000, wsp.data_ptr(), wsp.numel())
     torch.cuda.synchronize()
-    tp = time.perf_counter() - t
-    print(f"plain PCG {sp['iterations']} its {tp * 1e3:.1f} ms; V-cycle+U-Net {res[2][0]['iterations']} its "
-          f"{res[2][1] * 1e3:.1f} ms; V-cycle+Jacobi {res[3][0]['iterations']} its {res[3][1] * 1e3:.1f} ms")
-    for k in (2, 3):
-        assert res[k][0]["converged"] == 1 and res[k][0]["fallback"] == 0
-    assert res[2][0]["iterations"] < 0.6 * res[3][0]["iterations"]
-    assert res[2][1] < res[3][1] < tp
-    assert res[2][1] * 3 < tp
+    rel2, _ = oracle.residual_f64(host(x), b, mat)
+    print(f"  {sp}; oracle relres {rel2:.3e}")
+    assert sp["converged"] == 1 and rel2 <= 1.05e-6
+    assert res[2]["iterations"] < 0.6 * res[3]["iterations"]
+    assert res[2]["iterations"] * 20 < sp["iterations"]
 
 
 def test_training_reduces_loss(simcore, torch_gpu):
