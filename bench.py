@@ -4,22 +4,25 @@ Workload (the headline metric): one pressure projection on the 440x440x280
 pot (dx = 0.5 mm, ~54.2 M cells, FP32 fields), RHS the divergence of seeded
 N(0, 0.01) face velocities (seed 11, no plume), solved to a TRUE relative
 residual ||b - A x||_2/||b||_2 <= 1e-6 (FP64 check), inputs resident in HBM:
-  * learned-PCG: flexible CG preconditioned by a damped-Jacobi V-cycle whose
-    4x-coarsened grid is solved by the trained U-Net, its correction polished
-    by Chebyshev steps  -> `value`
+  * learned-PCG: flexible CG preconditioned by a V-cycle whose 4x-coarsened
+    grid is solved by the trained U-Net, its correction polished by
+    Chebyshev steps  -> `value`
   * plain PCG: Jacobi-preconditioned CG, FP32 vectors / FP64 scalars
-  * (ablation) the same V-cycle with one Jacobi sweep from zero instead of
-    the network, followed by the same Chebyshev polish
+  * (ablations) the same V-cycle with non-learned coarse solves
 ms/projection is the whole projection: divergence -> solve -> gradient
 subtraction -> divergence -> on-device max|div|.  Each step restores the
 input velocities (untimed) so every projection solves the same system.
 
-Also measured live: the roofline of the mask-free Jacobi sweep and of the
-PCG iteration kernels (HIP events on the launch stream), the first frames of
-the warm-started sequence of correlated smooth right-hand sides (AR(1)
-Gaussian random fields plus a drifting plume, FP64 refinement around the
-learned flexible CG vs around Jacobi-PCG), and the CPU baseline (the
-oracle's PCG on a bounded sample on the host cores).
+Also measured live, with HIP events on the stream each kernel runs on:
+`roofline` for the learned solve's dominant kernel (fcg_spmv_dir_kernel) and
+its two other fine-level passes (timed inside learned solves run with pass
+timing on), the Jacobi sweep's two-sweep kernel, the PCG iteration, the
+200-sweep Jacobi projection of configs 1-2 at all three grids, conduction,
+the face passes and advection; the warm-started sequence of correlated
+smooth right-hand sides (AR(1) Gaussian random fields plus a drifting plume,
+FP64 refinement around the learned flexible CG vs around Jacobi-PCG); and
+the CPU baseline: the oracle's plain PCG projection at 440x440x280, solved in
+full on the host cores, and its 200-sweep Jacobi projections.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -42,22 +45,36 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth, GB/s
 METRIC = "ms/projection to rel-res 1e-6 at 440x440x280: learned-PCG vs plain PCG"
 
-# Algorithmic bytes per cell (FP32 fields, one-byte class code where used).
-B_JACOBI = 12.0   # p_in, div, p_out (class carried in p, no mask read)
-B_PCG_ITER = 47.0  # spmv 4+1+4, update 4+4+(4+4)+(4+4)+1, direction 4+1+(4+4)
-# Fine-level algorithmic bytes per cell of one learned flexible-CG iteration:
-# direction + spmv + deferred x step 25 (code, z, p_old, x; p, Ap, x),
-# residual update + pre-smoothing + residual + restriction 17.5 (code, r_old,
-# Ap; r, z, rc/8 x 4), prolongation + post-sweep + r.z 13.5 (code, z_in,
-# coarse e/8 x 4, r; z).
-B_LEARNED_FINE = 56.0
-# HBM bytes per cell one two-sweep launch of jacobi_pair_kernel actually
-# moves at 440x440x280, measured with rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE (one counter per run) and calibrated by kernels of known traffic
-# in the same runs (a one-field streaming read counts 0.5 of its bytes on
-# FETCH_SIZE; a one-field fill writes exactly its bytes on WRITE_SIZE).
-# 266,018.7 KiB x 1.99976 read + 220,094.4 KiB written per launch.
+# Algorithmic bytes per cell: FP32 fields 4 B; m is the class bytes per cell
+# a kernel reads (the reference's int32 mask: m = 4; this build's one-byte
+# class code: m = 1).
+# Jacobi sweep, 12 + m: p_in, div read, p_out written.  Inside the fused
+# solve the class rides in p itself (NaN sentinels), so the middle sweeps
+# read no mask at all (m = 0, 12 B/cell); B_JACOBI_REF is the reference's
+# layout (m = 4).
+B_JACOBI = 12.0
+B_JACOBI_REF = 16.0
+# PCG iteration, fused-minimum convention 40 + 2m with m = 1.
+B_PCG_ITER = 42.0
+# The learned flexible-CG iteration's three fine-level passes:
+#   fcg_spmv_dir_kernel: code, z, p_old, x read; p, Ap, x written = 25
+#   pre2_restrict_kernel<true>: code, r_old, Ap read; r, z written, the
+#       restricted residual (1/8 of a field) written = 17.5
+#   prolong_post_kernel: code, z_in, r, the coarse correction (1/8) read;
+#       z written = 13.5
+PASS_BYTES = {"fcg_spmv_dir_kernel": 25.0, "pre2_restrict_kernel<true>": 17.5, "prolong_post_kernel": 13.5}
+B_LEARNED_FINE = sum(PASS_BYTES.values())  # 56
+# HBM bytes per cell one launch actually moves at 440x440x280, measured with
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (one counter per run) and
+# calibrated by kernels of known traffic in the same runs (a one-field
+# streaming read counts 0.5 of its bytes on FETCH_SIZE; a one-field fill
+# writes exactly its bytes on WRITE_SIZE).  jacobi_pair_kernel: 266,018.7 KiB
+# x 1.99976 read + 220,094.4 KiB written per launch.
 TRAFFIC_JACOBI_PAIR = 14.207
+# The learned iteration's fine-level passes, measured the same way over
+# `bench.py --legs passes` (reads = FETCH_SIZE x 1.99965 from max_abs_kernel,
+# writes = WRITE_SIZE; dispatches that returned at once left out).
+TRAFFIC_PASSES = {"fcg_spmv_dir_kernel": 28.27, "pre2_restrict_kernel<true>": 23.36, "prolong_post_kernel": 16.48}
 
 
 def log(msg: str) -> None:
@@ -73,12 +90,15 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-ablation", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=12, help="oracle PCG iterations in the CPU sample")
-    ap.add_argument("--legs", default="learned,pcg,ablation,jacobi,mixed,thermal,faces,advection,sequence,cpu",
+    ap.add_argument("--legs", default="learned,passes,pcg,ablation,jacobi,jacobi_configs,mixed,thermal,faces,"
+                                      "advection,sequence,cpu",
                     help="comma list of the legs to run (the JSON fields of skipped legs are null)")
-    ap.add_argument("--seq-frames", type=int, default=6,
-                    help="frames of the warm-started AR(1) sequence leg (0: skip; the full run is "
-                         "python -m boilsim.sequence --frames 100)")
+    ap.add_argument("--seq-frames", type=int, default=100,
+                    help="frames of the warm-started AR(1) sequence leg (0: skip)")
+    ap.add_argument("--seq-methods", default="learned,pcg-fp32",
+                    help="methods of the sequence leg (learned, pcg-fp32, pcg-fp16)")
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="host threads of the CPU baseline at most (the GPU's share of the host's cores)")
     return ap.parse_args()
 
 
@@ -294,6 +314,167 @@ def advection_leg(core, torch, gmat, mat, ux0, uy0, uz0, cfg, nx, ny, nz, n, rep
     return res
 
 
+def pass_leg(core, torch, projection, cal_src, n, reps=3):
+    """The learned solve's fine-level passes timed inside `reps` learned
+    projections run with pass timing on (bs_fcg_pass_timing: the solves
+    launch eagerly and bracket fcg_spmv_dir_kernel, pre2_restrict_kernel<true>
+    and prolong_post_kernel with HIP events on the solve's stream).  A plain
+    device copy of one field runs first: its FETCH_SIZE / WRITE_SIZE counters
+    calibrate a counter run over this leg (bench.py --legs passes)."""
+    cal = torch.empty_like(cal_src)
+    cal.copy_(cal_src)  # n floats read, n floats written
+    torch.cuda.synchronize()
+    core.fcg_pass_timing_start()
+    try:
+        for _ in range(reps):
+            projection("learned")
+        torch.cuda.synchronize()
+    finally:
+        res = core.fcg_pass_timing_stop()
+    out = {}
+    for k, v in res.items():
+        b = PASS_BYTES[k]
+        gbs = b * n / (v["us_mean"] * 1e-6) / 1e9 if v["us_mean"] > 0 else float("nan")
+        tr = TRAFFIC_PASSES.get(k)
+        out[k] = {"us_median": v["us_median"], "us_mean": v["us_mean"], "launches": v["launches"],
+                  "bytes_per_cell": b, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS,
+                  "traffic_bytes_per_cell": tr, "traffic_over_algorithmic": (tr / b) if tr else None}
+        log(f"pass {k}: {v['us_mean']:.1f} us mean ({v['us_median']:.1f} median, {v['launches']} launches), "
+            f"{gbs:.0f} GB/s algorithmic at {b} B/cell ({gbs / HBM_PEAK_GBS:.2f} of peak)")
+    return out
+
+
+def jacobi_configs_leg(core, torch, reps=10, warm=3):
+    """Configs 1-2: ms per 200-sweep Jacobi projection (bs_projection_jacobi:
+    zero p, divergence, 200 sweeps, gradient, divergence + max|div|) of the
+    seed-7 velocities with the plume, at 64x64x48, 110x110x70 and
+    440x440x280, and at 110x110x68 (the grid of the reference's published
+    3.86 / 5.86 ms on an RTX 6000 Ada, analytic pot mask), HIP events on the
+    null stream (the launcher's), median of `reps` after `warm`."""
+    from boilsim.fixtures import CONFIGS, DT, RHO, pot_mask, seeded_velocities
+
+    out = {}
+    grids = [(CONFIGS[k].shape, CONFIGS[k].dx) for k in ("small", "dev", "prod")] + [((110, 110, 68), 2e-3)]
+    for shape, dx in grids:
+        mat = pot_mask(*shape)
+        u0 = [torch.from_numpy(a).cuda() for a in seeded_velocities(*shape, seed=7, plume=True)]
+        u = [a.clone() for a in u0]
+        gm = torch.from_numpy(mat).cuda()
+        p, pt, dv = (torch.empty(shape, dtype=torch.float32, device="cuda") for _ in range(3))
+        mb = torch.zeros(2, dtype=torch.int32, device="cuda")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ts = []
+        for r in range(warm + reps):
+            for a, b in zip(u, u0):
+                a.copy_(b)
+            torch.cuda.synchronize()
+            e0.record()
+            core.projection_jacobi(p.data_ptr(), pt.data_ptr(), dv.data_ptr(), *(a.data_ptr() for a in u),
+                                   gm.data_ptr(), *shape, dx, DT, RHO, 200, mb.data_ptr())
+            e1.record()
+            torch.cuda.synchronize()
+            if r >= warm:
+                ts.append(e0.elapsed_time(e1))
+        key = "x".join(map(str, shape))
+        out[key] = float(np.median(ts))
+        log(f"jacobi projection {key}, 200 sweeps: {out[key]:.3f} ms (median of {reps})")
+        del u0, u, gm, p, pt, dv
+    out["reference_published_ms_110x110x68"] = {"hand_written_cuda": 3.86, "warp": 5.86,
+                                                "hardware": "RTX 6000 Ada (960 GB/s)"}
+    return out
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_leg(core, torch, mat, vel11, cfg, max_threads, d):
+    """The CPU baseline: the oracle (this repository's C restatement of the
+    reference arithmetic, -O3 -ffp-contract=off, OpenMP) on the host cores,
+    reported beside the GPU, never a target.
+      * one full plain-PCG projection at 440x440x280 (divergence, Jacobi-PCG
+        to a true relative residual 1e-6 with residual replacement -- the
+        semantics of the GPU's plain PCG, bs_pcg_solve --, gradient,
+        divergence + max|div|), timed whole;
+      * the 200-sweep Jacobi projection of configs 1-2 at the three grids;
+      * one thread: 3 PCG iterations at 440 and the 64x64x48 projection;
+      * the shipped Jacobi path against the oracle on config 1 (bit parity)."""
+    import oracle
+    from boilsim.fixtures import CONFIGS, DT, RHO, pot_mask, seeded_velocities
+
+    aff = len(os.sched_getaffinity(0))
+    threads = max(1, min(aff, max_threads))
+    oracle.set_threads(threads)
+    model = cpu_model()
+    ux, uy, uz = (a.copy() for a in vel11)
+    t0 = time.perf_counter()
+    div = oracle.divergence(ux, uy, uz, mat, cfg.dx)
+    x, it, restarts, rel = oracle.pcg_true(div, mat, cfg.dx, DT, RHO, 1e-6, 20000)
+    oracle.gradient(ux, uy, uz, x, mat, cfg.dx, DT, RHO)
+    div2 = oracle.divergence(ux, uy, uz, mat, cfg.dx)
+    maxdiv = float(np.abs(div2).max())
+    t_full = time.perf_counter() - t0
+    log(f"cpu: full plain-PCG projection at {cfg.shape}: {t_full:.1f} s, {it} iterations, true relres {rel:.3e}, "
+        f"{threads} threads ({model})")
+    del ux, uy, uz, div, div2, x
+    jac = {}
+    for name in ("small", "dev", "prod"):
+        c = CONFIGS[name]
+        m = pot_mask(*c.shape)
+        v = seeded_velocities(*c.shape, seed=7, plume=True)
+        t0 = time.perf_counter()
+        oracle.projection(*v, m, c.dx, DT, RHO, 200)
+        jac["x".join(map(str, c.shape))] = (time.perf_counter() - t0) * 1e3
+        log(f"cpu: 200-sweep Jacobi projection {c.shape}: {jac['x'.join(map(str, c.shape))]:.1f} ms")
+        del v, m
+    # one thread
+    oracle.set_threads(1)
+    div_np = oracle.divergence(*vel11, mat, cfg.dx)
+    t0 = time.perf_counter()
+    _, it1, _ = oracle.pcg_ref(div_np, mat, cfg.dx, DT, RHO, 1e-30, 3)
+    ms_it1 = (time.perf_counter() - t0) * 1e3 / max(it1, 1)
+    c1 = CONFIGS["small"]
+    m1 = pot_mask(*c1.shape)
+    v1 = seeded_velocities(*c1.shape, seed=7, plume=True)
+    t0 = time.perf_counter()
+    oracle.projection(*[a.copy() for a in v1], m1, c1.dx, DT, RHO, 200)
+    ms_j1 = (time.perf_counter() - t0) * 1e3
+    oracle.set_threads(threads)
+    log(f"cpu: one thread: {ms_it1:.0f} ms per PCG iteration at 440 (3 sampled, set-up included), "
+        f"64x64x48 Jacobi projection {ms_j1:.1f} ms")
+    # parity of the shipped Jacobi path with the oracle on config 1
+    d1 = oracle.divergence(*v1, m1, c1.dx)
+    pr, tr = np.zeros(c1.shape, np.float32), np.zeros(c1.shape, np.float32)
+    oracle.pressure_solve(pr, tr, d1, m1, c1.dx, DT, RHO, 200)
+    pg, tg = torch.zeros(c1.shape, device="cuda"), torch.zeros(c1.shape, device="cuda")
+    dg, mg = d(d1), d(m1)
+    core.pressure_solve(pg.data_ptr(), tg.data_ptr(), dg.data_ptr(), mg.data_ptr(), *c1.shape, c1.dx, DT, RHO, 200)
+    torch.cuda.synchronize()
+    got = pg.cpu().numpy()
+    fl = m1 == 0
+    relp = np.abs(got - pr)[fl] / np.maximum(np.abs(pr[fl]), 1e-6)
+    parity = {"case": "64x64x48 pot, seed-7 velocities + plume, 200 Jacobi sweeps from 0",
+              "bits_equal": bool(np.array_equal(got.view(np.uint32), pr.view(np.uint32))),
+              "max_rel_fluid": float(relp.max()), "gate": 1e-4}
+    log(f"parity (Jacobi, 200 sweeps, 64x64x48): bits equal {parity['bits_equal']}, max rel {parity['max_rel_fluid']:.1e}")
+    return {"value": t_full * 1e3, "unit": "ms/projection", "cores": threads, "kind": "port",
+            "sample": f"one full plain-PCG projection at {cfg.shape[0]}x{cfg.shape[1]}x{cfg.shape[2]} (full solve, "
+                      f"not a sample): divergence, the oracle's Jacobi-PCG to a true relative residual 1e-6 ({it} "
+                      f"iterations, {restarts} residual replacement(s), true relres {rel:.2e}), gradient, "
+                      f"divergence + max|div|, on {threads} threads",
+            "cpu_model": model, "host_cores_in_affinity": aff, "iterations": it, "relres_true": rel,
+            "max_div_after": maxdiv, "jacobi_ms_per_projection_200_sweeps": jac,
+            "single_thread": {"ms_per_pcg_iteration_440": ms_it1, "jacobi_projection_64x64x48_ms": ms_j1},
+            "parity": parity}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -303,8 +484,7 @@ def main():
 
     import sim_core
     from boilsim.fixtures import CONFIGS, DT, RHO, pot_mask, seeded_velocities
-    from boilsim.learned import PRECOND_MG_JACOBI, PRECOND_MG_UNET
-    from boilsim.unet import UNet
+    from boilsim.learned import PRECOND_MG_JACOBI, PRECOND_MG_UNET, default_net
 
     if sim_core.cuda is None:
         raise RuntimeError("libboilsim_hip.so not loadable: run __graft_entry__.build() first")
@@ -323,7 +503,7 @@ def main():
     p = torch.zeros(cfg.shape, dtype=torch.float32, device="cuda")
     div = torch.zeros_like(p)
     bits = torch.zeros(2, dtype=torch.int32, device="cuda")  # max|div|, max|u| as float bits
-    net = UNet.load()
+    net = default_net()
     ws_pcg = torch.empty(core.pcg_workspace_bytes(nx, ny, nz), dtype=torch.uint8, device="cuda")
     ws_f = {k: torch.empty(core.fcg_workspace_bytes(nx, ny, nz, k), dtype=torch.uint8, device="cuda")
             for k in (PRECOND_MG_UNET, PRECOND_MG_JACOBI)}
@@ -373,6 +553,12 @@ def main():
             f"max|div| after {maxdiv:.3e}")
         return ms, times, stats, maxdiv
 
+    def fresh_projection(solver):
+        gux.copy_(ux0)
+        guy.copy_(uy0)
+        guz.copy_(uz0)
+        return projection(solver)
+
     # Initial divergence (for the report).
     core.compute_divergence(ptr(div), ptr(gux), ptr(guy), ptr(guz), ptr(gmat), nx, ny, nz, cfg.dx)
     core.max_abs(ptr(div), n, ptr(bits))
@@ -394,20 +580,23 @@ def main():
         ms_a, _, st_a, _ = leg("mg-jacobi")
         abl = {"ms_per_projection": ms_a, "iterations": st_a["iterations"], "relres_true": st_a["relres_true"]}
 
+    gdiv = torch.empty_like(p)  # the initial divergence (the solves' RHS)
+    core.compute_divergence(ptr(gdiv), ptr(ux0), ptr(uy0), ptr(uz0), ptr(gmat), nx, ny, nz, cfg.dx)
+    passes = pass_leg(core, torch, fresh_projection, gdiv, n) if "passes" in legs else None
+
     # Roofline of the Jacobi sweep: 200 mask-free sweeps timed with HIP events
     # on the null stream, the stream the launcher uses.  A plain device copy of
     # one field (n floats) runs first: its FETCH_SIZE / WRITE_SIZE counters
     # calibrate the traffic measured for the sweep kernel.
     us_sweep = us_pair = jac_gbs = nan
-    gdiv = torch.empty_like(p)  # the initial divergence (the solves' RHS)
-    core.compute_divergence(ptr(gdiv), ptr(ux0), ptr(uy0), ptr(uz0), ptr(gmat), nx, ny, nz, cfg.dx)
     if "jacobi" in legs:
         cal = torch.empty_like(p)
         cal.copy_(gdiv)  # n floats read, n floats written
         torch.cuda.synchronize()
         us_pair, us_sweep = jacobi_roofline(core, sim_core, torch, p, gdiv, ux0, uy0, uz0, gmat, cfg, nx, ny, nz, n)
         jac_gbs = 2 * B_JACOBI * n / (us_pair * 1e-6) / 1e9
-    us_iter = ms_p * 1e3 / max(st_p["iterations"], 1) if st_p else nan
+    jcfg = jacobi_configs_leg(core, torch) if "jacobi_configs" in legs else None
+    us_iter = st_p["solve_ms_median"] * 1e3 / max(st_p["iterations"], 1) if st_p else nan
     pcg_gbs = B_PCG_ITER * n / (us_iter * 1e-6) / 1e9 if st_p else nan
     # learned iteration: the solve's median time over its iterations (coarse
     # levels, the network and set-up included, so a lower bound on the fine
@@ -449,67 +638,30 @@ def main():
     if "sequence" in legs:
         from boilsim.sequence import run_sequence, summarize
 
-        log(f"AR(1) sequence: {args.seq_frames} frames, learned vs Jacobi-PCG (FP64 refinement, warm starts)")
-        res = run_sequence(core, torch, gmat, args.seq_frames, methods=("learned", "pcg-fp32", "pcg-fp16"),
-                           net=net, tol=args.tol)
+        methods = tuple(args.seq_methods.split(","))
+        log(f"AR(1) sequence: {args.seq_frames} frames, {', '.join(methods)} (FP64 refinement, warm starts)")
+
+        def seq_log(t, recs):
+            if t % 10 == 9 or t == args.seq_frames - 1:
+                log(f"  frame {t}: " + ", ".join(f"{m} {r['ms']:.1f} ms {r['inner_iterations']} its"
+                                               for m, r in recs.items()))
+
+        res = run_sequence(core, torch, gmat, args.seq_frames, methods=methods, net=net, tol=args.tol, log=seq_log)
         seq = summarize(res)
         for m, v in seq.items():
             log(f"  {m}: median {v['ms_per_solve_median']:.1f} ms/solve, {v['iterations_mean']:.1f} its, "
                 f"max relres {v['max_relres_true']:.2e}, err vs FP64 {v['max_err_vs_fp64']:.2e}")
-        seq["speedup_learned_vs_pcg_fp32"] = (seq["pcg-fp32"]["ms_per_solve_median"] /
-                                              seq["learned"]["ms_per_solve_median"])
+        if "learned" in seq and "pcg-fp32" in seq:
+            seq["speedup_learned_vs_pcg_fp32"] = (seq["pcg-fp32"]["ms_per_solve_median"] /
+                                                  seq["learned"]["ms_per_solve_median"])
         seq["rhs"] = "AR(1) GRF (step corr 0.95, ell 12 cells, unit variance) + drifting plume; warm starts"
 
     cpu = None
-    if "cpu" in legs and st_p:
-        import oracle
-
-        threads = min(len(os.sched_getaffinity(0)), 16)
-        oracle.set_threads(threads)
-        div_np = gdiv.cpu().numpy()
-        t0 = time.perf_counter()
-        _, it_c, _ = oracle.pcg_ref(div_np, mat, cfg.dx, DT, RHO, 1e-30, args.cpu_iters)
-        dt_c = time.perf_counter() - t0
-        ms_per_it = dt_c * 1e3 / max(it_c, 1)
-        cpu = {"value": ms_per_it * st_p["iterations"], "unit": "ms/projection (plain PCG solve, same iterations)",
-               "cores": threads, "kind": "port",
-               "sample": f"{it_c} iterations of the oracle's FP32/FP64 Jacobi-PCG at {nx}x{ny}x{nz} on {threads} "
-                         f"threads ({dt_c:.1f} s incl. set-up), scaled to the GPU plain-PCG count "
-                         f"{st_p['iterations']}"}
-        log(f"cpu baseline: {ms_per_it:.1f} ms/iteration on {threads} threads")
-        # the same on one thread (a serial loop over cells), a shorter sample
-        oracle.set_threads(1)
-        t0 = time.perf_counter()
-        _, it_1, _ = oracle.pcg_ref(div_np, mat, cfg.dx, DT, RHO, 1e-30, 3)
-        ms_1 = (time.perf_counter() - t0) * 1e3 / max(it_1, 1)
-        oracle.set_threads(threads)
-        cpu["single_thread"] = {"ms_per_iteration": ms_1, "iterations_sampled": it_1,
-                                "ms_per_projection": ms_1 * st_p["iterations"]}
-        log(f"cpu baseline: {ms_1:.1f} ms/iteration on 1 thread")
-        # parity of the shipped Jacobi path with the oracle on config 1
-        # (64x64x48 pot, seed-7 velocities with the plume, 200 sweeps from 0)
-        from boilsim.fixtures import CONFIGS as _C
-
-        c1 = _C["small"]
-        m1 = pot_mask(*c1.shape)
-        v1 = seeded_velocities(*c1.shape, seed=7, plume=True)
-        d1 = oracle.divergence(*v1, m1, c1.dx)
-        pr, tr = np.zeros(c1.shape, np.float32), np.zeros(c1.shape, np.float32)
-        oracle.pressure_solve(pr, tr, d1, m1, c1.dx, DT, RHO, 200)
-        pg, tg = torch.zeros(c1.shape, device="cuda"), torch.zeros(c1.shape, device="cuda")
-        dg, mg = d(d1), d(m1)
-        core.pressure_solve(ptr(pg), ptr(tg), ptr(dg), ptr(mg), *c1.shape, c1.dx, DT, RHO, 200)
-        torch.cuda.synchronize()
-        got = pg.cpu().numpy()
-        fl = m1 == 0
-        rel = np.abs(got - pr)[fl] / np.maximum(np.abs(pr[fl]), 1e-6)
-        cpu["parity"] = {"case": "64x64x48 pot, seed-7 velocities + plume, 200 Jacobi sweeps from 0",
-                         "bits_equal": bool(np.array_equal(got.view(np.uint32), pr.view(np.uint32))),
-                         "max_rel_fluid": float(rel.max()), "gate": 1e-4}
-        log(f"parity (Jacobi, 200 sweeps, 64x64x48): bits equal {cpu['parity']['bits_equal']}, "
-            f"max rel {cpu['parity']['max_rel_fluid']:.1e}")
+    if "cpu" in legs:
+        cpu = cpu_leg(core, torch, mat, (ux, uy, uz), cfg, args.cpu_threads, d)
 
     fin = lambda v: v if v == v else None  # NaN (leg skipped) -> null
+    K0 = "fcg_spmv_dir_kernel"
     out = {
         "metric": METRIC,
         "value": fin(ms_l),
@@ -536,15 +688,31 @@ def main():
         "speedup_vs_plain_pcg": fin(ms_p / ms_l),
         "ablation_mg_jacobi_coarse": abl,
         "max_div_before": maxdiv0,
-        "roofline": {"bound": "hbm", "achieved": fin(jac_gbs), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": fin(jac_gbs / HBM_PEAK_GBS),
-                     "traffic": TRAFFIC_JACOBI_PAIR * n if (cfg.name == "prod" and TRAFFIC_JACOBI_PAIR) else None,
-                     "traffic_unit": "bytes/launch (PMC, calibrated)",
-                     "kernel": "jacobi_pair_kernel (two mask-free sweeps per launch)",
-                     "bytes_per_cell": 2 * B_JACOBI, "us_per_launch": fin(us_pair),
-                     "us_per_sweep_200_sweep_driver": fin(us_sweep)},
+        "roofline": None if not passes else {
+            "bound": "hbm", "achieved": passes[K0]["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": passes[K0]["frac"],
+            "traffic": (TRAFFIC_PASSES[K0] * n if (cfg.name == "prod" and TRAFFIC_PASSES.get(K0)) else None),
+            "traffic_unit": "bytes/launch (PMC FETCH_SIZE + WRITE_SIZE, calibrated)",
+            "kernel": K0 + " (the learned solve's dominant kernel: direction p = z + beta p, SpMV Ap and p.Ap, "
+                           "the previous iteration's x += alpha p, one i-marching pass)",
+            "bytes_per_cell": PASS_BYTES[K0], "us_per_launch": passes[K0]["us_mean"],
+            "us_median": passes[K0]["us_median"], "launches_timed": passes[K0]["launches"]},
+        "roofline_fine_passes": passes,
+        "roofline_jacobi": {"bound": "hbm", "achieved": fin(jac_gbs), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": fin(jac_gbs / HBM_PEAK_GBS),
+                            "frac_at_reference_layout": fin(jac_gbs / HBM_PEAK_GBS * B_JACOBI_REF / B_JACOBI),
+                            "traffic": TRAFFIC_JACOBI_PAIR * n if (cfg.name == "prod" and TRAFFIC_JACOBI_PAIR) else None,
+                            "traffic_unit": "bytes/launch (PMC, calibrated)",
+                            "kernel": "jacobi_pair_kernel (two mask-free sweeps per launch)",
+                            "bytes_per_cell": 2 * B_JACOBI,
+                            "convention": "12 B/cell per sweep (p_in, div, p_out; the class rides in p as NaN "
+                                          "sentinels, m = 0); frac_at_reference_layout counts 12 + m with the "
+                                          "reference's int32 mask (m = 4, 16 B/cell)",
+                            "us_per_launch": fin(us_pair), "us_per_sweep_200_sweep_driver": fin(us_sweep)},
+        "jacobi_ms_per_projection": jcfg,
         "roofline_pcg_iteration": {"bound": "hbm", "achieved": fin(pcg_gbs), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                    "frac": fin(pcg_gbs / HBM_PEAK_GBS), "bytes_per_cell": B_PCG_ITER,
+                                   "convention": "fused-minimum 40 + 2m B/cell, m = 1 (one-byte class code)",
                                    "us_per_iteration": fin(us_iter)},
         "roofline_learned_iteration": {"bound": "hbm", "achieved": fin(learned_gbs), "peak": HBM_PEAK_GBS,
                                        "unit": "GB/s", "frac": fin(learned_gbs / HBM_PEAK_GBS),

@@ -1302,8 +1302,10 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
     // values go to the fine temporary (that pass reads them and writes z).
     const bool fused_post = v0 == 4;
     float* zpre = fused_post ? W.zt0 : z;
+    if (fold) pass_mark(st, 1, 0);
     int rc = pre_restrict(st, T0, v0, W.L0, W.L1, C1, code0, r, zpre, W.r1, done, 1, 7680, K0, fold);
     if (rc != hipSuccess) return rc;
+    if (fold) pass_mark(st, 1, 1);
     if (fold) {
         r = fold->r_out;  // the rest of the cycle works on the updated residual
         if (fold->after) fold->after(st, fold->after_arg);
@@ -1356,8 +1358,10 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
     if (fused_post) {
         const PpShape P = pp_shape(nx, ny, nz);
         if (P.grid != mg_dots_groups(nx, ny, nz)) return (int)hipErrorInvalidValue;
+        pass_mark(st, 2, 0);
         prolong_post_kernel<<<P.grid, 256, 0, st>>>(nx, ny, nz, P.ni, P.jtiles, P.ktiles, code0, W.zt0, W.z1, r, z_old,
                                                     z, dots_part, kPostW, done);
+        pass_mark(st, 2, 1);
         return (int)hipGetLastError();
     }
     rc = prolong_level(st, T0, v0, W.L0, W.L1, C1, code0, W.z1, z, W.zt0, done);

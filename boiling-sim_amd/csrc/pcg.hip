@@ -683,7 +683,10 @@ static int pcg_iterate_t(const Tiles& T, int vec, const PcgBuffers& B, int it0, 
     BS_RETURN_IF_ERR((hipError_t)ensure_poll());
     hipEvent_t ev[2];
     BS_RETURN_IF_ERR(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    BS_RETURN_IF_ERR(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    if (const hipError_t e = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming); e != hipSuccess) {
+        (void)hipEventDestroy(ev[0]);
+        return (int)e;
+    }
     const size_t done_off = offsetof(PcgCtrl, done);
     int it = it0, batch = 8, slot = 0;
     bool pending[2] = {false, false};
@@ -1353,6 +1356,36 @@ static void fcg_fold_check(hipStream_t st, void* arg) {
 // z = M(r) (+ partials of r.z and r.z_old into partZ).  With r_out (V-cycle
 // only), r is the residual before the update r_out = r - alpha Ap, which the
 // cycle's first pass performs (followed by the convergence check).
+// ---------------------------------------------------------------------------
+// Pass timing (diagnostic, bs_fcg_pass_timing): while it is on, flexible-CG
+// solves launch eagerly (no graphs) and pass_mark brackets each fine-level
+// pass with HIP events on the solve's stream.
+// ---------------------------------------------------------------------------
+constexpr int kTimedPasses = 3, kTimedMarks = 256;
+struct PassTimer {
+    bool on = false;
+    bool created = false;
+    hipEvent_t ev[kTimedPasses][2][kTimedMarks];
+    int n[kTimedPasses] = {0, 0, 0};
+};
+static PassTimer g_timer;
+
+void pass_mark(hipStream_t st, int pass, int edge) {
+    if (!g_timer.on || pass < 0 || pass >= kTimedPasses) return;
+    int& k = g_timer.n[pass];
+    if (k >= kTimedMarks) return;
+    (void)hipEventRecord(g_timer.ev[pass][edge][k], st);
+    if (edge == 1) ++k;
+}
+
+// BOILSIM_FCG_EAGER=1 launches the kernels directly instead of replaying the
+// captured graphs (same results; used to check the graph path); pass timing
+// needs the direct launches too.
+static bool fcg_eager() {
+    const char* e = getenv("BOILSIM_FCG_EAGER");
+    return g_timer.on || (e && e[0] == '1');
+}
+
 static int fcg_precond(hipStream_t st, const FcgBuffers& F, const float* r, float* z, const float* z_old,
                        float* r_out = nullptr) {
     const PcgBuffers& B = F.B;
@@ -1393,8 +1426,10 @@ static int fcg_enqueue_iteration(const Tiles& T, int vec, const FcgBuffers& F, i
     if (F.fused) {
         const RollShape rs = roll_shape(T.nx, T.ny, T.nz);
         float* pb[2] = {B.p, F.p2};
+        pass_mark(st, 0, 0);
         fcg_spmv_dir_kernel<<<rs.grid, 256, 0, st>>>(T.nx, T.ny, T.nz, rs.ni, rs.jtiles, rs.ktiles, B.code, F.z[par],
                                                      pb[par], pb[par ^ 1], B.x, B.ap, B.partT, B.ctrl);
+        pass_mark(st, 0, 1);
         fcg_pap_reduce_kernel<<<1, 1024, 0, st>>>(B.partT, rs.grid, B.ctrl);
         if (F.fold) {
             // r[par] -> r[1-par] in the V-cycle's first pass, then the check
@@ -1517,8 +1552,7 @@ static int unit_graph(const Tiles& T, int vec, const FcgBuffers& F, hipGraphExec
 // The first preconditioner application z[0] = M(r) and the first direction's
 // scalars (one graph launch instead of the cycle's ~55 kernel launches).
 static int fcg_begin(const Tiles& T, int vec, const FcgBuffers& F, hipStream_t st) {
-    const char* eager_env = getenv("BOILSIM_FCG_EAGER");
-    if (eager_env && eager_env[0] == '1') {
+    if (fcg_eager()) {
         const int rc = fcg_precond(st, F, F.B.r, F.z[0], nullptr);
         if (rc != hipSuccess) return rc;
         fcg_start(T, vec, F, st);
@@ -1535,17 +1569,17 @@ static int fcg_begin(const Tiles& T, int vec, const FcgBuffers& F, hipStream_t s
 static int fcg_iterate(const Tiles& T, int vec, const FcgBuffers& F, int max_iter, hipStream_t st) {
     const PcgBuffers& B = F.B;
     BS_RETURN_IF_ERR((hipError_t)ensure_poll());
-    // BOILSIM_FCG_EAGER=1 launches the kernels directly instead of replaying
-    // the captured graphs (same results; used to check the graph path).
-    const char* eager_env = getenv("BOILSIM_FCG_EAGER");
-    const bool eager = eager_env && eager_env[0] == '1';
+    const bool eager = fcg_eager();
     hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};  // pair, single of parity 0, single of parity 1
     int rc = hipSuccess;
     for (int u = 0; u < 3 && !eager && rc == hipSuccess; ++u) rc = unit_graph(T, vec, F, &exec[u], u);
     if (rc != hipSuccess) return rc;
     hipEvent_t ev[2];
     BS_RETURN_IF_ERR(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    BS_RETURN_IF_ERR(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    if (const hipError_t e = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming); e != hipSuccess) {
+        (void)hipEventDestroy(ev[0]);
+        return (int)e;
+    }
     // Units (an iteration pair, or a single iteration) are queued two deep,
     // each followed by a copy of the control block into a pinned slot, and
     // polled oldest first.  A unit is milliseconds of GPU work, so the host
@@ -2273,6 +2307,55 @@ extern "C" int bs_fcg_refine_solve(double* x, float* x32, const float* div_u, co
         stats->relres_inner_last = inner_rel;
         stats->fallback = fell_back;
         stats->pcg_iterations = pcg_it;
+    }
+    return hipSuccess;
+}
+
+// Pass timing of the learned flexible CG (diagnostic; no counterpart in the
+// reference).  enable != 0 clears the records and turns timing on: the
+// following bs_fcg_solve / bs_fcg_refine_solve calls launch their kernels
+// directly and record HIP events on their stream around each fine-level pass
+// (pass 0: fcg_spmv_dir_kernel, direction + SpMV + p.Ap + the deferred x
+// step; 1: pre2_restrict_kernel<true>, the V-cycle's first fine pass with the
+// folded residual update; 2: prolong_post_kernel, its last fine pass).
+// enable == 0 waits for the records, turns timing off and writes, per pass,
+// the median and mean launch time in microseconds and the number of launches
+// timed (arrays of 3; null skips).
+extern "C" int bs_fcg_pass_timing(int enable, double* us_median, double* us_mean, int* counts) {
+    using namespace bs;
+    if (enable) {
+        if (!g_timer.created) {
+            for (int p = 0; p < kTimedPasses; ++p)
+                for (int e = 0; e < 2; ++e)
+                    for (int k = 0; k < kTimedMarks; ++k)
+                        BS_RETURN_IF_ERR(hipEventCreate(&g_timer.ev[p][e][k]));
+            g_timer.created = true;
+        }
+        for (int p = 0; p < kTimedPasses; ++p) g_timer.n[p] = 0;
+        g_timer.on = true;
+        return hipSuccess;
+    }
+    g_timer.on = false;
+    for (int p = 0; p < kTimedPasses; ++p) {
+        const int m = g_timer.n[p];
+        double d[kTimedMarks];
+        double sum = 0.0;
+        for (int k = 0; k < m; ++k) {
+            BS_RETURN_IF_ERR(hipEventSynchronize(g_timer.ev[p][1][k]));
+            float ms = 0.0f;
+            BS_RETURN_IF_ERR(hipEventElapsedTime(&ms, g_timer.ev[p][0][k], g_timer.ev[p][1][k]));
+            d[k] = 1e3 * (double)ms;
+            sum += d[k];
+        }
+        for (int a = 1; a < m; ++a)  // insertion sort for the median
+            for (int b = a; b > 0 && d[b - 1] > d[b]; --b) {
+                const double t = d[b];
+                d[b] = d[b - 1];
+                d[b - 1] = t;
+            }
+        if (us_median) us_median[p] = m ? (m & 1 ? d[m / 2] : 0.5 * (d[m / 2 - 1] + d[m / 2])) : 0.0;
+        if (us_mean) us_mean[p] = m ? sum / m : 0.0;
+        if (counts) counts[p] = m;
     }
     return hipSuccess;
 }

@@ -59,6 +59,14 @@ size_t step_ws_bytes();
 void scale_to_optimal(hipStream_t st, int nx, int ny, int nz, const uint8_t* code, const float* r, float* e,
                       double* part, const int* done);
 
+// Timing marks of the flexible CG's fine-level passes (diagnostic): while
+// pass timing is on (bs_fcg_pass_timing), pass_mark(st, pass, 0) records a
+// HIP event on st right before pass `pass` is enqueued and pass_mark(st,
+// pass, 1) right after it (0: direction + SpMV, 1: the V-cycle's first fine
+// pass with the folded residual update, 2: its last fine pass); otherwise it
+// does nothing.
+void pass_mark(hipStream_t st, int pass, int edge);
+
 // Resolves (once per device, with it current) the current device's address
 // of the fixed scratch array of bs_pcg_solve_noalloc and returns it through
 // *out when out is not null.

@@ -130,6 +130,7 @@ SIGNATURES = {
     "bs_mg_chebyshev": [VP, VP, VP, VP, I, I, I, I, c_double, S],
     "bs_mg_prolong_post": [VP, VP, VP, VP, VP, VP, VP, VP, c_size_t, POINTER(c_int), I, I, I, I, c_double, S],
     "bs_fcg_solve": [VP, VP, VP, I, I, I, F, F, F, I, I, c_double, I, I, I, VP, VP, VP, c_size_t, S, c_void_p],
+    "bs_fcg_pass_timing": [I, c_void_p, c_void_p, c_void_p],
     "bs_conduction_sweep_launch": [VP, VP, VP, VP, VP, VP, VP, I, I, I, I, F, F, F, F, I, I, S],
     "bs_conduction_solve_launch": [VP, VP, VP, VP, VP, VP, VP, I, I, I, I, F, F, F, F, I, I, I, S],
     "bs_refine_workspace_bytes": [I, I, I],

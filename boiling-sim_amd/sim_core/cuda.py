@@ -313,6 +313,25 @@ class SimCore:
         )
         return st.as_dict()
 
+    # Fine-level passes of the learned flexible CG, in bs_fcg_pass_timing's order.
+    TIMED_PASSES = ("fcg_spmv_dir_kernel", "pre2_restrict_kernel<true>", "prolong_post_kernel")
+
+    @staticmethod
+    def fcg_pass_timing_start() -> None:
+        """Turn pass timing on (see bs_fcg_pass_timing): later flexible-CG
+        solves launch eagerly and time their fine-level passes."""
+        _lib.call("bs_fcg_pass_timing", 1, None, None, None)
+
+    @classmethod
+    def fcg_pass_timing_stop(cls) -> dict:
+        """Turn pass timing off; {kernel: {"us_median", "us_mean", "launches"}}."""
+        med = (ctypes.c_double * 3)()
+        mean = (ctypes.c_double * 3)()
+        cnt = (ctypes.c_int * 3)()
+        _lib.call("bs_fcg_pass_timing", 0, ctypes.addressof(med), ctypes.addressof(mean), ctypes.addressof(cnt))
+        return {k: {"us_median": med[i], "us_mean": mean[i], "launches": cnt[i]}
+                for i, k in enumerate(cls.TIMED_PASSES)}
+
     def mg_smooth_restrict(self, code_ptr, r_ptr, z_ptr, rc_ptr, nx, ny, nz, fused=True, lo=0.0, stream=0) -> None:
         """The V-cycle's first pass on one level: z = two smoothing steps from
         zero (damped Jacobi for lo <= 0, Chebyshev over [lo, 2] otherwise), rc =

@@ -286,6 +286,17 @@ int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx, int ny, i
                  float rho, int mat_fluid, int mat_air, double tol, int max_iter, int warm_start,
                  int precond, const float* unet_params, const void* unet_packed, void* workspace,
                  size_t workspace_bytes, bs_stream_t stream, bs_fcg_stats* stats);
+/* Pass timing of the learned flexible CG (diagnostic; no counterpart in the
+ * reference).  enable != 0 clears the records and turns timing on: later
+ * bs_fcg_solve / bs_fcg_refine_solve calls launch their kernels directly (no
+ * graphs) and record HIP events on their stream around each fine-level pass
+ * (0: fcg_spmv_dir_kernel — direction, SpMV, p.Ap and the deferred x step;
+ * 1: pre2_restrict_kernel<true> — the V-cycle's first fine pass with the
+ * folded residual update; 2: prolong_post_kernel — its last fine pass).
+ * enable == 0 waits for the records, turns timing off and writes per pass the
+ * median and mean launch time in microseconds and the number of launches
+ * timed (arrays of 3; null pointers are skipped).  Not thread-safe. */
+int bs_fcg_pass_timing(int enable, double* us_median, double* us_mean, int* counts);
 /* The V-cycle's first pass on one level (diagnostic; no counterpart in the
  * reference): z = two smoothing steps from zero on A z = r over class codes
  * `code` (0 air, 1 solid, 2 + solid neighbours for fluid) — damped Jacobi

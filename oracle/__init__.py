@@ -102,6 +102,8 @@ def lib():
         L.orc_pcg_f64.restype = I
         L.orc_pcg_ref.argtypes = [P, P, P, I, I, I, F, F, F, F, I, I, I, P]
         L.orc_pcg_ref.restype = I
+        L.orc_pcg_true.argtypes = [P, P, P, I, I, I, F, F, F, c_double, I, I, I, I, P, P]
+        L.orc_pcg_true.restype = I
         _lib = L
     return _lib
 
@@ -273,6 +275,24 @@ def pcg_f64(b, mat, tol=1e-12, max_iter=100000, x0=None, mat_fluid=MAT_FLUID, ma
     x = np.zeros(mat.shape, np.float64) if x0 is None else np.array(x0, dtype=np.float64, order="C")
     it = lib().orc_pcg_f64(_p(x), _p(b), _p(mat), nx, ny, nz, float(tol), int(max_iter), mat_fluid, mat_air)
     return x, int(it)
+
+
+def pcg_true(div, mat, dx, dt, rho, tol, max_iter, max_restarts=4, mat_fluid=MAT_FLUID, mat_air=MAT_AIR):
+    """Jacobi-PCG (FP32 vectors, FP64 scalars) to a true relative residual
+    tol with residual replacement, the semantics of bs_pcg_solve.  Returns
+    (x, iterations, restarts, true relative residual)."""
+    mat = _i32(mat)
+    nx, ny, nz = mat.shape
+    div = _f32(div)
+    x = np.empty(mat.shape, np.float32)
+    rel = c_double()
+    rst = ctypes.c_int()
+    it = lib().orc_pcg_true(_p(x), _p(div), _p(mat), nx, ny, nz, float(dx), float(dt), float(rho), float(tol),
+                            int(max_iter), int(max_restarts), mat_fluid, mat_air, ctypes.addressof(rel),
+                            ctypes.addressof(rst))
+    if it < 0:
+        raise MemoryError("orc_pcg_true: allocation failed")
+    return x, int(it), int(rst.value), rel.value
 
 
 def pcg_ref(div, mat, dx, dt, rho, tol, max_iter, mat_fluid=MAT_FLUID, mat_air=MAT_AIR):

@@ -420,72 +420,249 @@ done:
  * iter+1 on that break.  Vectors are FP32 and the scalars (dots, alpha, beta)
  * FP64 — the precision split of the HIP solver, which replaces the
  * reference's FP32 dots. Returns the iteration count; *bsq_out gets b.b. */
+static inline float pcg_nb(const float* p, const int32_t* mat, int64_t c, float ps, int mf, int ma) {
+    return nbr_value(mat[c], p[c], ps, mf, ma);
+}
+
+static inline float pcg_z(float r, float dg) {
+    return dg == 0.0f ? 0.0f : (dg < 0.0f ? r : r / dg);
+}
+
+/* The PCG's state: FP32 vectors and per-plane FP64 partials (summed in
+ * plane order, so every dot is deterministic for any thread count). */
+typedef struct {
+    int nx, ny, nz, mf, ma;
+    int64_t n, plane;
+    const int32_t* mat;
+    float *b, *r, *p, *q, *dg;
+    double* part; /* 3 x nx */
+} PcgState;
+
+static int pcg_state_init(PcgState* S, const float* div, const int32_t* mat, int nx, int ny, int nz, float dx,
+                          float dt, float rho, int mf, int ma) {
+    S->nx = nx; S->ny = ny; S->nz = nz; S->mf = mf; S->ma = ma; S->mat = mat;
+    S->n = (int64_t)nx * ny * nz;
+    S->plane = (int64_t)ny * nz;
+    const size_t fb = sizeof(float) * (size_t)S->n;
+    S->b = (float*)malloc(fb); S->r = (float*)malloc(fb); S->p = (float*)malloc(fb);
+    S->q = (float*)malloc(fb); S->dg = (float*)malloc(fb);
+    S->part = (double*)malloc(sizeof(double) * (size_t)nx * 3);
+    double* d = (double*)malloc(sizeof(double) * (size_t)S->n);
+    if (!S->b || !S->r || !S->p || !S->q || !S->dg || !S->part || !d) { free(d); return -1; }
+    orc_diag(d, mat, nx, ny, nz, mf, ma);
+    const float scale = -rho * dx * dx / dt;
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < S->n; ++t) {
+        S->b[t] = (mat[t] == mf) ? scale * div[t] : 0.0f;
+        S->dg[t] = (float)d[t]; /* 0: non-fluid, -1: z = r, else 6 - #solid */
+    }
+    free(d);
+    return 0;
+}
+
+static void pcg_state_free(PcgState* S) {
+    free(S->b); free(S->r); free(S->p); free(S->q); free(S->dg); free(S->part);
+}
+
+static double pcg_sum(const double* v, int nx) {
+    double s = 0.0;
+    for (int i = 0; i < nx; ++i) s += v[i];
+    return s;
+}
+
+/* r = b, p = z = M^-1 r; returns r.z. */
+static double pcg_cold_start(PcgState* S) {
+    const int64_t plane = S->plane;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < S->nx; ++i) {
+        double s = 0.0;
+        for (int64_t t = i * plane; t < (i + 1) * plane; ++t) {
+            S->r[t] = S->b[t];
+            S->p[t] = pcg_z(S->r[t], S->dg[t]);
+            s += (double)S->r[t] * (double)S->p[t];
+        }
+        S->part[i] = s;
+    }
+    return pcg_sum(S->part, S->nx);
+}
+
+/* q = A p in float32 (6*p_self - s, s summed in the reference order);
+ * returns p.q. */
+static double pcg_spmv_pass(PcgState* S) {
+    const int nx = S->nx, ny = S->ny, nz = S->nz, mf = S->mf, ma = S->ma;
+    const int64_t plane = S->plane;
+    const int32_t* mat = S->mat;
+    const float* p = S->p;
+    float* q = S->q;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i) {
+        double s_pq = 0.0;
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int64_t c = CIDX(i, j, k);
+                if (mat[c] != mf) { q[c] = 0.0f; continue; }
+                const float ps = p[c];
+                float s = i > 0 ? pcg_nb(p, mat, c - plane, ps, mf, ma) : 0.0f;
+                s = s + (i < nx - 1 ? pcg_nb(p, mat, c + plane, ps, mf, ma) : 0.0f);
+                s = s + (j > 0 ? pcg_nb(p, mat, c - nz, ps, mf, ma) : 0.0f);
+                s = s + (j < ny - 1 ? pcg_nb(p, mat, c + nz, ps, mf, ma) : 0.0f);
+                s = s + (k > 0 ? pcg_nb(p, mat, c - 1, ps, mf, ma) : 0.0f);
+                s = s + (k < nz - 1 ? pcg_nb(p, mat, c + 1, ps, mf, ma) : 0.0f);
+                const float qc = 6.0f * ps - s;
+                q[c] = qc;
+                s_pq += (double)ps * (double)qc;
+            }
+        S->part[i] = s_pq;
+    }
+    return pcg_sum(S->part, nx);
+}
+
+/* x += af p, r -= af q; returns r.r in *rr and r.z (z = M^-1 r) in *rz. */
+static void pcg_update_pass(PcgState* S, float* x, float af, double* rr, double* rz) {
+    const int nx = S->nx;
+    const int64_t plane = S->plane;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i) {
+        double s_rr = 0.0, s_rz = 0.0;
+        for (int64_t t = i * plane; t < (i + 1) * plane; ++t) {
+            x[t] = x[t] + af * S->p[t];
+            const float rt = S->r[t] - af * S->q[t];
+            S->r[t] = rt;
+            s_rr += (double)rt * (double)rt;
+            s_rz += (double)rt * (double)pcg_z(rt, S->dg[t]);
+        }
+        S->part[nx + i] = s_rr;
+        S->part[2 * nx + i] = s_rz;
+    }
+    *rr = pcg_sum(S->part + nx, nx);
+    *rz = pcg_sum(S->part + 2 * nx, nx);
+}
+
+/* p = M^-1 r + bf p. */
+static void pcg_direction_pass(PcgState* S, float bf) {
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < S->n; ++t) S->p[t] = pcg_z(S->r[t], S->dg[t]) + bf * S->p[t];
+}
+
+/* The true residual b - A x of the FP32 iterate, formed in FP64 (the
+ * neighbour sum and the subtraction in double); returns its squared norm.
+ * With restart, r takes it rounded to FP32 and p = M^-1 r, and *rz = r.z. */
+static double pcg_true_residual(PcgState* S, const float* x, int restart, double* rz) {
+    const int nx = S->nx, ny = S->ny, nz = S->nz, mf = S->mf, ma = S->ma;
+    const int64_t plane = S->plane;
+    const int32_t* mat = S->mat;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i) {
+        double s_rr = 0.0, s_rz = 0.0;
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int64_t c = CIDX(i, j, k);
+                if (mat[c] != mf) {
+                    if (restart) { S->r[c] = 0.0f; S->p[c] = 0.0f; }
+                    continue;
+                }
+                const double xs = (double)x[c];
+                double s = i > 0 ? (double)pcg_nb(x, mat, c - plane, x[c], mf, ma) : 0.0;
+                s += i < nx - 1 ? (double)pcg_nb(x, mat, c + plane, x[c], mf, ma) : 0.0;
+                s += j > 0 ? (double)pcg_nb(x, mat, c - nz, x[c], mf, ma) : 0.0;
+                s += j < ny - 1 ? (double)pcg_nb(x, mat, c + nz, x[c], mf, ma) : 0.0;
+                s += k > 0 ? (double)pcg_nb(x, mat, c - 1, x[c], mf, ma) : 0.0;
+                s += k < nz - 1 ? (double)pcg_nb(x, mat, c + 1, x[c], mf, ma) : 0.0;
+                const double r64 = (double)S->b[c] - (6.0 * xs - s);
+                s_rr += r64 * r64;
+                if (restart) {
+                    const float rf = (float)r64;
+                    const float zf = pcg_z(rf, S->dg[c]);
+                    S->r[c] = rf;
+                    S->p[c] = zf;
+                    s_rz += (double)rf * (double)zf;
+                }
+            }
+        S->part[nx + i] = s_rr;
+        S->part[2 * nx + i] = s_rz;
+    }
+    if (rz) *rz = pcg_sum(S->part + 2 * nx, nx);
+    return pcg_sum(S->part + nx, nx);
+}
+
 int orc_pcg_ref(float* p_out, const float* div, const int32_t* mat, int nx, int ny, int nz,
                 float dx, float dt, float rho, float tol, int max_iter,
                 int mat_fluid, int mat_air, double* bsq_out) {
     if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
-    const int64_t n = (int64_t)nx * ny * nz;
-    float* b = (float*)malloc(sizeof(float) * (size_t)n);
-    float* r = (float*)malloc(sizeof(float) * (size_t)n);
-    float* z = (float*)malloc(sizeof(float) * (size_t)n);
-    float* p = (float*)malloc(sizeof(float) * (size_t)n);
-    float* q = (float*)malloc(sizeof(float) * (size_t)n);
-    float* dg = (float*)malloc(sizeof(float) * (size_t)n);
-    double* d = (double*)malloc(sizeof(double) * (size_t)n);
-    orc_diag(d, mat, nx, ny, nz, mat_fluid, mat_air);
-    const float scale = -rho * dx * dx / dt;
-    for (int64_t t = 0; t < n; ++t) {
-        b[t] = (mat[t] == mat_fluid) ? scale * div[t] : 0.0f;
-        dg[t] = (float)d[t]; /* 0: non-fluid, -1: z = r, else 6 - #solid */
+    PcgState S;
+    if (pcg_state_init(&S, div, mat, nx, ny, nz, dx, dt, rho, mat_fluid, mat_air) != 0) {
+        pcg_state_free(&S);
+        return -1;
     }
-    const double bsq = dot_f(b, b, n);
+    const double bsq = dot_f(S.b, S.b, S.n);
     if (bsq_out) *bsq_out = bsq;
     int iter = 0;
-    memset(p_out, 0, sizeof(float) * (size_t)n);
+    memset(p_out, 0, sizeof(float) * (size_t)S.n);
     if (bsq == 0.0) goto done;
     const double stop = (double)tol * (double)tol * bsq;
-    for (int64_t t = 0; t < n; ++t) {
-        r[t] = b[t];
-        z[t] = dg[t] == 0.0f ? 0.0f : (dg[t] < 0.0f ? r[t] : r[t] / dg[t]);
-        p[t] = z[t];
-    }
-    double rz = dot_f(r, z, n);
+    double rz = pcg_cold_start(&S);
+    /* Three fused passes per iteration, the same float operations as the
+     * reference's kernel sequence: (1) q = A p with p.q; (2) x += alpha p,
+     * r -= alpha q with r.r and r.z; (3) p = z + beta p, z formed again from r. */
     for (iter = 0; iter < max_iter; ++iter) {
-        /* q = A p in float32: 6*p_self - s with s summed in the reference order. */
-#pragma omp parallel for schedule(static)
-        for (int i = 0; i < nx; ++i)
-            for (int j = 0; j < ny; ++j)
-                for (int k = 0; k < nz; ++k) {
-                    const int64_t c = CIDX(i, j, k);
-                    if (mat[c] != mat_fluid) { q[c] = 0.0f; continue; }
-                    const float ps = p[c];
-                    float s = nbr_at(p, mat, i - 1, j, k, nx, ny, nz, ps, mat_fluid, mat_air);
-                    s = s + nbr_at(p, mat, i + 1, j, k, nx, ny, nz, ps, mat_fluid, mat_air);
-                    s = s + nbr_at(p, mat, i, j - 1, k, nx, ny, nz, ps, mat_fluid, mat_air);
-                    s = s + nbr_at(p, mat, i, j + 1, k, nx, ny, nz, ps, mat_fluid, mat_air);
-                    s = s + nbr_at(p, mat, i, j, k - 1, nx, ny, nz, ps, mat_fluid, mat_air);
-                    s = s + nbr_at(p, mat, i, j, k + 1, nx, ny, nz, ps, mat_fluid, mat_air);
-                    q[c] = 6.0f * ps - s;
-                }
-        const double pq = dot_f(p, q, n);
+        const double pq = pcg_spmv_pass(&S);
         const double alpha = pq != 0.0 ? rz / pq : 0.0;
-        const float af = (float)alpha;
-        for (int64_t t = 0; t < n; ++t) {
-            p_out[t] = p_out[t] + af * p[t];
-            r[t] = r[t] - af * q[t];
-        }
-        if (iter >= 8 && iter % 5 == 0) {
-            const double rr = dot_f(r, r, n);
-            if (rr < stop) { ++iter; break; }
-        }
-        for (int64_t t = 0; t < n; ++t) z[t] = dg[t] == 0.0f ? 0.0f : (dg[t] < 0.0f ? r[t] : r[t] / dg[t]);
-        const double rzn = dot_f(r, z, n);
+        double rr, rzn;
+        pcg_update_pass(&S, p_out, (float)alpha, &rr, &rzn);
+        if (iter >= 8 && iter % 5 == 0 && rr < stop) { ++iter; break; }
         const double beta = rz != 0.0 ? rzn / rz : 0.0;
-        const float bf = (float)beta;
-        for (int64_t t = 0; t < n; ++t) p[t] = z[t] + bf * p[t];
+        pcg_direction_pass(&S, (float)beta);
         rz = rzn;
     }
 done:
-    free(b); free(r); free(z); free(p); free(q); free(dg); free(d);
+    pcg_state_free(&S);
+    return iter;
+}
+
+/* The same PCG to a TRUE relative residual tol, as bs_pcg_solve
+ * (boiling-sim_amd/csrc/pcg.hip): cold start, the recursive residual tested
+ * every iteration, then ||b - A x|| formed in FP64; while that is above tol
+ * the solve restarts from x with r = b - A x (residual replacement), at most
+ * max_restarts times, the iteration count continuing.  Returns the
+ * iterations; *relres_out = ||b - A x||_2 / ||b||_2, *restarts_out the
+ * restarts taken. */
+int orc_pcg_true(float* x, const float* div, const int32_t* mat, int nx, int ny, int nz, float dx, float dt,
+                 float rho, double tol, int max_iter, int max_restarts, int mat_fluid, int mat_air,
+                 double* relres_out, int* restarts_out) {
+    if (relres_out) *relres_out = 0.0;
+    if (restarts_out) *restarts_out = 0;
+    if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
+    PcgState S;
+    if (pcg_state_init(&S, div, mat, nx, ny, nz, dx, dt, rho, mat_fluid, mat_air) != 0) {
+        pcg_state_free(&S);
+        return -1;
+    }
+    memset(x, 0, sizeof(float) * (size_t)S.n);
+    const double bsq = dot_f(S.b, S.b, S.n);
+    int iter = 0, restarts = 0;
+    if (bsq == 0.0) goto done;
+    const double stop = tol * tol * bsq;
+    double rz = pcg_cold_start(&S);
+    for (;;) {
+        while (iter < max_iter) {
+            const double pq = pcg_spmv_pass(&S);
+            const double alpha = pq != 0.0 ? rz / pq : 0.0;
+            double rr, rzn;
+            pcg_update_pass(&S, x, (float)alpha, &rr, &rzn);
+            ++iter;
+            if (rr < stop) break;
+            const double beta = rz != 0.0 ? rzn / rz : 0.0;
+            pcg_direction_pass(&S, (float)beta);
+            rz = rzn;
+        }
+        const double rr_true = pcg_true_residual(&S, x, 0, NULL);
+        if (relres_out) *relres_out = sqrt(rr_true / bsq);
+        if (rr_true <= stop || restarts >= max_restarts || iter >= max_iter) break;
+        ++restarts;
+        pcg_true_residual(&S, x, 1, &rz);
+    }
+done:
+    if (restarts_out) *restarts_out = restarts;
+    pcg_state_free(&S);
     return iter;
 }

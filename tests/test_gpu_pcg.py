@@ -230,3 +230,33 @@ def test_building_blocks(simcore, torch_gpu):
     _lib.call("bs_zero_buffer_launch", yy.data_ptr(), x.size)
     torch.cuda.synchronize()
     assert not host(yy).any()
+
+
+@pytest.mark.parametrize("name", ["small", "dev"])
+def test_pcg_solve_matches_oracle_true_residual_pcg(simcore, torch_gpu, name):
+    """bs_pcg_solve against the oracle's restatement of the same algorithm
+    (oracle.pcg_true: every-iteration test, FP64 true residual, residual
+    replacement): the same iteration and restart counts, and solutions whose
+    difference is far below their error against the FP64 solution."""
+    torch = torch_gpu
+    cfg, mat, div, dt, rho = _pot(name)
+    shape = cfg.shape
+    ws = torch.empty(simcore.pcg_workspace_bytes(*shape), dtype=torch.uint8, device="cuda")
+    x = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    mat_d, div_d = dev(torch, mat), dev(torch, div)
+    st = simcore.pcg_solve(x.data_ptr(), div_d.data_ptr(), mat_d.data_ptr(), *shape, cfg.dx, dt, rho, 1e-6, 20000,
+                           ws.data_ptr(), ws.numel())
+    torch.cuda.synchronize()
+    xo, it, rst, rel = oracle.pcg_true(div, mat, cfg.dx, dt, rho, 1e-6, 20000)
+    xg = host(x)
+    b = oracle.rhs_f64(div, mat, cfg.dx, dt, rho)
+    x64, _ = oracle.pcg_f64(b, mat, tol=1e-12)
+    fl = mat == 0
+    nrm = np.linalg.norm(x64[fl])
+    d = np.linalg.norm((xg - xo)[fl]) / nrm
+    e = np.linalg.norm((xg - x64)[fl]) / nrm
+    print(f"[{name}] gpu {st}; oracle {it} its {rst} restarts relres {rel:.3e}; |gpu-oracle| {d:.3e}, "
+          f"bits equal {same_bits(xg, xo)}; error vs FP64 {e:.3e}")
+    assert st["iterations"] == it and st["restarts"] == rst
+    assert abs(st["relres_true"] - rel) <= 1e-3 * rel
+    assert d < 1e-3 * e

@@ -290,3 +290,36 @@ def test_residual_oracle_matches_reference_restatement():
     want = np.abs(Ap - b).max() / np.abs(b).max()
     _, got = oracle.residual_f64(p, oracle.rhs_f64(div, mat, dx, dt, rho), mat)
     assert abs(got - want) <= 1e-6 * want
+
+
+def test_pcg_true_residual_semantics():
+    """oracle.pcg_true (the semantics of bs_pcg_solve): k iterations with no
+    test reached are the reference-semantics PCG's k iterations bit for bit
+    (the same passes, only the stopping test differs); to tol it stops at a
+    true FP64 relative residual <= tol (the oracle's own residual), restarting
+    from b - A x when the recursive residual has drifted; the FP64 PCG's
+    solution of the same system agrees to what that residual allows."""
+    from boilsim.fixtures import CONFIGS, DT, RHO, pot_mask, seeded_velocities
+
+    cfg = CONFIGS["small"]
+    mat = pot_mask(*cfg.shape)
+    ux, uy, uz = seeded_velocities(*cfg.shape, seed=11, plume=False)
+    div = oracle.divergence(ux, uy, uz, mat, cfg.dx)
+    for k in (1, 7, 40):
+        xt, it, rst, _ = oracle.pcg_true(div, mat, cfg.dx, DT, RHO, 0.0, k, max_restarts=0)
+        xr, itr, _ = oracle.pcg_ref(div, mat, cfg.dx, DT, RHO, 0.0, k)
+        assert it == itr == k and rst == 0
+        assert np.array_equal(xt.view(np.uint32), xr.view(np.uint32))
+    # b as the solver forms it: (-rho dx dx / dt) div rounded to FP32 per cell
+    f = np.float32
+    bscale = ((f(-RHO) * f(cfg.dx)) * f(cfg.dx)) / f(DT)
+    b = np.where(mat == 0, (bscale * div).astype(f), f(0)).astype(np.float64)
+    for tol in (1e-4, 1e-6, 1e-7):
+        x, it, rst, rel = oracle.pcg_true(div, mat, cfg.dx, DT, RHO, tol, 20000)
+        rel2, _ = oracle.residual_f64(x, b, mat)
+        print(f"tol {tol:g}: {it} iterations, {rst} restarts, relres {rel:.3e} (oracle {rel2:.3e})")
+        assert rel <= tol and rel2 <= 1.01 * tol
+        assert abs(rel2 - rel) <= 1e-3 * rel2 + 1e-12
+    x64, _ = oracle.pcg_f64(b, mat, tol=1e-12)
+    fl = mat == 0
+    assert np.linalg.norm((x - x64)[fl]) / np.linalg.norm(x64[fl]) < 1e-3
