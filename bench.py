@@ -77,6 +77,18 @@ TRAFFIC_JACOBI_PAIR = 14.207
 TRAFFIC_PASSES = {"fcg_spmv_dir_kernel": 28.27, "pre2_restrict_kernel<true>": 23.36, "prolong_post_kernel": 16.48}
 
 
+# The same flexible CG and V-cycle with non-learned solves of the 4x-coarsened
+# level (preconditioner kinds of bs_fcg_solve).
+ABLATIONS = {"mg-jacobi": 3, "mg-chebyshev60": 5, "mg-deep": 6}
+ABLATION_DOC = {
+    "mg-jacobi": "one Jacobi sweep + the 20 Chebyshev polish steps (less work than the network)",
+    "mg-chebyshev60": "one Jacobi sweep + 60 Chebyshev steps over [0.001, 2] (the best non-learned coarse solve "
+                      "measured; more work than the network + its 20-step polish)",
+    "mg-deep": "two more geometric levels (55x55x35, 28x28x18) with Chebyshev smoothing and a 24-iteration "
+               "Jacobi-PCG bottom solve, + the 20 polish steps",
+}
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -484,7 +496,7 @@ def main():
 
     import sim_core
     from boilsim.fixtures import CONFIGS, DT, RHO, pot_mask, seeded_velocities
-    from boilsim.learned import PRECOND_MG_JACOBI, PRECOND_MG_UNET, default_net
+    from boilsim.learned import PRECOND_MG_UNET, default_net
 
     if sim_core.cuda is None:
         raise RuntimeError("libboilsim_hip.so not loadable: run __graft_entry__.build() first")
@@ -506,7 +518,7 @@ def main():
     net = default_net()
     ws_pcg = torch.empty(core.pcg_workspace_bytes(nx, ny, nz), dtype=torch.uint8, device="cuda")
     ws_f = {k: torch.empty(core.fcg_workspace_bytes(nx, ny, nz, k), dtype=torch.uint8, device="cuda")
-            for k in (PRECOND_MG_UNET, PRECOND_MG_JACOBI)}
+            for k in set(ABLATIONS.values()) | {PRECOND_MG_UNET}}
     torch.cuda.synchronize()
     ptr = lambda t: t.data_ptr()
 
@@ -519,7 +531,7 @@ def main():
             st = core.pcg_solve(ptr(p), ptr(div), ptr(gmat), nx, ny, nz, cfg.dx, DT, RHO, args.tol, 20000,
                                 ptr(ws_pcg), ws_pcg.numel())
         else:
-            k = PRECOND_MG_UNET if solver == "learned" else PRECOND_MG_JACOBI
+            k = PRECOND_MG_UNET if solver == "learned" else ABLATIONS[solver]
             st = core.fcg_solve(ptr(p), ptr(div), ptr(gmat), nx, ny, nz, cfg.dx, DT, RHO, args.tol, 5000, k,
                                 ptr(net.params), ptr(net.packed), ptr(ws_f[k]), ws_f[k].numel())
         es1.record()
@@ -575,10 +587,14 @@ def main():
     nan = float("nan")
     ms_l, t_l, st_l, md_l = leg("learned") if "learned" in legs else (nan, [], None, None)
     ms_p, t_p, st_p, md_p = leg("pcg") if "pcg" in legs else (nan, [], None, None)
-    abl = None
+    abl = {}
     if "ablation" in legs:
-        ms_a, _, st_a, _ = leg("mg-jacobi")
-        abl = {"ms_per_projection": ms_a, "iterations": st_a["iterations"], "relres_true": st_a["relres_true"]}
+        for name in ABLATIONS:
+            ms_a, _, st_a, _ = leg(name)
+            abl[name] = {"ms_per_projection": ms_a, "solve_ms_median": st_a["solve_ms_median"],
+                         "iterations": st_a["iterations"], "relres_true": st_a["relres_true"],
+                         "converged": st_a["converged"], "fallback": st_a["fallback"],
+                         "coarse_solve": ABLATION_DOC[name]}
 
     gdiv = torch.empty_like(p)  # the initial divergence (the solves' RHS)
     core.compute_divergence(ptr(gdiv), ptr(ux0), ptr(uy0), ptr(uz0), ptr(gmat), nx, ny, nz, cfg.dx)
@@ -686,7 +702,7 @@ def main():
             "iterations": st_l["iterations"], "relres_true": st_l["relres_true"],
             "fallback": st_l["fallback"], "max_div_after": md_l, "times_ms": t_l},
         "speedup_vs_plain_pcg": fin(ms_p / ms_l),
-        "ablation_mg_jacobi_coarse": abl,
+        "ablations_no_network": abl or None,
         "max_div_before": maxdiv0,
         "roofline": None if not passes else {
             "bound": "hbm", "achieved": passes[K0]["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",

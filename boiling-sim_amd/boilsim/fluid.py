@@ -31,7 +31,7 @@ import numpy as np
 
 from .geometry import MAT_AIR, MAT_FLUID
 
-VALID_SOLVERS = ("jacobi", "cg-rust", "pcg-hip", "lpcg-hip")
+VALID_SOLVERS = ("jacobi", "cg-rust", "pcg-hip", "lpcg-hip", "mgpcg-hip")
 
 
 @dataclass
@@ -164,20 +164,21 @@ def _get_solve_workspace(core, shape, device="cuda"):
 
 
 def _select_pressure_solver() -> str:
-    """"jacobi", "rust-jacobi", "cg-rust", "pcg-hip" or "lpcg-hip"
-    (fluid.py:585-609, extended)."""
+    """"jacobi", "rust-jacobi", "cg-rust", "pcg-hip", "lpcg-hip" or
+    "mgpcg-hip".  The reference's contract (fluid.py:585-609): an unknown
+    BOILINGSIM_PRESSURE_SOLVER, or BOILINGSIM_USE_RUST_PRESSURE=1 together
+    with a solver other than jacobi, raises ValueError."""
     use_rust_jacobi = os.environ.get("BOILINGSIM_USE_RUST_PRESSURE") == "1"
     solver = os.environ.get("BOILINGSIM_PRESSURE_SOLVER", "jacobi").lower()
     if solver not in VALID_SOLVERS:
         raise ValueError(
-            f"BOILINGSIM_PRESSURE_SOLVER={solver!r} is not recognised; valid values are "
-            "'jacobi' (default), 'cg-rust', 'pcg-hip' or 'lpcg-hip'."
+            f"unknown pressure solver BOILINGSIM_PRESSURE_SOLVER={solver!r}; expected one of "
+            + ", ".join(repr(v) for v in VALID_SOLVERS) + " ('jacobi' when unset)"
         )
     if use_rust_jacobi and solver != "jacobi":
         raise ValueError(
-            "BOILINGSIM_USE_RUST_PRESSURE=1 and BOILINGSIM_PRESSURE_SOLVER="
-            f"{solver} are mutually exclusive -- they route through different code paths. "
-            "Pick exactly one."
+            f"conflicting pressure solver settings: BOILINGSIM_USE_RUST_PRESSURE=1 selects the fused "
+            f"Jacobi solve, BOILINGSIM_PRESSURE_SOLVER={solver} another solver; set only one of them"
         )
     if solver != "jacobi":
         return solver
@@ -209,6 +210,10 @@ def _solve(core, solver, grid: Grid, ws: FluidWorkspace, cfg: ScenarioConfig, dt
         st = core.pcg_solve(grid.p.data_ptr(), ws.div_u.data_ptr(), grid.mat.data_ptr(), nx, ny, nz,
                             dx, dt, rho, float(cfg.solver.pressure_tol), mi, buf.data_ptr(),
                             buf.numel(), warm_start=False)
+    elif solver == "mgpcg-hip":
+        from .learned import PRECOND_MG_CHEB, learned_pcg_solve
+
+        st = learned_pcg_solve(core, grid, ws, dt, rho, float(cfg.solver.pressure_tol), mi, precond=PRECOND_MG_CHEB)
     else:  # lpcg-hip
         from .learned import learned_pcg_solve
 

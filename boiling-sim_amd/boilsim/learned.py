@@ -26,6 +26,10 @@ _UNIT = (1.0, -1.0, 1.0)
 K_CHOICES = (0, 1, 2, 4, 8, 16, 32, 64, 128)
 
 PRECOND_UNET, PRECOND_MG_UNET, PRECOND_MG_JACOBI = 1, 2, 3
+# The same V-cycle without the network: 60 Chebyshev steps on the coarsest
+# level (5), or two more geometric levels and a bottom PCG (6).
+PRECOND_MG_CHEB, PRECOND_MG_DEEP = 5, 6
+NETWORK_KINDS = (PRECOND_UNET, PRECOND_MG_UNET, 4)
 
 
 def grf_torch(torch, fluid, seed: int, ell: float | None = None, ell_range=(4.0, 16.0)):
@@ -199,8 +203,10 @@ def learned_pcg_solve(core, grid, ws, dt, rho, tol, max_iter, net: UNet | None =
     says a cap ended an unconverged solve."""
     import torch
 
-    if net is None:
+    if net is None and precond in NETWORK_KINDS:
         net = default_net()
+    params = net.params.data_ptr() if net is not None else 0
+    packed = net.packed.data_ptr() if net is not None else 0
     shape = grid.shape
     key = (shape, precond)
     if _FCG_WS["key"] != key:
@@ -209,8 +215,7 @@ def learned_pcg_solve(core, grid, ws, dt, rho, tol, max_iter, net: UNet | None =
         _FCG_WS["key"] = key
     buf = _FCG_WS["buf"]
     st = core.fcg_solve(grid.p.data_ptr(), ws.div_u.data_ptr(), grid.mat.data_ptr(), *shape, grid.dx, dt, rho,
-                        tol, max_iter, precond, net.params.data_ptr(), net.packed.data_ptr(), buf.data_ptr(),
-                        buf.numel(), warm_start=warm_start)
+                        tol, max_iter, precond, params, packed, buf.data_ptr(), buf.numel(), warm_start=warm_start)
     used = st["iterations"] + st["pcg_iterations"]
     if st["converged"] or not refine or used >= max_iter:
         if not st["converged"]:
@@ -224,7 +229,7 @@ def learned_pcg_solve(core, grid, ws, dt, rho, tol, max_iter, net: UNet | None =
     x64, rbuf = _REFINE_WS["x64"], _REFINE_WS["buf"]
     x64.copy_(grid.p)
     rs = core.fcg_refine_solve(x64.data_ptr(), ws.div_u.data_ptr(), grid.mat.data_ptr(), *shape, grid.dx, dt, rho,
-                               tol, precond, net.params.data_ptr(), net.packed.data_ptr(), rbuf.data_ptr(),
+                               tol, precond, params, packed, rbuf.data_ptr(),
                                rbuf.numel(), max_outer=MAX_REFINE_STEPS, max_inner=max_iter - used,
                                warm_start=True, x32_ptr=grid.p.data_ptr())
     out = dict(st)

@@ -67,6 +67,27 @@ constexpr double kL1ChebLo = 0.02;
 // iterations; [0.1, 2] or [0.5, 2] and post weights 0.7 or 0.9 are worse).
 constexpr double kPreLo = 0.3;
 constexpr float kPostW = 1.0f;
+// Non-learned coarse solves of the 4x level, for comparison with the network
+// (coarse kinds 3 and 4, preconditioner kinds 5 and 6):
+//   kind 5: one damped-Jacobi sweep and kAblChebSteps Chebyshev steps over
+//           [kAblChebLo, 2].  Measured at 440x440x280 (velocity seeds 11, 21,
+//           31, 41, 51; flexible-CG iterations / solve ms): 60 steps over
+//           [0.001, 2] 13/14/13/13/13 and 19.4-20.8 ms, against 15 in all five
+//           and 21.4-21.5 ms for the network with its 20-step polish; 40 steps
+//           over [0.001, 2] 15-16; 80 over [0.0005, 2] 12-13 at 19.4-21.2 ms;
+//           60 or 80 over [0.005, 2] 15-17.  The network followed by 40 steps
+//           over [0.001, 2] takes 13-15 iterations (20.2-23.3 ms): given the
+//           stronger polish, it no longer beats a single Jacobi sweep;
+//   kind 6: two more geometric levels below the 4x one (each: two-step
+//           Chebyshev pre-smoothing, restriction; prolongation and
+//           kDeepPostSteps Chebyshev post-smoothing steps), a bottom solve
+//           of kBotIters Jacobi-PCG iterations in one workgroup, then the
+//           same 20-step polish as the network path.
+constexpr int kAblChebSteps = 60;
+constexpr double kAblChebLo = 0.001;
+constexpr int kDeepPostSteps = 4;
+constexpr int kBotThreads = 1024, kBotMax = 16384, kBotPer = kBotMax / kBotThreads;
+constexpr int kBotIters = 24;
 
 __device__ __forceinline__ float inv_diag(uint8_t code) {
     const int dg = 8 - (int)code;
@@ -1090,6 +1111,104 @@ __global__ __launch_bounds__(256) void prolong_post_kernel(int nx, int ny, int n
 
 // z_out = z_in + P ec from level LC to its refinement LF (tiles TF, vector
 // width vf); z_out may be z_in.
+// Sum over the workgroup of bottom_pcg_kernel (1024 threads, 16 waves): every
+// thread gets the total; `slot` alternates so a reduction never overwrites
+// partials another thread may still be reading.
+__device__ __forceinline__ double bot_sum(double v, double* red, int slot) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0) red[slot * 16 + (threadIdx.x >> 6)] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) s += red[slot * 16 + q];
+    return s;
+}
+
+// Bottom solve of the deep coarse V-cycle: `its` Jacobi-PCG iterations on
+// A e = r from e = 0 over a level of at most kBotMax cells, in one
+// workgroup: the search direction and the codes live in LDS (the SpMV's
+// neighbours), r, e and the iteration's Ap and z in registers (cells t,
+// t + 1024, ... of thread t); dot products in FP64.  Stops early when p.Ap
+// is no longer positive (r = 0).
+__global__ __launch_bounds__(kBotThreads) void bottom_pcg_kernel(MLvl L, const uint8_t* __restrict__ code,
+                                                                 const float* __restrict__ r_in,
+                                                                 float* __restrict__ e_out, int its,
+                                                                 const int* __restrict__ done) {
+    if (done && *done) return;
+    __shared__ float sp[kBotMax];
+    __shared__ uint8_t sc[kBotMax];
+    __shared__ double red[2 * 16];
+    const int t = threadIdx.x;
+    const int n = (int)L.n, nz = L.nz, pl = L.ny * L.nz;
+    float r[kBotPer], e[kBotPer];
+    double rz = 0.0;
+#pragma unroll
+    for (int q = 0; q < kBotPer; ++q) {
+        const int c = t + q * kBotThreads;
+        e[q] = 0.0f;
+        r[q] = 0.0f;
+        if (c < n) {
+            const uint8_t cc = code[c];
+            sc[c] = cc;
+            r[q] = cc >= kCodeFluid0 ? r_in[c] : 0.0f;
+            const float z = precond(r[q], cc);
+            sp[c] = z;
+            rz += (double)r[q] * (double)z;
+        }
+    }
+    rz = bot_sum(rz, red, 1);  // its barrier also publishes sp and sc
+    for (int it = 0; it < its; ++it) {
+        float ap[kBotPer];
+        double pap = 0.0;
+#pragma unroll
+        for (int q = 0; q < kBotPer; ++q) {
+            const int c = t + q * kBotThreads;
+            ap[q] = 0.0f;
+            if (c >= n || sc[c] < kCodeFluid0) continue;
+            const int i = c / pl, rem = c - i * pl, j = rem / nz, k = rem - j * nz;
+            const float ps = sp[c];
+            float s = (i > 0 ? nbr_term(sc[c - pl], sp[c - pl], ps) : 0.0f) +
+                      (i < L.nx - 1 ? nbr_term(sc[c + pl], sp[c + pl], ps) : 0.0f);
+            s = s + (j > 0 ? nbr_term(sc[c - nz], sp[c - nz], ps) : 0.0f);
+            s = s + (j < L.ny - 1 ? nbr_term(sc[c + nz], sp[c + nz], ps) : 0.0f);
+            s = s + (k > 0 ? nbr_term(sc[c - 1], sp[c - 1], ps) : 0.0f);
+            s = s + (k < nz - 1 ? nbr_term(sc[c + 1], sp[c + 1], ps) : 0.0f);
+            ap[q] = 6.0f * ps - s;
+            pap += (double)ps * (double)ap[q];
+        }
+        pap = bot_sum(pap, red, 0);
+        if (!(pap > 0.0)) break;  // the same decision in every thread
+        const float al = (float)(rz / pap);
+        float z[kBotPer];
+        double rzn = 0.0;
+#pragma unroll
+        for (int q = 0; q < kBotPer; ++q) {
+            const int c = t + q * kBotThreads;
+            z[q] = 0.0f;
+            if (c >= n) continue;
+            e[q] = e[q] + al * sp[c];
+            r[q] = r[q] - al * ap[q];
+            z[q] = precond(r[q], sc[c]);
+            rzn += (double)r[q] * (double)z[q];
+        }
+        rzn = bot_sum(rzn, red, 1);
+        const float be = rz > 0.0 ? (float)(rzn / rz) : 0.0f;
+        rz = rzn;
+        // every thread has finished reading sp (the barrier above)
+#pragma unroll
+        for (int q = 0; q < kBotPer; ++q) {
+            const int c = t + q * kBotThreads;
+            if (c < n) sp[c] = z[q] + be * sp[c];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < kBotPer; ++q) {
+        const int c = t + q * kBotThreads;
+        if (c < n) e_out[c] = e[q];
+    }
+}
+
 static int prolong_level(hipStream_t st, const Tiles& TF, int vf, const MLvl& LF, const MLvl& LC,
                          const Tiles& TC, const uint8_t* fcode, const float* ec, const float* z_in, float* z_out,
                          const int* done);
@@ -1097,11 +1216,14 @@ static int prolong_level(hipStream_t st, const Tiles& TF, int vf, const MLvl& LF
 static inline size_t al(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct MgWs {
-    MLvl L0, L1, L2;
+    MLvl L0, L1, L2, L3, L4;
     float* zt0;               // fine temporary
     uint8_t *code1, *code2;
     int *mat1, *mat2;
     float *r1, *z1, *zt1, *r2, *e2, *zt2, *rs2, *en2;
+    uint8_t *code3, *code4;  // the two levels below the 4x one (coarse kind 4)
+    int *mat3, *mat4;
+    float *r3, *z3, *zt3, *r4, *e4;
     unsigned int* nfl;
     double* part;  // 2 x kStepWg partials of the optimal-step products
     void* unet;
@@ -1114,6 +1236,8 @@ static MgWs carve_mg(void* base, int nx, int ny, int nz) {
     W.L0 = mk(nx, ny, nz);
     W.L1 = mk((nx + 1) / 2, (ny + 1) / 2, (nz + 1) / 2);
     W.L2 = mk((W.L1.nx + 1) / 2, (W.L1.ny + 1) / 2, (W.L1.nz + 1) / 2);
+    W.L3 = mk((W.L2.nx + 1) / 2, (W.L2.ny + 1) / 2, (W.L2.nz + 1) / 2);
+    W.L4 = mk((W.L3.nx + 1) / 2, (W.L3.ny + 1) / 2, (W.L3.nz + 1) / 2);
     size_t off = 0;
     auto take = [&](size_t b) -> char* {
         char* p = base ? (char*)base + off : nullptr;
@@ -1133,6 +1257,15 @@ static MgWs carve_mg(void* base, int nx, int ny, int nz) {
     W.zt2 = (float*)take(W.L2.n * 4);
     W.rs2 = (float*)take(W.L2.n * 4);
     W.en2 = (float*)take(W.L2.n * 4);
+    W.code3 = (uint8_t*)take(W.L3.n);
+    W.mat3 = (int*)take(W.L3.n * 4);
+    W.r3 = (float*)take(W.L3.n * 4);
+    W.z3 = (float*)take(W.L3.n * 4);
+    W.zt3 = (float*)take(W.L3.n * 4);
+    W.code4 = (uint8_t*)take(W.L4.n);
+    W.mat4 = (int*)take(W.L4.n * 4);
+    W.r4 = (float*)take(W.L4.n * 4);
+    W.e4 = (float*)take(W.L4.n * 4);
     W.nfl = (unsigned int*)take(64);
     W.part = (double*)take(2 * kStepWg * sizeof(double));
     W.unet = take(unet_ws_bytes(W.L2.nx, W.L2.ny, W.L2.nz));
@@ -1164,8 +1297,9 @@ void scale_to_optimal(hipStream_t st, int nx, int ny, int nz, const uint8_t* cod
 __global__ void zero_u32(unsigned int* p) { if (threadIdx.x == 0) *p = 0u; }
 
 int mg_setup(hipStream_t st, void* ws, int nx, int ny, int nz, const int* mat, int mf, int ma,
-             const uint8_t* fcode_in, double* n_fluid_coarse) {
+             const uint8_t* fcode_in, double* n_fluid_coarse, bool deep) {
     MgWs W = carve_mg(ws, nx, ny, nz);
+    if (deep && W.L4.n > kBotMax) return (int)hipErrorInvalidValue;
     // The fine codes (given, or formed in a temporary: zt0 is free during
     // set-up), then the two coarse levels' classes and codes.
     const uint8_t* fcode = fcode_in;
@@ -1182,6 +1316,12 @@ int mg_setup(hipStream_t st, void* ws, int nx, int ny, int nz, const int* mat, i
     coarse_class_kernel<<<grid_for(W.L2.n), 256, 0, st>>>(W.L1, W.L2, W.code1, W.mat2);
     // the fluid count's atomics: 256 workgroups, one each
     codes_from_mat_kernel<<<256, 256, 0, st>>>(W.L2, W.mat2, 0, 2, W.code2, W.nfl);
+    if (deep) {
+        coarse_class_kernel<<<grid_for(W.L3.n), 256, 0, st>>>(W.L2, W.L3, W.code2, W.mat3);
+        codes_from_mat_kernel<<<grid_for(W.L3.n), 256, 0, st>>>(W.L3, W.mat3, 0, 2, W.code3, nullptr);
+        coarse_class_kernel<<<grid_for(W.L4.n), 256, 0, st>>>(W.L3, W.L4, W.code3, W.mat4);
+        codes_from_mat_kernel<<<grid_for(W.L4.n), 256, 0, st>>>(W.L4, W.mat4, 0, 2, W.code4, nullptr);
+    }
     BS_RETURN_IF_ERR(hipGetLastError());
     unsigned int h = 0;
     BS_RETURN_IF_ERR(hipMemcpyAsync(&h, W.nfl, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -1331,6 +1471,29 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
     auto polish = [&]() { chebyshev(st, T2, v2, W.code2, W.r2, W.e2, W.zt2, kChebSteps, kChebLo, done); };
     if (coarse_kind == 0) {
         jacobi0_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, done);
+        polish();
+    } else if (coarse_kind == 3) {
+        jacobi0_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, done);
+        chebyshev(st, T2, v2, W.code2, W.r2, W.e2, W.zt2, kAblChebSteps, kAblChebLo, done);
+    } else if (coarse_kind == 4) {
+        if (W.L4.n > kBotMax) return (int)hipErrorInvalidValue;
+        const int v3 = vec_for(W.L3.nz, {W.r3, W.z3, W.zt3, W.code3});
+        const Tiles T3 = make_tiles(W.L3.nx, W.L3.ny, W.L3.nz, v3);
+        const Tiles C3 = make_tiles(W.L3.nx, W.L3.ny, W.L3.nz, 1);
+        const Tiles C4 = make_tiles(W.L4.nx, W.L4.ny, W.L4.nz, 1);
+        // down: 4x -> 8x -> 16x, two-step Chebyshev pre-smoothing on each
+        rc = pre_restrict(st, T2, v2, W.L2, W.L3, C3, W.code2, W.r2, W.e2, W.r3, done, 0, 0, K1);
+        if (rc != hipSuccess) return rc;
+        rc = pre_restrict(st, T3, v3, W.L3, W.L4, C4, W.code3, W.r3, W.z3, W.r4, done, 0, 0, K1);
+        if (rc != hipSuccess) return rc;
+        bottom_pcg_kernel<<<1, kBotThreads, 0, st>>>(W.L4, W.code4, W.r4, W.e4, kBotIters, done);
+        // up: prolongation and Chebyshev post-smoothing on the 8x level, then
+        // the 4x level's prolongation and the same polish as the network path
+        rc = prolong_level(st, T3, v3, W.L3, W.L4, C4, W.code3, W.e4, W.z3, W.z3, done);
+        if (rc != hipSuccess) return rc;
+        chebyshev(st, T3, v3, W.code3, W.r3, W.z3, W.zt3, kDeepPostSteps, kL1ChebLo, done);
+        rc = prolong_level(st, T2, v2, W.L2, W.L3, C3, W.code2, W.z3, W.e2, W.e2, done);
+        if (rc != hipSuccess) return rc;
         polish();
     } else if (coarse_kind == 1) {
         rc = unet_apply(st, W.unet, W.L2.nx, W.L2.ny, W.L2.nz, params, packed, W.r2, W.code2, n_fluid_coarse,

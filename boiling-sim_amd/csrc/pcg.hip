@@ -1400,7 +1400,9 @@ static int fcg_precond(hipStream_t st, const FcgBuffers& F, const float* r, floa
         fcg_dots_kernel<<<kUNetPartWg, 256, 0, st>>>(n, r, z, z_old, F.partZ, B.ctrl);
         return (int)hipGetLastError();
     }
-    const int coarse = F.kind == 2 ? 1 : (F.kind == 4 ? 2 : 0);
+    // coarse solve of the 4x level: kinds 2/4 the network, 3 one Jacobi sweep
+    // + 20 Chebyshev steps, 5 + 60 steps, 6 two more geometric levels
+    const int coarse = F.kind == 2 ? 1 : F.kind == 4 ? 2 : F.kind == 5 ? 3 : F.kind == 6 ? 4 : 0;
     (void)n;
     MgFold fold;
     if (r_out) {
@@ -1666,6 +1668,9 @@ static size_t partz_slots(int nx, int ny, int nz) {
     return 2 * (g > kUNetPartWg ? g : (size_t)kUNetPartWg);
 }
 
+// Preconditioner kinds that run the network.
+static bool needs_unet(int kind) { return kind == 1 || kind == 2 || kind == 4; }
+
 static size_t precond_ws_bytes(int nx, int ny, int nz, int kind) {
     if (kind == 1) return unet_ws_bytes(nx, ny, nz);
     if (kind >= 2) return mg_ws_bytes(nx, ny, nz);
@@ -1692,8 +1697,8 @@ extern "C" int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx
                             void* workspace, size_t workspace_bytes, hipStream_t st, bs_fcg_stats* stats) {
     if (stats) memset(stats, 0, sizeof(*stats));
     if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
-    if (precond < 1 || precond > 4) return (int)hipErrorInvalidValue;
-    if (precond != 3 && (!unet_params || !unet_packed)) return (int)hipErrorInvalidValue;
+    if (precond < 1 || precond > 6) return (int)hipErrorInvalidValue;
+    if (needs_unet(precond) && (!unet_params || !unet_packed)) return (int)hipErrorInvalidValue;
     if (workspace_bytes < bs_fcg_workspace_bytes(nx, ny, nz, precond)) return (int)hipErrorInvalidValue;
     BS_RETURN_IF_ERR((hipError_t)ensure_poll());
     const size_t n = (size_t)nx * ny * nz;
@@ -1739,7 +1744,7 @@ extern "C" int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx
         F.n_fluid = (double)(unsigned)g_poll.final->pad;
     }
     if (precond >= 2) {
-        const int rc = mg_setup(st, F.pws, nx, ny, nz, mat, mat_fluid, mat_air, B.code, &F.n_fluid);
+        const int rc = mg_setup(st, F.pws, nx, ny, nz, mat, mat_fluid, mat_air, B.code, &F.n_fluid, precond == 6);
         if (rc != hipSuccess) return rc;
     }
     fcg_prepare_kernel<<<1, 64, 0, st>>>(B.ctrl, 0);
@@ -2204,8 +2209,8 @@ extern "C" int bs_fcg_refine_solve(double* x, float* x32, const float* div_u, co
                                    size_t workspace_bytes, hipStream_t st, bs_refine_stats* stats) {
     if (stats) memset(stats, 0, sizeof(*stats));
     if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
-    if (precond < 1 || precond > 4 || max_outer < 0) return (int)hipErrorInvalidValue;
-    if (precond != 3 && (!unet_params || !unet_packed)) return (int)hipErrorInvalidValue;
+    if (precond < 1 || precond > 6 || max_outer < 0) return (int)hipErrorInvalidValue;
+    if (needs_unet(precond) && (!unet_params || !unet_packed)) return (int)hipErrorInvalidValue;
     if (workspace_bytes < bs_fcg_refine_workspace_bytes(nx, ny, nz, precond)) return (int)hipErrorInvalidValue;
     BS_RETURN_IF_ERR((hipError_t)ensure_poll());
     const size_t n = (size_t)nx * ny * nz;
@@ -2243,7 +2248,7 @@ extern "C" int bs_fcg_refine_solve(double* x, float* x32, const float* div_u, co
         BS_RETURN_IF_ERR(hipStreamSynchronize(st));
         F.n_fluid = (double)h.n_fluid;
     } else {
-        rc = mg_setup(st, F.pws, nx, ny, nz, mat, mat_fluid, mat_air, B.code, &F.n_fluid);
+        rc = mg_setup(st, F.pws, nx, ny, nz, mat, mat_fluid, mat_air, B.code, &F.n_fluid, precond == 6);
         if (rc != hipSuccess) return rc;
     }
     // After an inner flexible CG stalls (fallback), the remaining steps run

@@ -20,15 +20,18 @@ int unet_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const float* pa
                const float* r, const uint8_t* code, double n_fluid, const double* rr_part, unsigned G,
                float* z, const float* z_old, double* rz_part, const int* done);
 
-// Multigrid-augmented preconditioner: damped-Jacobi V-cycle on the fine grid
-// and its 2x coarsening, with the 4x-coarsened level solved by the U-Net
-// (coarse_kind 1) or by damped Jacobi sweeps (coarse_kind 0).
+// Multigrid-augmented preconditioner: a V-cycle on the fine grid and its 2x
+// coarsening, with the 4x-coarsened level solved by the U-Net (coarse_kind 1,
+// 2) or without it: one Jacobi sweep + 20 Chebyshev steps (0), + 60 (3), or
+// two more geometric levels and a one-workgroup PCG bottom solve (4).
 size_t mg_ws_bytes(int nx, int ny, int nz);
 // Builds the coarse classes and codes from the fine mask (or from its fine
 // class codes fcode when not null); returns the fluid count of the coarsest
 // level through *n_fluid_coarse (synchronises once).
+// deep: also the two levels below the 4x one (coarse kind 4; an error when
+// the lowest has more than 16384 cells).
 int mg_setup(hipStream_t st, void* ws, int nx, int ny, int nz, const int* mat, int mf, int ma, const uint8_t* fcode,
-             double* n_fluid_coarse);
+             double* n_fluid_coarse, bool deep = false);
 // The flexible CG's residual update folded into the V-cycle's first fine
 // pass: the pass reads r_old (mg_apply's r) and Ap, works on and writes
 // r_out = r_old - (*alpha) Ap (r_out must not alias r_old) and one r.r

@@ -280,8 +280,13 @@ size_t bs_fcg_workspace_bytes(int nx, int ny, int nz, int precond);
  * correction polished by Chebyshev steps, 3 = the same V-cycle with one
  * Jacobi sweep and the same polish on the coarsest grid (no network), 4 = as
  * 2 with Jacobi sweeps before the network, which then corrects their
- * residual.  A stall (no 2 % residual drop in 20 iterations) hands
- * over to plain Jacobi-PCG from the current x. */
+ * residual, 5 = as 3 with 60 Chebyshev steps (no network; more work than the
+ * network and its polish), 6 = as 2 with two more geometric levels below the
+ * coarsest grid and a one-workgroup Jacobi-PCG bottom solve in place of the
+ * network (no network; an error when the lowest level exceeds 16384 cells).
+ * unet_params and unet_packed may be null for kinds 3, 5 and 6.  A stall (no
+ * 2 % residual drop in 20 iterations) hands over to plain Jacobi-PCG from the
+ * current x. */
 int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx, int ny, int nz, float dx, float dt,
                  float rho, int mat_fluid, int mat_air, double tol, int max_iter, int warm_start,
                  int precond, const float* unet_params, const void* unet_packed, void* workspace,

@@ -91,6 +91,7 @@ def test_error_string_roundtrip():
     ({"BOILINGSIM_PRESSURE_SOLVER": "CG-RUST"}, "cg-rust"),
     ({"BOILINGSIM_PRESSURE_SOLVER": "pcg-hip"}, "pcg-hip"),
     ({"BOILINGSIM_PRESSURE_SOLVER": "lpcg-hip"}, "lpcg-hip"),
+    ({"BOILINGSIM_PRESSURE_SOLVER": "mgpcg-hip"}, "mgpcg-hip"),
     ({"BOILINGSIM_USE_RUST_PRESSURE": "0", "BOILINGSIM_PRESSURE_SOLVER": "jacobi"}, "jacobi"),
 ])
 def test_solver_selection(monkeypatch, env, want):
@@ -106,6 +107,7 @@ def test_solver_selection(monkeypatch, env, want):
 @pytest.mark.parametrize("env", [
     {"BOILINGSIM_PRESSURE_SOLVER": "sor"},
     {"BOILINGSIM_USE_RUST_PRESSURE": "1", "BOILINGSIM_PRESSURE_SOLVER": "cg-rust"},
+    {"BOILINGSIM_USE_RUST_PRESSURE": "1", "BOILINGSIM_PRESSURE_SOLVER": "mgpcg-hip"},
 ])
 def test_solver_selection_errors(monkeypatch, env):
     from boilsim.fluid import _select_pressure_solver
