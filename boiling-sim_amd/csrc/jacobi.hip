@@ -361,6 +361,22 @@ static PairShape pair_shape(int nx, int ny, int nz) {
     return s;
 }
 
+// The pair kernel halves HBM traffic per sweep but re-reads two halo planes
+// per i-segment and idles k lanes on ragged k tiles.  While one sweep's
+// working set (p in, p out, div: 12 B/cell) fits the 256 MiB Infinity Cache,
+// single sweeps are served from it and are faster (200-sweep solves,
+// bit-identical results: 110x110x68 0.94 vs 1.52 ms, 300x300x192 6.26 vs
+// 8.03 ms); above it the pair kernel wins (360x360x228 12.8 vs 15.8 ms,
+// 440x440x280 20.0 vs 28.6 ms).  BOILSIM_JACOBI_PAIR=1 (=0) forces the pair
+// kernel on (off) whatever the size; both give the same bits.
+constexpr int64_t kInfinityCacheBytes = 256ll << 20;
+static bool pair_pays(int64_t cells) {
+    const char* e = getenv("BOILSIM_JACOBI_PAIR");
+    if (e && e[0] == '1') return true;
+    if (e && e[0] == '0') return false;
+    return 12 * cells > kInfinityCacheBytes;
+}
+
 // Sentinel -> 0 in place (restores the reference's zero at non-fluid cells).
 template <int VEC>
 __global__ __launch_bounds__(256) void decode_kernel(float* __restrict__ p, int64_t nvec) {
@@ -548,7 +564,7 @@ extern "C" int bs_jacobi_solve_launch(float* p, float* p_tmp, const float* div_u
         // the reference's.  The last two sweeps always run singly, so both
         // buffers end exactly as the reference leaves them.
         const int middle = n_iter >= 3 ? n_iter - 3 : 0;
-        const int pairs = vec == 4 ? (middle / 2) & ~1 : 0;
+        const int pairs = vec == 4 && pair_pays(n) ? (middle / 2) & ~1 : 0;
         if (pairs > 0) {
             const PairShape ps = pair_shape(nx, ny, nz);
             for (int q = 0; q < pairs; ++q) {

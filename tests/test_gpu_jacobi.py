@@ -139,12 +139,16 @@ def test_200_sweeps_pot(simcore, torch_gpu, name):
 @pytest.mark.parametrize("shape", [(97, 83, 68), (40, 30, 56), (33, 29, 116), (9, 7, 4), (64, 440, 280)])
 @pytest.mark.parametrize("n_iter", [5, 9, 30])
 @pytest.mark.parametrize("edge", [False, True])
-def test_two_sweep_pass_bit_exact(simcore, torch_gpu, shape, n_iter, edge):
-    """The fused driver runs its middle sweeps two per pass (nz % 4 == 0):
-    p and p_tmp still end bit-identical to the reference's one-sweep ping-pong,
-    on grids with partial (j, k) tiles, several i segments, and fluid on the
-    domain edges (out-of-bounds neighbours are air)."""
+@pytest.mark.parametrize("pairs", ["1", "0"])
+def test_two_sweep_pass_bit_exact(simcore, torch_gpu, monkeypatch, shape, n_iter, edge, pairs):
+    """The fused driver runs its middle sweeps two per pass (nz % 4 == 0) when
+    a sweep's working set exceeds the Infinity Cache; these grids are smaller,
+    so BOILSIM_JACOBI_PAIR=1 forces that kernel on (=0 off): either way p and
+    p_tmp end bit-identical to the reference's one-sweep ping-pong, on grids
+    with partial (j, k) tiles, several i segments, and fluid on the domain
+    edges (out-of-bounds neighbours are air)."""
     torch = torch_gpu
+    monkeypatch.setenv("BOILSIM_JACOBI_PAIR", pairs)
     if edge:
         mat, p0, div_u = fluid_everywhere_edge_case(5, shape)
         dx, dt, rho = 0.002, 0.001, 997.0
@@ -244,9 +248,9 @@ def test_config1_matches_committed_digests(simcore, torch_gpu):
 @pytest.mark.parametrize("name", ["dev", "prod"])
 def test_config2_matches_committed_digests(simcore, torch_gpu, name):
     """Config 2 on the device at 110x110x70 and 440x440x280 (seed-7 velocities
-    with the plume): the divergence and 200 sweeps (p and p_tmp, the
-    two-sweeps-per-pass path) hash to the committed digests of the oracle's
-    outputs, bit for bit, at the full production size."""
+    with the plume): the divergence and 200 sweeps (p and p_tmp; at
+    440x440x280 the two-sweeps-per-pass kernel, selected by size) hash to the
+    committed digests of the oracle's outputs, bit for bit."""
     import hashlib
     import json
     import os
