@@ -697,9 +697,15 @@ __global__ __launch_bounds__(256) void prolong4_kernel(Tiles F, Tiles CV, const 
 // planes: r double-buffered, z double-buffered, codes triple-buffered (the
 // residual at q-1 still reads plane q-1's codes while plane q+1's are
 // stored), which leaves one barrier per plane; the global loads run two
-// planes ahead.  The march through a segment is a chain of per-plane
-// latencies, so the pass pays off only where the level fills the chip with
-// several rounds of workgroups (the fine level).
+// planes ahead.  z1 is formed once per cell when its plane arrives (in
+// registers for the i-neighbours, in the LDS plane for the j/k ones) rather
+// than at each of its seven uses: the same product, so the same bits, at a
+// third less VALU work (the pass is issue-bound, not HBM-bound).  z1 and z
+// are set to exactly +0 at non-fluid cells, where the smoother never uses
+// them, so a neighbour term is one select (nbr_term0).  The march through a
+// segment is a chain of per-plane latencies, so the pass pays off only where
+// the level fills the chip with several rounds of workgroups (the fine
+// level).
 // ---------------------------------------------------------------------------
 constexpr int kPrTJ = 12, kPrKT = 14;
 
@@ -729,6 +735,10 @@ static PrShape pr_shape(int nx, int ny, int nz, int target) {
 static bool pr_fits(const MLvl& L, int vec) { return vec == 4 && (L.nx & 1) == 0 && (L.ny & 1) == 0 && (L.nz & 3) == 0; }
 
 __device__ __forceinline__ uint8_t byte_of(uint32_t w, int e) { return (uint8_t)((w >> (8 * e)) & 0xFFu); }
+
+// nbr_term for values that are exactly +0 at air cells (and at positions
+// outside the grid): the air case needs no select of its own.
+__device__ __forceinline__ float nbr_term0(uint8_t c, float v, float ps) { return c == kCodeSolid ? ps : v; }
 
 // FOLD: the flexible CG's residual update runs in the same pass: the kernel
 // reads r_old and Ap, works on r = r_old - alpha Ap (alpha = *alpha_p, the
@@ -784,12 +794,25 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
         if (!FOLD) return rv;
         return make_float4(rv.x - af * av.x, rv.y - af * av.y, rv.z - af * av.z, rv.w - af * av.w);
     };
+    // z1 = c1 D^-1 r of one 4-cell chunk (+0 at non-fluid cells)
+    auto z1of = [&](float4 rv, uint32_t kw) {
+        const float v[4] = {rv.x, rv.y, rv.z, rv.w};
+        float o[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint8_t c = byte_of(kw, e);
+            o[e] = c >= kCodeFluid0 ? v[e] * winv(c, K.c1) : 0.0f;
+        }
+        return make_float4(o[0], o[1], o[2], o[3]);
+    };
     const int cnx = nx >> 1, cny = ny >> 1, cnz = nz >> 1;
     (void)cnx;
     // r at planes q-1, q, q+1 (updated when FOLD); raw r and Ap at q+2, q+3
     float4 Rm = fold(ldv(i0 - 2), lda(i0 - 2)), Rc = fold(ldv(i0 - 1), lda(i0 - 1)), Rn = fold(ldv(i0), lda(i0));
     float4 Rf = ldv(i0 + 1), Af = lda(i0 + 1);
     uint32_t Kmm = ldk(i0 - 3), Km = ldk(i0 - 2), Kc = ldk(i0 - 1), Kn = ldk(i0), Kf = ldk(i0 + 1);
+    // z1 at planes q-1, q, q+1
+    float4 Z1m = z1of(Rm, Km), Z1c = z1of(Rc, Kc), Z1n = z1of(Rn, Kn);
     float Zmm[4] = {0.f, 0.f, 0.f, 0.f}, Zm[4] = {0.f, 0.f, 0.f, 0.f};
     float acc[2] = {0.f, 0.f};
     double rr = 0.0;
@@ -799,7 +822,7 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
         float* VR = sR[par] + 4;
         float* VZ = sZ[par] + 4;
         uint8_t* C = sC[s3] + 8;
-        *reinterpret_cast<float4*>(VR + row * W + 4 * ch) = Rc;
+        *reinterpret_cast<float4*>(VR + row * W + 4 * ch) = Z1c;
         *reinterpret_cast<uint32_t*>(C + row * W + 4 * ch) = Kc;
         const float4 Rg = ldv(q + 3);  // two planes of loads in flight
         const float4 Ag = lda(q + 3);
@@ -815,31 +838,20 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
         // --- smoother at plane q (tile and its one-cell ring) ---------------
         float Zc[4] = {0.f, 0.f, 0.f, 0.f};
         if (smooth_row) {
+            // z1 of the j-neighbours and across the chunk edges from LDS (air,
+            // +0, outside the tile row)
             const float4 d4 = *reinterpret_cast<const float4*>(VR + (row - 1) * W + 4 * ch);
             const float4 u4 = *reinterpret_cast<const float4*>(VR + (row + 1) * W + 4 * ch);
             const uint32_t kd = *reinterpret_cast<const uint32_t*>(C + (row - 1) * W + 4 * ch);
             const uint32_t ku = *reinterpret_cast<const uint32_t*>(C + (row + 1) * W + 4 * ch);
-            // k-neighbours across the chunk edge (air outside the tile row)
             uint8_t cb = kCodeAir, cf = kCodeAir;
             float vb = 0.0f, vf = 0.0f;
             if (ch > 0) { cb = C[row * W + 4 * ch - 1]; vb = VR[row * W + 4 * ch - 1]; }
             if (ch < CH - 1) { cf = C[row * W + 4 * ch + 4]; vf = VR[row * W + 4 * ch + 4]; }
             const float rcv[4] = {Rc.x, Rc.y, Rc.z, Rc.w};
-            float vc[4], vl[4], vr[4], vd[4], vu[4];
-            {
-                const float l[4] = {Rm.x, Rm.y, Rm.z, Rm.w}, n[4] = {Rn.x, Rn.y, Rn.z, Rn.w};
-                const float dd[4] = {d4.x, d4.y, d4.z, d4.w}, uu[4] = {u4.x, u4.y, u4.z, u4.w};
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    vc[e] = rcv[e] * winv(byte_of(Kc, e), K.c1);
-                    vl[e] = l[e] * winv(byte_of(Km, e), K.c1);
-                    vr[e] = n[e] * winv(byte_of(Kn, e), K.c1);
-                    vd[e] = dd[e] * winv(byte_of(kd, e), K.c1);
-                    vu[e] = uu[e] * winv(byte_of(ku, e), K.c1);
-                }
-            }
-            vb *= winv(cb, K.c1);
-            vf *= winv(cf, K.c1);
+            const float vc[4] = {Z1c.x, Z1c.y, Z1c.z, Z1c.w}, vl[4] = {Z1m.x, Z1m.y, Z1m.z, Z1m.w};
+            const float vr[4] = {Z1n.x, Z1n.y, Z1n.z, Z1n.w};
+            const float vd[4] = {d4.x, d4.y, d4.z, d4.w}, vu[4] = {u4.x, u4.y, u4.z, u4.w};
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const uint8_t cc = byte_of(Kc, e);
@@ -849,11 +861,11 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
                 const float v_b = e > 0 ? vc[e - 1] : vb;
                 const uint8_t c_f = e < 3 ? byte_of(Kc, e + 1) : cf;
                 const float v_f = e < 3 ? vc[e + 1] : vf;
-                float s = nbr_term(byte_of(Km, e), vl[e], ps) + nbr_term(byte_of(Kn, e), vr[e], ps);
-                s = s + nbr_term(byte_of(kd, e), vd[e], ps);
-                s = s + nbr_term(byte_of(ku, e), vu[e], ps);
-                s = s + nbr_term(c_b, v_b, ps);
-                s = s + nbr_term(c_f, v_f, ps);
+                float s = nbr_term0(byte_of(Km, e), vl[e], ps) + nbr_term0(byte_of(Kn, e), vr[e], ps);
+                s = s + nbr_term0(byte_of(kd, e), vd[e], ps);
+                s = s + nbr_term0(byte_of(ku, e), vu[e], ps);
+                s = s + nbr_term0(c_b, v_b, ps);
+                s = s + nbr_term0(c_f, v_f, ps);
                 const float az = 6.0f * ps - s;
                 Zc[e] = K.c2 * (ps + winv(cc, K.c3) * (rcv[e] - az));
             }
@@ -884,11 +896,11 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
                     const float v_b = e > 0 ? Zm[e - 1] : vb;
                     const uint8_t c_f = e < 3 ? byte_of(Km, e + 1) : cf;
                     const float v_f = e < 3 ? Zm[e + 1] : vf;
-                    float s = nbr_term(byte_of(Kmm, e), Zmm[e], ps) + nbr_term(byte_of(Kc, e), Zc[e], ps);
-                    s += nbr_term(byte_of(kd, e), vd[e], ps);
-                    s += nbr_term(byte_of(ku, e), vu[e], ps);
-                    s += nbr_term(c_b, v_b, ps);
-                    s += nbr_term(c_f, v_f, ps);
+                    float s = nbr_term0(byte_of(Kmm, e), Zmm[e], ps) + nbr_term0(byte_of(Kc, e), Zc[e], ps);
+                    s += nbr_term0(byte_of(kd, e), vd[e], ps);
+                    s += nbr_term0(byte_of(ku, e), vu[e], ps);
+                    s += nbr_term0(c_b, v_b, ps);
+                    s += nbr_term0(c_f, v_f, ps);
                     acc[e >> 1] += rp[e] - (6.0f * ps - s);
                 }
             }
@@ -904,6 +916,7 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
         }
         Rm = Rc; Rc = Rn; Rn = fold(Rf, Af); Rf = Rg; Af = Ag;
         Kmm = Km; Km = Kc; Kc = Kn; Kn = Kf; Kf = Kg;
+        Z1m = Z1c; Z1c = Z1n; Z1n = z1of(Rn, Kn);
 #pragma unroll
         for (int e = 0; e < 4; ++e) { Zmm[e] = Zm[e]; Zm[e] = Zc[e]; }
         s3 = s3 == 2 ? 0 : s3 + 1;
@@ -1008,6 +1021,22 @@ __global__ __launch_bounds__(256) void prolong_post_kernel(int nx, int ny, int n
     }
     float4 Zc4 = ldz(i0 - 1), Zn4 = ldz(i0), Zf4 = ldz(i0 + 1);  // z_in at planes q, q+1, q+2
     uint32_t Kmm = 0, Km = 0, Kc = ldk(i0 - 1), Kn = ldk(i0), Kf = ldk(i0 + 1);
+    // The thread's coarse rows J, J2 and its clamped 4-value k window (used
+    // only where jk_in).  k is a multiple of 4, so fine cell k+e reads window
+    // slots 1 + e/2 and {0, 2, 1, 3}[e]: where prolong4_kernel's clamped K2
+    // differs from these (the chunks at k = 0 and k = nz - 4), the clamped
+    // window holds the same value in both slots.
+    const int Jr = j >> 1;
+    int J2r = Jr + ((j & 1) ? 1 : -1);
+    J2r = J2r < 0 ? 0 : (J2r >= cny ? cny - 1 : J2r);
+    const int rJ = (Jr - Jb) * kPpEK, rJ2 = (J2r - Jb) * kPpEK;
+    int cK[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        int kk = (k >> 1) - 1 + u;
+        kk = kk < 0 ? 0 : (kk >= cnz ? cnz - 1 : kk);
+        cK[u] = kk - Kb;
+    }
     float Tmm[4] = {0.f, 0.f, 0.f, 0.f}, Tm[4] = {0.f, 0.f, 0.f, 0.f};
     double da = 0.0, db = 0.0;
     int s3 = 0;
@@ -1027,41 +1056,30 @@ __global__ __launch_bounds__(256) void prolong_post_kernel(int nx, int ny, int n
         }
         __syncthreads();
         // --- prolongation at plane q (tile and ring) ------------------------
-        float Tc[4] = {Zc4.x, Zc4.y, Zc4.z, Zc4.w};
+        // zt on fluid cells; +0 elsewhere (the sweep never uses it there, so
+        // a neighbour term is one select)
+        float Tc[4] = {0.f, 0.f, 0.f, 0.f};
         if (jk_in && q >= 0 && q < nx && Kc != 0u) {  // Kc == 0: four air cells
             const int I = q >> 1;
             int I2 = I + ((q & 1) ? 1 : -1);
             I2 = I2 < 0 ? 0 : (I2 >= cnx ? cnx - 1 : I2);
-            const int J = j >> 1;
-            int J2 = J + ((j & 1) ? 1 : -1);
-            J2 = J2 < 0 ? 0 : (J2 >= cny ? cny - 1 : J2);
             const float* EI = sE[I % 3];
             const float* EI2 = sE[I2 % 3];
-            const int rJ = (J - Jb) * kPpEK, rJ2 = (J2 - Jb) * kPpEK;
-            const int kb = (k >> 1) - 1;
             float cw[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                int kk = kb + u;
-                kk = kk < 0 ? 0 : (kk >= cnz ? cnz - 1 : kk);
-                const int cK = kk - Kb;
                 float acc = 0.0f;
-                acc += 0.5625f * EI[rJ + cK];
-                acc += 0.1875f * EI[rJ2 + cK];
-                acc += 0.1875f * EI2[rJ + cK];
-                acc += 0.0625f * EI2[rJ2 + cK];
+                acc += 0.5625f * EI[rJ + cK[u]];
+                acc += 0.1875f * EI[rJ2 + cK[u]];
+                acc += 0.1875f * EI2[rJ + cK[u]];
+                acc += 0.0625f * EI2[rJ2 + cK[u]];
                 cw[u] = acc;
             }
+            const float zi[4] = {Zc4.x, Zc4.y, Zc4.z, Zc4.w};
+            constexpr int sK[4] = {1, 1, 2, 2}, sK2[4] = {0, 2, 1, 3};
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                if (byte_of(Kc, e) < kCodeFluid0) continue;
-                const int kq = k + e;
-                const int K = kq >> 1;
-                int K2 = K + ((kq & 1) ? 1 : -1);
-                K2 = K2 < 0 ? 0 : (K2 >= cnz ? cnz - 1 : K2);
-                const int sK = K - kb, sK2 = (K2 - kb) < 0 ? 0 : ((K2 - kb) >= 4 ? 3 : (K2 - kb));
-                Tc[e] += 0.75f * cw[sK] + 0.25f * cw[sK2];
-            }
+            for (int e = 0; e < 4; ++e)
+                if (byte_of(Kc, e) >= kCodeFluid0) Tc[e] = zi[e] + (0.75f * cw[sK[e]] + 0.25f * cw[sK2[e]]);
         }
         float* VZ = sZ[par] + 4;
         *reinterpret_cast<float4*>(VZ + row * W + 4 * ch) = make_float4(Tc[0], Tc[1], Tc[2], Tc[3]);
@@ -1087,11 +1105,11 @@ __global__ __launch_bounds__(256) void prolong_post_kernel(int nx, int ny, int n
                     const float v_b = e > 0 ? Tm[e - 1] : vb;
                     const uint8_t c_f = e < 3 ? byte_of(Km, e + 1) : cf;
                     const float v_f = e < 3 ? Tm[e + 1] : vf;
-                    float sum = nbr_term(byte_of(Kmm, e), Tmm[e], ps) + nbr_term(byte_of(Kc, e), Tc[e], ps);
-                    sum = sum + nbr_term(byte_of(kd, e), vd[e], ps);
-                    sum = sum + nbr_term(byte_of(ku, e), vu[e], ps);
-                    sum = sum + nbr_term(c_b, v_b, ps);
-                    sum = sum + nbr_term(c_f, v_f, ps);
+                    float sum = nbr_term0(byte_of(Kmm, e), Tmm[e], ps) + nbr_term0(byte_of(Kc, e), Tc[e], ps);
+                    sum = sum + nbr_term0(byte_of(kd, e), vd[e], ps);
+                    sum = sum + nbr_term0(byte_of(ku, e), vu[e], ps);
+                    sum = sum + nbr_term0(c_b, v_b, ps);
+                    sum = sum + nbr_term0(c_f, v_f, ps);
                     out[e] = ps + winv(cc, w) * (rv[e] - (6.0f * ps - sum));
                 }
                 da += (double)rv[e] * (double)out[e];
