@@ -249,6 +249,14 @@ __device__ __forceinline__ float nbr_term(uint8_t c, float v, float ps) {
     return c >= kCodeFluid0 ? v : (c == kCodeSolid ? ps : 0.0f);
 }
 
+// The multigrid kernels' form of the same operator on a fluid cell of code c:
+// (8 - c) v_c - s, s the sum of the fluid neighbours' values (a non-fluid
+// neighbour adds +0; its effect is in 8 - c = 6 - #solid neighbours).  One
+// select per neighbour (fz), or none where the values are +0 off the fluid by
+// construction; the rounding differs from the reference's 6 v_c - S(v).
+__device__ __forceinline__ float fz(uint8_t c, float v) { return c >= kCodeFluid0 ? v : 0.0f; }
+__device__ __forceinline__ float lap_mg(uint8_t c, float ps, float s) { return (float)(8 - (int)c) * ps - s; }
+
 // Loads of a 7-point neighbourhood for VEC cells (values and codes); out-of-
 // bounds neighbours read as air.
 template <int VEC>
@@ -292,6 +300,22 @@ struct Nbhd {
         s = s + nbr_term(c_b, v_b, ps);
         s = s + nbr_term(c_f, v_f, ps);
         return 6.0f * ps - s;
+    }
+
+    // The multigrid form (lap_mg): neighbour sum in the order -i, +i, -j, +j,
+    // -k, +k over fluid neighbours only.
+    __device__ __forceinline__ float apply_mg(int e) const {
+        if (cc[e] < kCodeFluid0) return 0.0f;
+        const uint8_t c_b = e > 0 ? cc[e - 1] : cb;
+        const float v_b = e > 0 ? vc[e - 1] : vb;
+        const uint8_t c_f = e < VEC - 1 ? cc[e + 1] : cf;
+        const float v_f = e < VEC - 1 ? vc[e + 1] : vf;
+        float s = fz(cl[e], vl[e]) + fz(cr[e], vr[e]);
+        s = s + fz(cd[e], vd[e]);
+        s = s + fz(cu[e], vu[e]);
+        s = s + fz(c_b, v_b);
+        s = s + fz(c_f, v_f);
+        return lap_mg(cc[e], vc[e], s);
     }
 
     // Same operator with the neighbour sum and the product in FP64.

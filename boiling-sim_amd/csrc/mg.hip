@@ -94,21 +94,20 @@ __device__ __forceinline__ float inv_diag(uint8_t code) {
     return dg > 0 ? 1.0f / (float)dg : 1.0f;
 }
 
-// (A v)[c] with codes, out of bounds = air.
+// (A v)[c] with codes, out of bounds = air (the multigrid form, lap_mg).
 __device__ __forceinline__ float apply_a(const MLvl& L, const uint8_t* __restrict__ code,
                                          const float* __restrict__ v, int64_t c, int i, int j, int k) {
     const uint8_t cc = code[c];
     if (cc < kCodeFluid0) return 0.0f;
-    const float ps = v[c];
     const int64_t plane = (int64_t)L.ny * L.nz;
     float s = 0.0f;
-    if (i > 0) s += nbr_term(code[c - plane], v[c - plane], ps);
-    if (i < L.nx - 1) s += nbr_term(code[c + plane], v[c + plane], ps);
-    if (j > 0) s += nbr_term(code[c - L.nz], v[c - L.nz], ps);
-    if (j < L.ny - 1) s += nbr_term(code[c + L.nz], v[c + L.nz], ps);
-    if (k > 0) s += nbr_term(code[c - 1], v[c - 1], ps);
-    if (k < L.nz - 1) s += nbr_term(code[c + 1], v[c + 1], ps);
-    return 6.0f * ps - s;
+    if (i > 0) s += fz(code[c - plane], v[c - plane]);
+    if (i < L.nx - 1) s += fz(code[c + plane], v[c + plane]);
+    if (j > 0) s += fz(code[c - L.nz], v[c - L.nz]);
+    if (j < L.ny - 1) s += fz(code[c + L.nz], v[c + L.nz]);
+    if (k > 0) s += fz(code[c - 1], v[c - 1]);
+    if (k < L.nz - 1) s += fz(code[c + 1], v[c + 1]);
+    return lap_mg(cc, v[c], s);
 }
 
 // Coarse class ids (0 fluid, 1 solid, 2 air) from the fine codes.
@@ -378,7 +377,7 @@ __global__ __launch_bounds__(256) void pre2_kernel(Tiles T, const uint8_t* __res
     nb.vf *= winv(nb.cf, K.c1);
 #pragma unroll
     for (int e = 0; e < VEC; ++e) {
-        const float az = nb.apply(e);
+        const float az = nb.apply_mg(e);
         out[e] = nb.cc[e] >= kCodeFluid0 ? K.c2 * (nb.vc[e] + winv(nb.cc[e], K.c3) * (rc[e] - az)) : 0.0f;
     }
     stf<VEC>(z, c0, out);
@@ -399,7 +398,7 @@ __global__ __launch_bounds__(256) void smooth_kernel(Tiles T, const uint8_t* __r
     ldf<VEC>(r, c0, rv);
 #pragma unroll
     for (int e = 0; e < VEC; ++e)
-        out[e] = nb.cc[e] >= kCodeFluid0 ? nb.vc[e] + winv(nb.cc[e]) * (rv[e] - nb.apply(e)) : 0.0f;
+        out[e] = nb.cc[e] >= kCodeFluid0 ? nb.vc[e] + winv(nb.cc[e]) * (rv[e] - nb.apply_mg(e)) : 0.0f;
     stf<VEC>(zo, c0, out);
 }
 
@@ -425,7 +424,7 @@ __global__ __launch_bounds__(256) void cheb_kernel(Tiles T, const uint8_t* __res
         const uint8_t cc = nb.cc[e];
         const int dg = 8 - (int)cc;
         const float dinv = dg > 0 ? __builtin_amdgcn_rcpf((float)dg) : 1.0f;
-        out[e] = cc >= kCodeFluid0 ? pv[e] + om * (nb.vc[e] - pv[e] + ga * dinv * (rv[e] - nb.apply(e))) : 0.0f;
+        out[e] = cc >= kCodeFluid0 ? pv[e] + om * (nb.vc[e] - pv[e] + ga * dinv * (rv[e] - nb.apply_mg(e))) : 0.0f;
     }
     stf<VEC>(x_out, c0, out);
 }
@@ -460,7 +459,7 @@ __global__ __launch_bounds__(256) void smooth_dots_kernel(Tiles T, const uint8_t
         if (z_old) ldf_nt<VEC>(z_old, c0, zo_);
 #pragma unroll
         for (int e = 0; e < VEC; ++e) {
-            out[e] = nb.cc[e] >= kCodeFluid0 ? nb.vc[e] + winv(nb.cc[e], w) * (rv[e] - nb.apply(e)) : 0.0f;
+            out[e] = nb.cc[e] >= kCodeFluid0 ? nb.vc[e] + winv(nb.cc[e], w) * (rv[e] - nb.apply_mg(e)) : 0.0f;
             a += (double)rv[e] * (double)out[e];
             if (z_old) b += (double)rv[e] * (double)zo_[e];
         }
@@ -497,7 +496,7 @@ __global__ __launch_bounds__(256) void restrict2_kernel(Tiles F, Tiles C, const 
             ldf<2>(r, f0, rv);
 #pragma unroll
             for (int e = 0; e < 2; ++e)
-                if (nb.cc[e] >= kCodeFluid0) s += rv[e] - nb.apply(e);
+                if (nb.cc[e] >= kCodeFluid0) s += rv[e] - nb.apply_mg(e);
         }
     }
     rc[cc] = 0.5f * s;
@@ -550,12 +549,12 @@ __global__ __launch_bounds__(256) void restrict4_kernel(Tiles F, Tiles C, const 
             for (int e = 0; e < 4; ++e) {
                 if (cv[a][b][e] < kCodeFluid0) continue;
                 const float ps = zv[a][b][e];
-                float s = nbr_term(cv[a - 1][b][e], zv[a - 1][b][e], ps) + nbr_term(cv[a + 1][b][e], zv[a + 1][b][e], ps);
-                s += nbr_term(cv[a][b - 1][e], zv[a][b - 1][e], ps);
-                s += nbr_term(cv[a][b + 1][e], zv[a][b + 1][e], ps);
-                s += e > 0 ? nbr_term(cv[a][b][e - 1], zv[a][b][e - 1], ps) : nbr_term(cb, zb, ps);
-                s += e < 3 ? nbr_term(cv[a][b][e + 1], zv[a][b][e + 1], ps) : nbr_term(cf, zf, ps);
-                acc[e >> 1] += rv[e] - (6.0f * ps - s);
+                float s = fz(cv[a - 1][b][e], zv[a - 1][b][e]) + fz(cv[a + 1][b][e], zv[a + 1][b][e]);
+                s += fz(cv[a][b - 1][e], zv[a][b - 1][e]);
+                s += fz(cv[a][b + 1][e], zv[a][b + 1][e]);
+                s += e > 0 ? fz(cv[a][b][e - 1], zv[a][b][e - 1]) : fz(cb, zb);
+                s += e < 3 ? fz(cv[a][b][e + 1], zv[a][b][e + 1]) : fz(cf, zf);
+                acc[e >> 1] += rv[e] - lap_mg(cv[a][b][e], ps, s);
             }
         }
     const float out[2] = {0.5f * acc[0], 0.5f * acc[1]};
@@ -694,18 +693,16 @@ __global__ __launch_bounds__(256) void prolong4_kernel(Tiles F, Tiles CV, const 
 // tile, which it sums over the 2x2x2 children of each coarse cell (times 0.5,
 // the restriction of restrict4_kernel; only the order of the sum differs).
 // i-neighbours are the thread's registers, j/k-neighbours come from LDS
-// planes: r double-buffered, z double-buffered, codes triple-buffered (the
-// residual at q-1 still reads plane q-1's codes while plane q+1's are
-// stored), which leaves one barrier per plane; the global loads run two
-// planes ahead.  z1 is formed once per cell when its plane arrives (in
-// registers for the i-neighbours, in the LDS plane for the j/k ones) rather
-// than at each of its seven uses: the same product, so the same bits, at a
-// third less VALU work (the pass is issue-bound, not HBM-bound).  z1 and z
-// are set to exactly +0 at non-fluid cells, where the smoother never uses
-// them, so a neighbour term is one select (nbr_term0).  The march through a
-// segment is a chain of per-plane latencies, so the pass pays off only where
-// the level fills the chip with several rounds of workgroups (the fine
-// level).
+// planes (z1 and z, each double-buffered), which leaves one barrier per
+// plane; the global loads run two planes ahead.  z1 is formed once per cell
+// when its plane arrives (in registers for the i-neighbours, in the LDS plane
+// for the j/k ones) rather than at each of its seven uses.  z1 and z are held
+// at exactly +0 on non-fluid cells, so A uses the multigrid form lap_mg with
+// no selects and no neighbour codes: only the centre's code is read, from
+// registers (the pass is issue-bound, not HBM-bound, so VALU work is what
+// these save).  The march through a segment is a chain of per-plane
+// latencies, so the pass pays off only where the level fills the chip with
+// several rounds of workgroups (the fine level).
 // ---------------------------------------------------------------------------
 constexpr int kPrTJ = 12, kPrKT = 14;
 
@@ -736,10 +733,6 @@ static bool pr_fits(const MLvl& L, int vec) { return vec == 4 && (L.nx & 1) == 0
 
 __device__ __forceinline__ uint8_t byte_of(uint32_t w, int e) { return (uint8_t)((w >> (8 * e)) & 0xFFu); }
 
-// nbr_term for values that are exactly +0 at air cells (and at positions
-// outside the grid): the air case needs no select of its own.
-__device__ __forceinline__ float nbr_term0(uint8_t c, float v, float ps) { return c == kCodeSolid ? ps : v; }
-
 // FOLD: the flexible CG's residual update runs in the same pass: the kernel
 // reads r_old and Ap, works on r = r_old - alpha Ap (alpha = *alpha_p, the
 // update kernel's float operations), writes r to r_out on the tile (r_out
@@ -759,7 +752,6 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
     static_assert(CH * SR == 256, "one thread per chunk of the tile and its ring");
     __shared__ __align__(16) float sR[2][SR * W + 8];
     __shared__ __align__(16) float sZ[2][SR * W + 8];
-    __shared__ __align__(16) uint8_t sC[3][SR * W + 16];
     const int tid = threadIdx.x;
     const int row = tid / CH, ch = tid - (tid / CH) * CH;
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
@@ -816,14 +808,11 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
     float Zmm[4] = {0.f, 0.f, 0.f, 0.f}, Zm[4] = {0.f, 0.f, 0.f, 0.f};
     float acc[2] = {0.f, 0.f};
     double rr = 0.0;
-    int s3 = 0;  // codes slot of plane q (planes q-1, q-2 in the two before it)
     for (int q = i0 - 1; q <= i1; ++q) {
         const int par = (q - i0 + 1) & 1;
         float* VR = sR[par] + 4;
         float* VZ = sZ[par] + 4;
-        uint8_t* C = sC[s3] + 8;
         *reinterpret_cast<float4*>(VR + row * W + 4 * ch) = Z1c;
-        *reinterpret_cast<uint32_t*>(C + row * W + 4 * ch) = Kc;
         const float4 Rg = ldv(q + 3);  // two planes of loads in flight
         const float4 Ag = lda(q + 3);
         const uint32_t Kg = ldk(q + 3);
@@ -842,12 +831,9 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
             // +0, outside the tile row)
             const float4 d4 = *reinterpret_cast<const float4*>(VR + (row - 1) * W + 4 * ch);
             const float4 u4 = *reinterpret_cast<const float4*>(VR + (row + 1) * W + 4 * ch);
-            const uint32_t kd = *reinterpret_cast<const uint32_t*>(C + (row - 1) * W + 4 * ch);
-            const uint32_t ku = *reinterpret_cast<const uint32_t*>(C + (row + 1) * W + 4 * ch);
-            uint8_t cb = kCodeAir, cf = kCodeAir;
             float vb = 0.0f, vf = 0.0f;
-            if (ch > 0) { cb = C[row * W + 4 * ch - 1]; vb = VR[row * W + 4 * ch - 1]; }
-            if (ch < CH - 1) { cf = C[row * W + 4 * ch + 4]; vf = VR[row * W + 4 * ch + 4]; }
+            if (ch > 0) vb = VR[row * W + 4 * ch - 1];
+            if (ch < CH - 1) vf = VR[row * W + 4 * ch + 4];
             const float rcv[4] = {Rc.x, Rc.y, Rc.z, Rc.w};
             const float vc[4] = {Z1c.x, Z1c.y, Z1c.z, Z1c.w}, vl[4] = {Z1m.x, Z1m.y, Z1m.z, Z1m.w};
             const float vr[4] = {Z1n.x, Z1n.y, Z1n.z, Z1n.w};
@@ -857,16 +843,14 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
                 const uint8_t cc = byte_of(Kc, e);
                 if (cc < kCodeFluid0) continue;
                 const float ps = vc[e];
-                const uint8_t c_b = e > 0 ? byte_of(Kc, e - 1) : cb;
                 const float v_b = e > 0 ? vc[e - 1] : vb;
-                const uint8_t c_f = e < 3 ? byte_of(Kc, e + 1) : cf;
                 const float v_f = e < 3 ? vc[e + 1] : vf;
-                float s = nbr_term0(byte_of(Km, e), vl[e], ps) + nbr_term0(byte_of(Kn, e), vr[e], ps);
-                s = s + nbr_term0(byte_of(kd, e), vd[e], ps);
-                s = s + nbr_term0(byte_of(ku, e), vu[e], ps);
-                s = s + nbr_term0(c_b, v_b, ps);
-                s = s + nbr_term0(c_f, v_f, ps);
-                const float az = 6.0f * ps - s;
+                float s = vl[e] + vr[e];
+                s = s + vd[e];
+                s = s + vu[e];
+                s = s + v_b;
+                s = s + v_f;
+                const float az = lap_mg(cc, ps, s);
                 Zc[e] = K.c2 * (ps + winv(cc, K.c3) * (rcv[e] - az));
             }
             if (inner && q >= i0 && q < i1)
@@ -878,12 +862,8 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
         if (p >= i0) {
             if (inner) {
                 const float* PZ = sZ[par ^ 1] + 4;  // plane p's smoothed values
-                const uint8_t* PC = sC[s3 == 0 ? 2 : s3 - 1] + 8;
                 const float4 d4 = *reinterpret_cast<const float4*>(PZ + (row - 1) * W + 4 * ch);
                 const float4 u4 = *reinterpret_cast<const float4*>(PZ + (row + 1) * W + 4 * ch);
-                const uint32_t kd = *reinterpret_cast<const uint32_t*>(PC + (row - 1) * W + 4 * ch);
-                const uint32_t ku = *reinterpret_cast<const uint32_t*>(PC + (row + 1) * W + 4 * ch);
-                const uint8_t cb = PC[row * W + 4 * ch - 1], cf = PC[row * W + 4 * ch + 4];
                 const float vb = PZ[row * W + 4 * ch - 1], vf = PZ[row * W + 4 * ch + 4];
                 const float rp[4] = {Rm.x, Rm.y, Rm.z, Rm.w};
                 const float vd[4] = {d4.x, d4.y, d4.z, d4.w}, vu[4] = {u4.x, u4.y, u4.z, u4.w};
@@ -892,16 +872,14 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
                     const uint8_t cc = byte_of(Km, e);
                     if (cc < kCodeFluid0) continue;
                     const float ps = Zm[e];
-                    const uint8_t c_b = e > 0 ? byte_of(Km, e - 1) : cb;
                     const float v_b = e > 0 ? Zm[e - 1] : vb;
-                    const uint8_t c_f = e < 3 ? byte_of(Km, e + 1) : cf;
                     const float v_f = e < 3 ? Zm[e + 1] : vf;
-                    float s = nbr_term0(byte_of(Kmm, e), Zmm[e], ps) + nbr_term0(byte_of(Kc, e), Zc[e], ps);
-                    s += nbr_term0(byte_of(kd, e), vd[e], ps);
-                    s += nbr_term0(byte_of(ku, e), vu[e], ps);
-                    s += nbr_term0(c_b, v_b, ps);
-                    s += nbr_term0(c_f, v_f, ps);
-                    acc[e >> 1] += rp[e] - (6.0f * ps - s);
+                    float s = Zmm[e] + Zc[e];
+                    s += vd[e];
+                    s += vu[e];
+                    s += v_b;
+                    s += v_f;
+                    acc[e >> 1] += rp[e] - lap_mg(cc, ps, s);
                 }
             }
             if (p & 1) {  // both fine planes of coarse plane p/2 are in: add row pairs
@@ -919,7 +897,6 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
         Z1m = Z1c; Z1c = Z1n; Z1n = z1of(Rn, Kn);
 #pragma unroll
         for (int e = 0; e < 4; ++e) { Zmm[e] = Zm[e]; Zm[e] = Zc[e]; }
-        s3 = s3 == 2 ? 0 : s3 + 1;
     }
     if (FOLD) {
         double v[1] = {rr};
@@ -939,8 +916,9 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
 // kernels'; only the dot products' summation order differs.  The coarse
 // values come from LDS: each coarse plane's 9 x 34 window under the tile is
 // loaded once into a ring of three (a fine plane reads coarse planes q/2 and
-// q/2 -/+ 1); zt planes are double-buffered, codes triple-buffered (the sweep
-// at q-1 still reads plane q-1's codes while plane q+1's are stored).
+// q/2 -/+ 1); zt planes are double-buffered.  zt is held at +0 on non-fluid
+// cells (the sweep never reads it there), so the sweep's A is lap_mg with no
+// neighbour codes.
 // ---------------------------------------------------------------------------
 constexpr int kPpTJ = 14, kPpKT = 14;
 constexpr int kPpER = kPpTJ / 2 + 2, kPpEK = 2 * kPpKT + 6;  // coarse window: 9 rows x 34
@@ -977,7 +955,6 @@ __global__ __launch_bounds__(256) void prolong_post_kernel(int nx, int ny, int n
     constexpr int CH = kPpKT + 2, SR = kPpTJ + 2, W = 4 * CH;
     static_assert(CH * SR == 256, "one thread per chunk of the tile and its ring");
     __shared__ __align__(16) float sZ[2][SR * W + 8];
-    __shared__ __align__(16) uint8_t sC[3][SR * W + 16];
     __shared__ float sE[3][kPpEN];
     const int tid = threadIdx.x;
     const int row = tid / CH, ch = tid - (tid / CH) * CH;
@@ -1039,11 +1016,8 @@ __global__ __launch_bounds__(256) void prolong_post_kernel(int nx, int ny, int n
     }
     float Tmm[4] = {0.f, 0.f, 0.f, 0.f}, Tm[4] = {0.f, 0.f, 0.f, 0.f};
     double da = 0.0, db = 0.0;
-    int s3 = 0;
     for (int q = i0 - 1; q <= i1; ++q) {
         const int par = (q - i0 + 1) & 1;
-        uint8_t* C = sC[s3] + 8;
-        *reinterpret_cast<uint32_t*>(C + row * W + 4 * ch) = Kc;
         if (q & 1) load_coarse((q >> 1) + 1);
         const float4 Zg4 = ldz(q + 3);  // planes q .. q+2 are held: three planes of loads in flight
         const uint32_t Kg = ldk(q + 3);
@@ -1086,12 +1060,8 @@ __global__ __launch_bounds__(256) void prolong_post_kernel(int nx, int ny, int n
         // --- post-sweep at plane p = q - 1 and the dot products ------------
         if (inner && p >= i0) {
             const float* PZ = sZ[par ^ 1] + 4;
-            const uint8_t* PC = sC[s3 == 0 ? 2 : s3 - 1] + 8;
             const float4 d4 = *reinterpret_cast<const float4*>(PZ + (row - 1) * W + 4 * ch);
             const float4 u4 = *reinterpret_cast<const float4*>(PZ + (row + 1) * W + 4 * ch);
-            const uint32_t kd = *reinterpret_cast<const uint32_t*>(PC + (row - 1) * W + 4 * ch);
-            const uint32_t ku = *reinterpret_cast<const uint32_t*>(PC + (row + 1) * W + 4 * ch);
-            const uint8_t cb = PC[row * W + 4 * ch - 1], cf = PC[row * W + 4 * ch + 4];
             const float vb = PZ[row * W + 4 * ch - 1], vf = PZ[row * W + 4 * ch + 4];
             const float vd[4] = {d4.x, d4.y, d4.z, d4.w}, vu[4] = {u4.x, u4.y, u4.z, u4.w};
             float out[4];
@@ -1101,16 +1071,14 @@ __global__ __launch_bounds__(256) void prolong_post_kernel(int nx, int ny, int n
                 out[e] = 0.0f;
                 if (cc >= kCodeFluid0) {
                     const float ps = Tm[e];
-                    const uint8_t c_b = e > 0 ? byte_of(Km, e - 1) : cb;
                     const float v_b = e > 0 ? Tm[e - 1] : vb;
-                    const uint8_t c_f = e < 3 ? byte_of(Km, e + 1) : cf;
                     const float v_f = e < 3 ? Tm[e + 1] : vf;
-                    float sum = nbr_term0(byte_of(Kmm, e), Tmm[e], ps) + nbr_term0(byte_of(Kc, e), Tc[e], ps);
-                    sum = sum + nbr_term0(byte_of(kd, e), vd[e], ps);
-                    sum = sum + nbr_term0(byte_of(ku, e), vu[e], ps);
-                    sum = sum + nbr_term0(c_b, v_b, ps);
-                    sum = sum + nbr_term0(c_f, v_f, ps);
-                    out[e] = ps + winv(cc, w) * (rv[e] - (6.0f * ps - sum));
+                    float sum = Tmm[e] + Tc[e];
+                    sum = sum + vd[e];
+                    sum = sum + vu[e];
+                    sum = sum + v_b;
+                    sum = sum + v_f;
+                    out[e] = ps + winv(cc, w) * (rv[e] - lap_mg(cc, ps, sum));
                 }
                 da += (double)rv[e] * (double)out[e];
                 if (z_old) db += (double)rv[e] * (double)zo[e];
@@ -1121,7 +1089,6 @@ __global__ __launch_bounds__(256) void prolong_post_kernel(int nx, int ny, int n
         Kmm = Km; Km = Kc; Kc = Kn; Kn = Kf; Kf = Kg;
 #pragma unroll
         for (int e = 0; e < 4; ++e) { Tmm[e] = Tm[e]; Tm[e] = Tc[e]; }
-        s3 = s3 == 2 ? 0 : s3 + 1;
     }
     double v[2] = {da, db};
     write_partials<2>(v, part);
