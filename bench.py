@@ -74,7 +74,7 @@ TRAFFIC_JACOBI_PAIR = 14.207
 # The learned iteration's fine-level passes, measured the same way over
 # `bench.py --legs passes` (reads = FETCH_SIZE x 1.99965 from max_abs_kernel,
 # writes = WRITE_SIZE; dispatches that returned at once left out).
-TRAFFIC_PASSES = {"fcg_spmv_dir_kernel": 28.27, "pre2_restrict_kernel<true>": 23.36, "prolong_post_kernel": 16.48}
+TRAFFIC_PASSES = {"fcg_spmv_dir_kernel": 28.26, "pre2_restrict_kernel<true>": 22.58, "prolong_post_kernel": 16.34}
 
 
 # The same flexible CG and V-cycle with non-learned solves of the 4x-coarsened
@@ -107,8 +107,8 @@ def parse():
                     help="comma list of the legs to run (the JSON fields of skipped legs are null)")
     ap.add_argument("--seq-frames", type=int, default=100,
                     help="frames of the warm-started AR(1) sequence leg (0: skip)")
-    ap.add_argument("--seq-methods", default="learned,pcg-fp32",
-                    help="methods of the sequence leg (learned, pcg-fp32, pcg-fp16)")
+    ap.add_argument("--seq-methods", default="learned,mg,pcg-fp32",
+                    help="methods of the sequence leg (learned, mg, pcg-fp32, pcg-fp16)")
     ap.add_argument("--cpu-threads", type=int, default=16,
                     help="host threads of the CPU baseline at most (the GPU's share of the host's cores)")
     return ap.parse_args()
@@ -670,6 +670,9 @@ def main():
         if "learned" in seq and "pcg-fp32" in seq:
             seq["speedup_learned_vs_pcg_fp32"] = (seq["pcg-fp32"]["ms_per_solve_median"] /
                                                   seq["learned"]["ms_per_solve_median"])
+        if "learned" in seq and "mg" in seq:
+            seq["ms_ratio_learned_over_mg"] = (seq["learned"]["ms_per_solve_median"] /
+                                               seq["mg"]["ms_per_solve_median"])
         seq["rhs"] = "AR(1) GRF (step corr 0.95, ell 12 cells, unit variance) + drifting plume; warm starts"
 
     cpu = None

@@ -24,7 +24,7 @@ import time
 
 import numpy as np
 
-from .learned import _UNIT, PRECOND_MG_UNET, grf_torch
+from .learned import _UNIT, PRECOND_MG_CHEB, PRECOND_MG_UNET, grf_torch
 
 STEP_CORR = 0.95
 ELL = 12.0
@@ -62,14 +62,17 @@ def ar1_frames(torch, mat_d, n_frames, seed=0, ell=ELL):
 
 class SequenceSolver:
     """Warm-started FP64 solves of consecutive frames with one method:
-    "learned" (refinement around the learned flexible CG) or "pcg-fp32" /
-    "pcg-fp16" (refinement around Jacobi-PCG with that vector storage)."""
+    "learned" (refinement around the learned flexible CG), "mg" (the same
+    flexible CG and V-cycle with 60 Chebyshev steps instead of the network on
+    the coarsest level) or "pcg-fp32" / "pcg-fp16" (refinement around
+    Jacobi-PCG with that vector storage)."""
 
     def __init__(self, core, torch, mat_d, method, net=None, tol=1e-6):
         self.core, self.torch, self.mat_d, self.method, self.net, self.tol = core, torch, mat_d, method, net, tol
         self.shape = tuple(mat_d.shape)
-        if method == "learned":
-            nbytes = core.fcg_refine_workspace_bytes(*self.shape, PRECOND_MG_UNET)
+        self.precond = {"learned": PRECOND_MG_UNET, "mg": PRECOND_MG_CHEB}.get(method)
+        if self.precond is not None:
+            nbytes = core.fcg_refine_workspace_bytes(*self.shape, self.precond)
         else:
             nbytes = core.refine_workspace_bytes(*self.shape)
         self.ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
@@ -83,10 +86,11 @@ class SequenceSolver:
         args = (x.data_ptr(), b.data_ptr(), self.mat_d.data_ptr(), *self.shape, dx, dt, rho, tol)
         if warm is None:
             warm = self.started or x is not self.x
-        if self.method == "learned":
-            st = self.core.fcg_refine_solve(*args, PRECOND_MG_UNET, self.net.params.data_ptr(),
-                                            self.net.packed.data_ptr(), self.ws.data_ptr(), self.ws.numel(),
-                                            inner_tol=1e-3, warm_start=warm)
+        if self.precond is not None:
+            net = self.net if self.precond == PRECOND_MG_UNET else None
+            st = self.core.fcg_refine_solve(*args, self.precond, net.params.data_ptr() if net else 0,
+                                            net.packed.data_ptr() if net else 0, self.ws.data_ptr(),
+                                            self.ws.numel(), inner_tol=1e-3, warm_start=warm)
         else:
             st = self.core.refine_solve(*args, self.ws.data_ptr(), self.ws.numel(), storage=self.method[4:],
                                         warm_start=warm)
@@ -151,7 +155,7 @@ def main(argv=None):
     ap.add_argument("--frames", type=int, default=100)
     ap.add_argument("--grid", default="prod", choices=["small", "dev", "prod"])
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--methods", default="learned,pcg-fp32,pcg-fp16")
+    ap.add_argument("--methods", default="learned,mg,pcg-fp32,pcg-fp16")
     ap.add_argument("--error-every", type=int, default=10)
     ap.add_argument("--out", default=None, help="JSON file for the per-frame records and the summary")
     a = ap.parse_args(argv)

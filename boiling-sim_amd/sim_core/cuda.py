@@ -297,9 +297,13 @@ class SimCore:
                   unet_params_ptr, unet_packed_ptr, workspace_ptr, workspace_bytes, warm_start=False,
                   mat_fluid=0, mat_air=2, stream=0) -> dict:
         """Flexible CG with preconditioner kind ``precond`` (1 U-Net, 2 V-cycle +
-        U-Net coarse solve + Chebyshev polish, 3 the same V-cycle without the
-        network, 4 Jacobi sweeps + U-Net on their residual) to a true relative
-        residual ``tol``; falls back to plain PCG on a stall."""
+        U-Net coarse solve + Chebyshev polish, 3 the same V-cycle with a Jacobi
+        sweep + the polish instead of the network, 4 Jacobi sweeps + U-Net on
+        their residual, 5 the V-cycle with a Jacobi sweep + 60 Chebyshev steps
+        on the coarsest level, 6 the V-cycle with two more levels and a bottom
+        PCG) to a true relative residual ``tol``; falls back to plain PCG on a
+        stall or a non-positive r.z.  Kinds 3, 5 and 6 take null network
+        pointers."""
         st = _lib.FcgStats()
         _lib.call(
             "bs_fcg_solve",

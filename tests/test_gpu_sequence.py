@@ -2,7 +2,8 @@
 every frame of every method reaches the tolerance by the oracle's FP64
 residual, and its error is stated against the oracle's own FP64 PCG solution
 of the same frame to 1e-12 (independent of the device), on the 110x110x70
-pot and, for the learned method, at the production grid 440x440x280.  The
+pot (the learned flexible CG, the same V-cycle without the network, and
+Jacobi-PCG) and, for the learned method, at the production grid 440x440x280.  The
 reference cold-starts every projection (python/boilingsim/fluid.py:627-629);
 warm starts are new here, so only the oracle can judge them."""
 
@@ -28,7 +29,7 @@ def test_sequence_dev_against_oracle(simcore, torch_gpu):
     mat = pot_mask(110, 110, 70)
     mat_d = torch.from_numpy(mat).cuda()
     fl = mat == 0
-    methods = ("learned", "pcg-fp32")
+    methods = ("learned", "mg", "pcg-fp32")
     solvers = {m: SequenceSolver(simcore, torch, mat_d, m, net=default_net(), tol=TOL) for m in methods}
     its = {m: [] for m in methods}
     x_ref = None
@@ -47,9 +48,10 @@ def test_sequence_dev_against_oracle(simcore, torch_gpu):
             assert err < 1e-4
             its[m].append(st["inner_iterations"])
     # Warm starts cut the plain method's iterations after the first frame;
-    # the learned method needs far fewer.
+    # both V-cycle methods (with and without the network) need far fewer.
     assert max(its["pcg-fp32"][1:]) < its["pcg-fp32"][0]
     assert 4 * np.mean(its["learned"]) < np.mean(its["pcg-fp32"])
+    assert 4 * np.mean(its["mg"]) < np.mean(its["pcg-fp32"])
 
 
 def test_sequence_production_grid_against_oracle(simcore, torch_gpu):

@@ -2,10 +2,11 @@
 (oracle/pot_geometry.py; python/boilingsim/geometry.py:51-214, 317-339) for
 the reference's 200-step gate case (single_carrot.yaml at dx = 4 mm,
 python/tests/test_pressure_parity.py:159-224), pinned by: the grid size the
-reference's own report quotes for dx = 2 mm, an evaluation in float64 and an
-FMA-contracted float32 one giving the same mask (no cell hinges on rounding),
-a brute-force per-cell loop, and closed forms for the carrot and the water
-column."""
+reference's own report quotes for dx = 2 mm; at 4 mm the same float32 mask
+with and without FMA contraction of the cell centres, and a float64
+evaluation that differs only on the one layer whose centres lie exactly on
+the carrot's base (shown in exact arithmetic); a brute-force per-cell loop;
+and closed forms for the carrot and the water column."""
 
 from __future__ import annotations
 
