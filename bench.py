@@ -109,8 +109,8 @@ def parse():
                     help="frames of the warm-started AR(1) sequence leg (0: skip)")
     ap.add_argument("--seq-methods", default="learned,mg,pcg-fp32",
                     help="methods of the sequence leg (learned, mg, pcg-fp32, pcg-fp16)")
-    ap.add_argument("--cpu-threads", type=int, default=16,
-                    help="host threads of the CPU baseline at most (the GPU's share of the host's cores)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="host threads of the CPU baseline (0: every core in the process's affinity)")
     return ap.parse_args()
 
 
@@ -422,7 +422,7 @@ def cpu_leg(core, torch, mat, vel11, cfg, max_threads, d):
     from boilsim.fixtures import CONFIGS, DT, RHO, pot_mask, seeded_velocities
 
     aff = len(os.sched_getaffinity(0))
-    threads = max(1, min(aff, max_threads))
+    threads = aff if max_threads <= 0 else max(1, min(aff, max_threads))
     oracle.set_threads(threads)
     model = cpu_model()
     ux, uy, uz = (a.copy() for a in vel11)

@@ -802,7 +802,7 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
     // r at planes q-1, q, q+1 (updated when FOLD); raw r and Ap at q+2, q+3
     float4 Rm = fold(ldv(i0 - 2), lda(i0 - 2)), Rc = fold(ldv(i0 - 1), lda(i0 - 1)), Rn = fold(ldv(i0), lda(i0));
     float4 Rf = ldv(i0 + 1), Af = lda(i0 + 1);
-    uint32_t Kmm = ldk(i0 - 3), Km = ldk(i0 - 2), Kc = ldk(i0 - 1), Kn = ldk(i0), Kf = ldk(i0 + 1);
+    uint32_t Km = ldk(i0 - 2), Kc = ldk(i0 - 1), Kn = ldk(i0), Kf = ldk(i0 + 1);
     // z1 at planes q-1, q, q+1
     float4 Z1m = z1of(Rm, Km), Z1c = z1of(Rc, Kc), Z1n = z1of(Rn, Kn);
     float Zmm[4] = {0.f, 0.f, 0.f, 0.f}, Zm[4] = {0.f, 0.f, 0.f, 0.f};
@@ -893,7 +893,7 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
             }
         }
         Rm = Rc; Rc = Rn; Rn = fold(Rf, Af); Rf = Rg; Af = Ag;
-        Kmm = Km; Km = Kc; Kc = Kn; Kn = Kf; Kf = Kg;
+        Km = Kc; Kc = Kn; Kn = Kf; Kf = Kg;
         Z1m = Z1c; Z1c = Z1n; Z1n = z1of(Rn, Kn);
 #pragma unroll
         for (int e = 0; e < 4; ++e) { Zmm[e] = Zm[e]; Zm[e] = Zc[e]; }
@@ -997,7 +997,7 @@ __global__ __launch_bounds__(256) void prolong_post_kernel(int nx, int ny, int n
         load_coarse(m0 + 1);
     }
     float4 Zc4 = ldz(i0 - 1), Zn4 = ldz(i0), Zf4 = ldz(i0 + 1);  // z_in at planes q, q+1, q+2
-    uint32_t Kmm = 0, Km = 0, Kc = ldk(i0 - 1), Kn = ldk(i0), Kf = ldk(i0 + 1);
+    uint32_t Km = 0, Kc = ldk(i0 - 1), Kn = ldk(i0), Kf = ldk(i0 + 1);
     // The thread's coarse rows J, J2 and its clamped 4-value k window (used
     // only where jk_in).  k is a multiple of 4, so fine cell k+e reads window
     // slots 1 + e/2 and {0, 2, 1, 3}[e]: where prolong4_kernel's clamped K2
@@ -1086,7 +1086,7 @@ __global__ __launch_bounds__(256) void prolong_post_kernel(int nx, int ny, int n
             *reinterpret_cast<float4*>(z_out + (int64_t)p * pl + off) = make_float4(out[0], out[1], out[2], out[3]);
         }
         Zc4 = Zn4; Zn4 = Zf4; Zf4 = Zg4;
-        Kmm = Km; Km = Kc; Kc = Kn; Kn = Kf; Kf = Kg;
+        Km = Kc; Kc = Kn; Kn = Kf; Kf = Kg;
 #pragma unroll
         for (int e = 0; e < 4; ++e) { Tmm[e] = Tm[e]; Tm[e] = Tc[e]; }
     }

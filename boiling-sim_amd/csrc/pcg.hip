@@ -602,6 +602,13 @@ struct HostPoll {
 };
 static HostPoll g_poll;
 
+// The pinned polling block above, the flexible CG's graph cache and capture
+// stream, and the pass timer are process-wide.  Every solver entry point
+// holds this lock for the whole call, so solves issued from different host
+// threads run one after another instead of racing on them (and a graph held
+// by one solve cannot be evicted by another).
+static std::recursive_mutex g_solve_mu;
+
 static int ensure_poll() {
     if (g_poll.slots) return hipSuccess;
     void* h = nullptr;
@@ -754,6 +761,7 @@ extern "C" int bs_pressure_solve_pcg_launch(
     float* ws_z, float* ws_p_search, float* ws_Ap, float* dot_workspace, float* dev_alpha,
     float* dev_beta, float* dev_rzold, float* dev_rznew, float* dev_bsq, float* dev_rsq,
     float* dev_pAp, int* iter_count_host) {
+    std::lock_guard<std::recursive_mutex> solve_lock(g_solve_mu);
     if (nx <= 0 || ny <= 0 || nz <= 0) {
         if (iter_count_host) *iter_count_host = 0;
         return hipSuccess;
@@ -934,6 +942,7 @@ extern "C" int bs_pcg_solve(float* x, const float* div_u, const int* mat, int nx
                             float dx, float dt, float rho, int mat_fluid, int mat_air, double tol,
                             int max_iter, int warm_start, int max_restarts, void* workspace,
                             size_t workspace_bytes, hipStream_t st, bs_solve_stats* stats) {
+    std::lock_guard<std::recursive_mutex> solve_lock(g_solve_mu);
     if (stats) memset(stats, 0, sizeof(*stats));
     if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
     if (workspace_bytes < bs_pcg_workspace_bytes(nx, ny, nz)) return (int)hipErrorInvalidValue;
@@ -988,6 +997,7 @@ extern "C" int bs_pcg_solve_noalloc(float* x, const float* div_u, const int* mat
                                     float dx, float dt, float rho, int mat_fluid, int mat_air, double tol,
                                     int max_iter, int warm_start, int max_restarts, hipStream_t st,
                                     bs_solve_stats* stats) {
+    std::lock_guard<std::recursive_mutex> solve_lock(g_solve_mu);
     if (stats) memset(stats, 0, sizeof(*stats));
     if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
     if (bs_pcg_workspace_bytes(nx, ny, nz) > kScratchBytes) return (int)hipErrorInvalidValue;
@@ -1695,6 +1705,7 @@ extern "C" int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx
                             float dt, float rho, int mat_fluid, int mat_air, double tol, int max_iter,
                             int warm_start, int precond, const float* unet_params, const void* unet_packed,
                             void* workspace, size_t workspace_bytes, hipStream_t st, bs_fcg_stats* stats) {
+    std::lock_guard<std::recursive_mutex> solve_lock(g_solve_mu);
     if (stats) memset(stats, 0, sizeof(*stats));
     if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
     if (precond < 1 || precond > 6) return (int)hipErrorInvalidValue;
@@ -2131,6 +2142,7 @@ extern "C" int bs_refine_solve(double* x, float* x32, const float* div_u, const 
                                float dx, float dt, float rho, int mat_fluid, int mat_air, double tol, int max_outer,
                                int storage, double inner_tol, int max_inner, int warm_start, void* workspace,
                                size_t workspace_bytes, hipStream_t st, bs_refine_stats* stats) {
+    std::lock_guard<std::recursive_mutex> solve_lock(g_solve_mu);
     if (stats) memset(stats, 0, sizeof(*stats));
     if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
     if (storage < 0 || storage > 2 || max_outer < 0) return (int)hipErrorInvalidValue;
@@ -2207,6 +2219,7 @@ extern "C" int bs_fcg_refine_solve(double* x, float* x32, const float* div_u, co
                                    int max_outer, double inner_tol, int max_inner, int warm_start, int precond,
                                    const float* unet_params, const void* unet_packed, void* workspace,
                                    size_t workspace_bytes, hipStream_t st, bs_refine_stats* stats) {
+    std::lock_guard<std::recursive_mutex> solve_lock(g_solve_mu);
     if (stats) memset(stats, 0, sizeof(*stats));
     if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
     if (precond < 1 || precond > 6 || max_outer < 0) return (int)hipErrorInvalidValue;
@@ -2327,6 +2340,7 @@ extern "C" int bs_fcg_refine_solve(double* x, float* x32, const float* div_u, co
 // the median and mean launch time in microseconds and the number of launches
 // timed (arrays of 3; null skips).
 extern "C" int bs_fcg_pass_timing(int enable, double* us_median, double* us_mean, int* counts) {
+    std::lock_guard<std::recursive_mutex> solve_lock(g_solve_mu);
     using namespace bs;
     if (enable) {
         if (!g_timer.created) {

@@ -10,7 +10,11 @@
  * (crates/cuda-kernels/src/pressure_solve_pcg.cu:82-88).  Return values are
  * hipError_t codes (0 = success), as the reference's launchers return
  * cudaError_t.  Entry points without a `stream` argument use the null stream,
- * like the reference's default-stream launchers.
+ * like the reference's default-stream launchers.  The iterative solvers
+ * (bs_pressure_solve_pcg_launch, bs_pcg_solve, bs_pcg_solve_noalloc,
+ * bs_fcg_solve, bs_refine_solve, bs_fcg_refine_solve) share process-wide
+ * polling and graph state and hold one lock for the whole call: calls from
+ * several host threads run one after another.
  */
 #ifndef BOILSIM_HIP_H
 #define BOILSIM_HIP_H

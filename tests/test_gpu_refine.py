@@ -70,6 +70,36 @@ def test_refine_divergence_rhs(simcore, torch_gpu, storage):
     assert not xh[~fl].any()
 
 
+@pytest.mark.parametrize("storage", ["fp16", "bf16"])
+def test_refine_16bit_production_grid(simcore, torch_gpu, storage):
+    """440x440x280 pot, divergence of the seed-11 faces (bench.py's
+    mixed-precision leg): the refinement around 16-bit inner vectors reaches a
+    true 1e-6 by the oracle's FP64 residual of the returned FP64 iterate, and
+    the residual it reports is the oracle's to 1e-9 relative."""
+    torch = torch_gpu
+    from boilsim.fixtures import CONFIGS, DT as DT_P, RHO as RHO_P, pot_mask, seeded_velocities
+
+    cfg = CONFIGS["prod"]
+    shape = cfg.shape
+    print(f"[refine 440] {storage}", flush=True)
+    mat = pot_mask(*shape)
+    ux, uy, uz = seeded_velocities(*shape, seed=11, plume=False)
+    div = oracle.divergence(ux, uy, uz, mat, cfg.dx)
+    del ux, uy, uz
+    mat_d, div_d = dev(torch, mat), dev(torch, div)
+    ws = torch.empty(simcore.refine_workspace_bytes(*shape), dtype=torch.uint8, device="cuda")
+    x = torch.zeros(shape, dtype=torch.float64, device="cuda")
+    st = simcore.refine_solve(x.data_ptr(), div_d.data_ptr(), mat_d.data_ptr(), *shape, cfg.dx, DT_P, RHO_P, 1e-6,
+                              ws.data_ptr(), ws.numel(), storage=storage)
+    torch.cuda.synchronize()
+    xh = host(x)
+    rel2, _ = oracle.residual_f64(xh, b32(div, mat, cfg.dx, DT_P, RHO_P), mat)
+    print(f"  {st}; oracle relres {rel2:.3e}", flush=True)
+    assert st["converged"] == 1 and rel2 <= 1.0001e-6
+    assert abs(rel2 - st["relres_true"]) <= 1e-9 * rel2
+    assert not xh[mat != 0].any()
+
+
 def test_refine_smooth_rhs_beats_fp32_floor(simcore, torch_gpu):
     """110x110x70 pot with a smooth right-hand side (Gaussian random field,
     correlation length 12 cells): the FP32 true-residual PCG stalls above
