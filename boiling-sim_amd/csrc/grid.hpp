@@ -351,10 +351,10 @@ __device__ __forceinline__ double wave_sum64(double v) {
     return v;
 }
 
-// Workgroup sums of NV values; results valid in every thread.
-template <int NV>
+// Sums of NV values over a workgroup of NT threads; results valid in every thread.
+template <int NV, int NT = kThreads>
 __device__ __forceinline__ void block_sums(double (&v)[NV]) {
-    __shared__ double red[NV][kThreads / 64];
+    __shared__ double red[NV][NT / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
@@ -366,16 +366,16 @@ __device__ __forceinline__ void block_sums(double (&v)[NV]) {
     for (int k = 0; k < NV; ++k) {
         double s = red[k][0];
 #pragma unroll
-        for (int w = 1; w < kThreads / 64; ++w) s += red[k][w];
+        for (int w = 1; w < NT / 64; ++w) s += red[k][w];
         v[k] = s;
     }
     __syncthreads();
 }
 
 // Writes this workgroup's NV partials.
-template <int NV>
+template <int NV, int NT = kThreads>
 __device__ __forceinline__ void write_partials(double (&v)[NV], double* __restrict__ partials) {
-    block_sums<NV>(v);
+    block_sums<NV, NT>(v);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < NV; ++k) partials[(size_t)k * gridDim.x + blockIdx.x] = v[k];
