@@ -140,21 +140,43 @@ inline ConvTiles conv_tiles(const Lvl& L) {
                      (uint32_t)((L.nz + kTK - 1) / kTK)};
 }
 
+// Inference-only extras of the first encoder convolutions.  FEAT: the input
+// is not a tensor but the network's features of the residual, formed while
+// staging the halo exactly as features_kernel forms them (r/rms on fluid
+// cells, the one-hot class; rms from the r.r partials, summed by every
+// workgroup in the same order), so the feature tensor never goes to memory;
+// workgroup 0 also stores rms.  POOL: the 2x2x2 average of the block's
+// (bf16-rounded) outputs into the coarser grid, pool_kernel's float
+// operations: the blocks are even-aligned, so every coarse cell's children
+// lie in one block.
+struct ConvExtra {
+    const float* r = nullptr;
+    const uint8_t* code = nullptr;
+    const double* rr_part = nullptr;
+    unsigned G = 0;
+    double n_fluid = 0.0;
+    float* rms_out = nullptr;
+    __bf16* pool_out = nullptr;
+    int cnx = 0, cny = 0, cnz = 0;
+};
+
 // UPIN: `in` lives on the 2x-coarser grid (ny, nz = cny, cnz) and the
 // convolution reads its nearest-neighbour upsampling, in[i/2][j/2][k/2], as
 // up_kernel would have written it (the upsampled tensor never goes to
 // memory).
-template <int CIN, int COUT, bool UPIN = false>
+template <int CIN, int COUT, bool UPIN = false, bool FEAT = false, bool POOL = false>
 __global__ __launch_bounds__(256) void conv_mfma_kernel(
     Lvl L, ConvTiles T, const __bf16* __restrict__ in, const bf16x8* __restrict__ wfrag,
     const float* __restrict__ bias, int relu, __bf16* __restrict__ out, const __bf16* __restrict__ skip,
-    __bf16* __restrict__ out_sum, const int* __restrict__ done, int cny = 0, int cnz = 0) {
+    __bf16* __restrict__ out_sum, const int* __restrict__ done, int cny = 0, int cnz = 0, ConvExtra X = ConvExtra{}) {
     if (done && *done) return;
     static_assert(CIN % 8 == 0, "channels come in groups of 8");
+    static_assert(!FEAT || CIN == 8, "the features are 8 channels");
     constexpr int KS = ksteps_of(CIN);
     constexpr int MT = mtiles_of(COUT);
     constexpr int NG = CIN / 8;  // channel groups
     __shared__ bf16x8 tile[NG * kHalo];
+    __shared__ __align__(16) __bf16 otile[POOL ? kTI * kTJ * kTK * COUT : 8];
     const uint32_t nblk = gridDim.x;
     const uint32_t b = xcd_remap(blockIdx.x, nblk);
     const int tk = (int)(b % T.tk);
@@ -162,6 +184,14 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(
     const int tj = (int)(r % T.tj), ti = (int)(r / T.tj);
     const int i0 = ti * kTI, j0 = tj * kTJ, k0 = tk * kTK;
     const int64_t plane = (int64_t)L.ny * L.nz;
+    float inv = 0.0f;
+    if constexpr (FEAT) {
+        double v[1] = {0.0};
+        if (X.rr_part) sum_partials<1>(X.rr_part, X.G, v);
+        const double rms = X.rr_part ? sqrt(v[0] / (X.n_fluid > 0 ? X.n_fluid : 1.0)) : 1.0;
+        inv = rms > 0.0 ? (float)(1.0 / rms) : 0.0f;
+        if (blockIdx.x == 0 && threadIdx.x == 0 && X.rms_out) *X.rms_out = (float)rms;
+    }
 
     // stage the halo block: element v = (cell h, group q), consecutive
     // threads on consecutive k of one row
@@ -172,9 +202,21 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(
         const int gi = i0 - 1 + hi, gj = j0 - 1 + hj, gk = k0 - 1 + hk;
         bf16x8 val = bf16x8{};
         if ((unsigned)gi < (unsigned)L.nx && (unsigned)gj < (unsigned)L.ny && (unsigned)gk < (unsigned)L.nz) {
-            const int64_t src = UPIN ? ((int64_t)(gi >> 1) * cny + (gj >> 1)) * cnz + (gk >> 1)
-                                     : (int64_t)gi * plane + (int64_t)gj * L.nz + gk;
-            val = *reinterpret_cast<const bf16x8*>(in + src * CIN + q * 8);
+            if constexpr (FEAT) {
+                const int64_t cell = (int64_t)gi * plane + (int64_t)gj * L.nz + gk;
+                const uint8_t cc = X.code[cell];
+                __bf16 f[8];
+                f[0] = (__bf16)(cc >= kCodeFluid0 ? X.r[cell] * inv : 0.0f);
+                f[1] = (__bf16)(cc >= kCodeFluid0 ? 1.0f : 0.0f);
+                f[2] = (__bf16)(cc == kCodeAir ? 1.0f : 0.0f);
+                f[3] = (__bf16)(cc == kCodeSolid ? 1.0f : 0.0f);
+                f[4] = f[5] = f[6] = f[7] = (__bf16)0.0f;
+                val = *reinterpret_cast<const bf16x8*>(f);
+            } else {
+                const int64_t src = UPIN ? ((int64_t)(gi >> 1) * cny + (gj >> 1)) * cnz + (gk >> 1)
+                                         : (int64_t)gi * plane + (int64_t)gj * L.nz + gk;
+                val = *reinterpret_cast<const bf16x8*>(in + src * CIN + q * 8);
+            }
         }
         tile[q * kHalo + h] = val;
     }
@@ -212,9 +254,11 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(
 
     // Epilogue: lane holds rows (channels) 4g..4g+3 of tile mt for cell c of row nt.
     const int ci = i0 + w, ck = k0 + c;
-    if (ci >= L.nx || ck >= L.nz) return;
+    const bool ik_in = ci < L.nx && ck < L.nz;
+    if (!POOL && !ik_in) return;
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
+        if (!ik_in) break;
         const int co0 = mt * 16 + g * 4;
         if (co0 >= COUT) continue;
         float bv[4];
@@ -234,12 +278,50 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(
             }
             const int64_t o = ((int64_t)ci * plane + (int64_t)cj * L.nz + ck) * COUT + co0;
             if (out) *reinterpret_cast<uint2*>(out + o) = *reinterpret_cast<const uint2*>(o4);
+            if constexpr (POOL)
+                *reinterpret_cast<uint2*>(otile + ((w * kTJ + nt) * kTK + c) * COUT + co0) =
+                    *reinterpret_cast<const uint2*>(o4);
             if (skip) {
                 const uint2 sk = *reinterpret_cast<const uint2*>(skip + o);
                 const __bf16* skv = reinterpret_cast<const __bf16*>(&sk);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) s4[q] = (__bf16)((float)o4[q] + (float)skv[q]);
                 *reinterpret_cast<uint2*>(out_sum + o) = *reinterpret_cast<const uint2*>(s4);
+            }
+        }
+    }
+    if constexpr (POOL) {
+        __syncthreads();
+        // 2 x 2 x 8 coarse cells under the block, 8-channel groups
+        constexpr int G8 = COUT / 8;
+        const int t = threadIdx.x;
+        if (t < 32 * G8) {
+            const int lc = t / G8, gq = t - (t / G8) * G8;
+            const int li = lc >> 4, lj = (lc >> 3) & 1, lk = lc & 7;
+            const int I = i0 / 2 + li, J = j0 / 2 + lj, K = k0 / 2 + lk;
+            if (I < X.cnx && J < X.cny && K < X.cnz) {
+                float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                int cnt = 0;
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+                        for (int d = 0; d < 2; ++d) {
+                            const int fi = 2 * li + a, fj = 2 * lj + bb, fk = 2 * lk + d;
+                            if (i0 + fi >= L.nx || j0 + fj >= L.ny || k0 + fk >= L.nz) continue;
+                            const bf16x8 x =
+                                *reinterpret_cast<const bf16x8*>(otile + ((fi * kTJ + fj) * kTK + fk) * COUT + gq * 8);
+#pragma unroll
+                            for (int q = 0; q < 8; ++q) acc[q] += (float)x[q];
+                            ++cnt;
+                        }
+                const float pinv = 1.0f / (float)cnt;
+                __bf16 o[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) o[q] = (__bf16)(acc[q] * pinv);
+                const int64_t cc = ((int64_t)I * X.cny + J) * X.cnz + K;
+                *reinterpret_cast<uint4*>(X.pool_out + cc * COUT + gq * 8) = *reinterpret_cast<const uint4*>(o);
             }
         }
     }
@@ -802,6 +884,17 @@ static void conv_fwd(hipStream_t st, const Lvl& L, const __bf16* in, const __bf1
                                                                         relu, out, skip, out_sum, done);
 }
 
+// conv_fwd with the inference extras (ConvExtra: features formed while
+// staging, pooled output).
+template <int CIN, int COUT, bool FEAT, bool POOL>
+static void conv_fwd_x(hipStream_t st, const Lvl& L, const __bf16* in, const __bf16* packed, int l,
+                       const float* params, int relu, __bf16* out, const int* done, const ConvExtra& X) {
+    const ConvTiles T = conv_tiles(L);
+    const bf16x8* wf = reinterpret_cast<const bf16x8*>(packed + frag_off(l, 0));
+    conv_mfma_kernel<CIN, COUT, false, FEAT, POOL><<<T.ti * T.tj * T.tk, 256, 0, st>>>(
+        L, T, in, wf, params + conv_b_off(l), relu, out, nullptr, nullptr, done, 0, 0, X);
+}
+
 // Input gradient of conv l: the forward kernel on g with the flipped weights.
 template <int GC, int OC>
 static void conv_bwd_data(hipStream_t st, const Lvl& L, const __bf16* g, const __bf16* packed, int l,
@@ -835,11 +928,23 @@ static int unet_forward(hipStream_t st, UNetWs& W, const float* params, const __
         rr_partials = W.part;
         G = kPartWg;
     }
-    features_kernel<<<eg1, 256, 0, st>>>(W.L1, r, code, rr_partials, G, n_fluid, W.rms, W.x0, done);
-    conv_fwd<8, 8>(st, W.L1, W.x0, packed, 0, params, 1, W.h1, nullptr, nullptr, done);
-    pool_kernel<8><<<elem_grid(W.L2.n), 256, 0, st>>>(W.L1, W.L2, W.h1, W.p1, done);
-    conv_fwd<8, 16>(st, W.L2, W.p1, packed, 1, params, 1, W.h2, nullptr, nullptr, done);
-    pool_kernel<16><<<elem_grid(W.L3.n * 2), 256, 0, st>>>(W.L2, W.L3, W.h2, W.p2, done);
+    if (training) {
+        features_kernel<<<eg1, 256, 0, st>>>(W.L1, r, code, rr_partials, G, n_fluid, W.rms, W.x0, done);
+        conv_fwd<8, 8>(st, W.L1, W.x0, packed, 0, params, 1, W.h1, nullptr, nullptr, done);
+        pool_kernel<8><<<elem_grid(W.L2.n), 256, 0, st>>>(W.L1, W.L2, W.h1, W.p1, done);
+        conv_fwd<8, 16>(st, W.L2, W.p1, packed, 1, params, 1, W.h2, nullptr, nullptr, done);
+        pool_kernel<16><<<elem_grid(W.L3.n * 2), 256, 0, st>>>(W.L2, W.L3, W.h2, W.p2, done);
+    } else {
+        // the same values without the feature tensor and the two pooling
+        // passes (GPU test: bit-identical to the training forward)
+        ConvExtra X1;
+        X1.r = r; X1.code = code; X1.rr_part = rr_partials; X1.G = G; X1.n_fluid = n_fluid; X1.rms_out = W.rms;
+        X1.pool_out = W.p1; X1.cnx = W.L2.nx; X1.cny = W.L2.ny; X1.cnz = W.L2.nz;
+        conv_fwd_x<8, 8, true, true>(st, W.L1, nullptr, packed, 0, params, 1, W.h1, done, X1);
+        ConvExtra X2;
+        X2.pool_out = W.p2; X2.cnx = W.L3.nx; X2.cny = W.L3.ny; X2.cnz = W.L3.nz;
+        conv_fwd_x<8, 16, false, true>(st, W.L2, W.p1, packed, 1, params, 1, W.h2, done, X2);
+    }
     conv_fwd<16, 32>(st, W.L3, W.p2, packed, 2, params, 1, W.h3a, nullptr, nullptr, done);
     conv_fwd<32, 32>(st, W.L3, W.h3a, packed, 3, params, 1, W.h3, nullptr, nullptr, done);
     // The decoder convolutions read their coarser input upsampled on the fly;

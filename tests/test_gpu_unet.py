@@ -73,9 +73,12 @@ def test_forward_backward_match_oracle(torch_gpu, shape, seed):
 @pytest.mark.parametrize("shape,seed", [((20, 18, 14), 4), ((11, 13, 9), 5), ((110, 110, 70), 6)])
 def test_inference_forward_matches_training_forward(torch_gpu, shape, seed):
     """Inference reads the decoder's upsampled inputs on the fly inside the
-    convolutions' staging; training writes them with up_kernel (its backward
-    needs them).  Nearest-neighbour upsampling copies values, so the two
-    forward passes agree bit for bit; with the oracle comparison of the
+    convolutions' staging, forms the features of r while staging the first
+    convolution's input and pools inside the two encoder convolutions;
+    training writes the feature, upsampled and pooled tensors with their own
+    kernels (its backward needs them).  The same values and float operations,
+    so the two forward passes agree bit for bit, including odd grid sizes
+    (coarse cells with missing children); with the oracle comparison of the
     training forward above this pins the inference path too."""
     torch = torch_gpu
     from boilsim.unet import UNet, init_params
