@@ -405,12 +405,24 @@ __global__ __launch_bounds__(256) void smooth_kernel(Tiles T, const uint8_t* __r
 // Chebyshev semi-iteration step on A x = r with the Jacobi splitting:
 // x_out = x_prev + om (x_cur - x_prev + ga D^-1 (r - A x_cur)) on fluid
 // cells, 0 elsewhere (x_out may be x_prev: read and written per cell).
+// gmode 1 / 2: the step reads x_cur and x_prev (1) or x_prev alone (2) as
+// stored by step_scale_kernel, gamma * value, with gamma formed from the
+// step partials (G per value) as that kernel forms it: the first two steps
+// of a polish that starts from the network's A-norm-optimal correction, with
+// the scaling pass folded in (the same float products, the same bits).
 template <int VEC>
 __global__ __launch_bounds__(256) void cheb_kernel(Tiles T, const uint8_t* __restrict__ code,
                                                    const float* __restrict__ r, const float* __restrict__ x_cur,
                                                    const float* x_prev, float* x_out, float om, float ga,
-                                                   const int* __restrict__ done) {
+                                                   const int* __restrict__ done, const double* __restrict__ gpart,
+                                                   unsigned gG, int gmode) {
     if (done && *done) return;
+    float gam = 1.0f;
+    if (gmode) {  // every thread of the workgroup, before any returns
+        double v[2];
+        sum_partials<2>(gpart, gG, v);
+        gam = (v[0] > 0.0 && v[1] > 0.0) ? (float)(v[0] / v[1]) : 0.0f;
+    }
     int i, j, k0;
     int64_t c0;
     if (!one_tile(T, VEC, i, j, k0, c0)) return;
@@ -419,6 +431,18 @@ __global__ __launch_bounds__(256) void cheb_kernel(Tiles T, const uint8_t* __res
     float rv[VEC], pv[VEC], out[VEC];
     ldf<VEC>(r, c0, rv);
     ldf<VEC>(x_prev, c0, pv);
+    if (gmode == 1) {
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            nb.vc[e] *= gam; nb.vl[e] *= gam; nb.vr[e] *= gam; nb.vd[e] *= gam; nb.vu[e] *= gam;
+        }
+        nb.vb *= gam;
+        nb.vf *= gam;
+    }
+    if (gmode) {
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) pv[e] *= gam;
+    }
 #pragma unroll
     for (int e = 0; e < VEC; ++e) {
         const uint8_t cc = nb.cc[e];
@@ -436,7 +460,8 @@ __global__ __launch_bounds__(256) void cheb_kernel(Tiles T, const uint8_t* __res
 // the Chebyshev semi-iterative method on the Jacobi splitting (Golub and
 // Varga).  Graph-capturable: the coefficients are kernel arguments.
 static void chebyshev(hipStream_t st, const Tiles& T, int vec, const uint8_t* code, const float* r, float* x,
-                      float* tmp, int steps, double lo, const int* done);
+                      float* tmp, int steps, double lo, const int* done, const double* gpart = nullptr,
+                      unsigned gG = 0);
 
 // Last smoothing sweep of the V-cycle fused with the flexible-CG products:
 // zo = zi + w D^-1 (r - A zi) and one partial of r.zo and of r.z_old per tile.
@@ -1385,8 +1410,11 @@ static int pre_restrict(hipStream_t st, const Tiles& TF, int vf, const MLvl& LF,
     return (int)hipGetLastError();
 }
 
+// gpart: x holds the network's correction before its A-norm-optimal scaling,
+// whose step partials (gG per value) are given: the first two steps apply
+// the scaling on load (gmode of cheb_kernel).
 static void chebyshev(hipStream_t st, const Tiles& T, int vec, const uint8_t* code, const float* r, float* x,
-                      float* tmp, int steps, double lo, const int* done) {
+                      float* tmp, int steps, double lo, const int* done, const double* gpart, unsigned gG) {
     const double hi = 2.0;
     const double th = 0.5 * (hi + lo), sg = th / (0.5 * (hi - lo));
     const float ga = (float)(1.0 / th);
@@ -1397,7 +1425,8 @@ static void chebyshev(hipStream_t st, const Tiles& T, int vec, const uint8_t* co
         if (q == 1) om = 1.0 / (1.0 - 1.0 / (2.0 * sg * sg));
         else if (q > 1) om = 1.0 / (1.0 - om / (4.0 * sg * sg));
         const float* xp = q == 0 ? cur : prv;  // step 0 has no previous iterate
-        MG_VEC(cheb_kernel, vec, T.ntiles, st, T, code, r, cur, xp, prv, (float)om, ga, done);
+        const int gm = gpart ? (q == 0 ? 1 : q == 1 ? 2 : 0) : 0;
+        MG_VEC(cheb_kernel, vec, T.ntiles, st, T, code, r, cur, xp, prv, (float)om, ga, done, gpart, gG, gm);
         float* t = cur;
         cur = prv;
         prv = t;
@@ -1484,8 +1513,10 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
         rc = unet_apply(st, W.unet, W.L2.nx, W.L2.ny, W.L2.nz, params, packed, W.r2, W.code2, n_fluid_coarse,
                         nullptr, 0, W.e2, nullptr, nullptr, done);
         if (rc != hipSuccess) return rc;
-        optimal_step(st, W.L2, W.code2, W.r2, W.e2, W.part, done);
-        polish();
+        // the A-norm-optimal step's products; the polish's first two steps
+        // apply the scaling (optimal_step's second pass folded in)
+        step_partials_kernel<<<kStepWg, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, W.part, done);
+        chebyshev(st, T2, v2, W.code2, W.r2, W.e2, W.zt2, kChebSteps, kChebLo, done, W.part, kStepWg);
     } else {
         jacobi0_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, done);
         jacobi_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, W.zt2, done);
