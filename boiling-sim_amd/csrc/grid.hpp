@@ -285,6 +285,30 @@ struct Nbhd {
         if (k0 + VEC < T.nz) { cf = code[c0 + VEC]; vf = ld1(v, c0 + VEC); }
     }
 
+    // load() for values that are exactly +0 on every non-fluid cell (and
+    // outside the grid): the neighbours' codes are not read (apply_mg then
+    // needs only the centre's), they are set to a fluid code.
+    template <typename S>
+    __device__ __forceinline__ void load_v(const Tiles& T, const uint8_t* __restrict__ code,
+                                           const S* __restrict__ v, int i, int j, int k0, int64_t c0) {
+        const int64_t plane = (int64_t)T.ny * T.nz;
+        ldc<VEC>(code, c0, cc);
+        ldv<VEC>(v, c0, vc);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            cl[e] = cr[e] = cd[e] = cu[e] = kCodeFluid0;
+            vl[e] = vr[e] = vd[e] = vu[e] = 0.0f;
+        }
+        if (i > 0) ldv<VEC>(v, c0 - plane, vl);
+        if (i < T.nx - 1) ldv<VEC>(v, c0 + plane, vr);
+        if (j > 0) ldv<VEC>(v, c0 - T.nz, vd);
+        if (j < T.ny - 1) ldv<VEC>(v, c0 + T.nz, vu);
+        cb = cf = kCodeFluid0;
+        vb = vf = 0.0f;
+        if (k0 > 0) vb = ld1(v, c0 - 1);
+        if (k0 + VEC < T.nz) vf = ld1(v, c0 + VEC);
+    }
+
     // 6*v - S(v) at element e in the reference's float order
     // (laplacian_spmv.cu:120-142); 0 for a non-fluid centre.
     __device__ __forceinline__ float apply(int e) const {

@@ -405,6 +405,8 @@ __global__ __launch_bounds__(256) void smooth_kernel(Tiles T, const uint8_t* __r
 // Chebyshev semi-iteration step on A x = r with the Jacobi splitting:
 // x_out = x_prev + om (x_cur - x_prev + ga D^-1 (r - A x_cur)) on fluid
 // cells, 0 elsewhere (x_out may be x_prev: read and written per cell).
+// znf: x_cur is exactly +0 on every non-fluid cell (Nbhd::load_v, no
+// neighbour codes); the caller's arrays otherwise.
 // gmode 1 / 2: the step reads x_cur and x_prev (1) or x_prev alone (2) as
 // stored by step_scale_kernel, gamma * value, with gamma formed from the
 // step partials (G per value) as that kernel forms it: the first two steps
@@ -415,7 +417,7 @@ __global__ __launch_bounds__(256) void cheb_kernel(Tiles T, const uint8_t* __res
                                                    const float* __restrict__ r, const float* __restrict__ x_cur,
                                                    const float* x_prev, float* x_out, float om, float ga,
                                                    const int* __restrict__ done, const double* __restrict__ gpart,
-                                                   unsigned gG, int gmode) {
+                                                   unsigned gG, int gmode, int znf) {
     if (done && *done) return;
     float gam = 1.0f;
     if (gmode) {  // every thread of the workgroup, before any returns
@@ -427,7 +429,8 @@ __global__ __launch_bounds__(256) void cheb_kernel(Tiles T, const uint8_t* __res
     int64_t c0;
     if (!one_tile(T, VEC, i, j, k0, c0)) return;
     Nbhd<VEC> nb;
-    nb.load(T, code, x_cur, i, j, k0, c0);
+    if (znf) nb.load_v(T, code, x_cur, i, j, k0, c0);
+    else nb.load(T, code, x_cur, i, j, k0, c0);
     float rv[VEC], pv[VEC], out[VEC];
     ldf<VEC>(r, c0, rv);
     ldf<VEC>(x_prev, c0, pv);
@@ -461,7 +464,7 @@ __global__ __launch_bounds__(256) void cheb_kernel(Tiles T, const uint8_t* __res
 // Varga).  Graph-capturable: the coefficients are kernel arguments.
 static void chebyshev(hipStream_t st, const Tiles& T, int vec, const uint8_t* code, const float* r, float* x,
                       float* tmp, int steps, double lo, const int* done, const double* gpart = nullptr,
-                      unsigned gG = 0);
+                      unsigned gG = 0, bool znf = true);
 
 // Last smoothing sweep of the V-cycle fused with the flexible-CG products:
 // zo = zi + w D^-1 (r - A zi) and one partial of r.zo and of r.z_old per tile.
@@ -1412,9 +1415,13 @@ static int pre_restrict(hipStream_t st, const Tiles& TF, int vf, const MLvl& LF,
 
 // gpart: x holds the network's correction before its A-norm-optimal scaling,
 // whose step partials (gG per value) are given: the first two steps apply
-// the scaling on load (gmode of cheb_kernel).
+// the scaling on load (gmode of cheb_kernel).  znf: x is exactly +0 on every
+// non-fluid cell, as every iterate of the V-cycle is (its kernels store +0
+// there), so the steps read no neighbour codes; the steps keep that
+// property, so tmp need not have it.
 static void chebyshev(hipStream_t st, const Tiles& T, int vec, const uint8_t* code, const float* r, float* x,
-                      float* tmp, int steps, double lo, const int* done, const double* gpart, unsigned gG) {
+                      float* tmp, int steps, double lo, const int* done, const double* gpart, unsigned gG,
+                      bool znf) {
     const double hi = 2.0;
     const double th = 0.5 * (hi + lo), sg = th / (0.5 * (hi - lo));
     const float ga = (float)(1.0 / th);
@@ -1426,7 +1433,8 @@ static void chebyshev(hipStream_t st, const Tiles& T, int vec, const uint8_t* co
         else if (q > 1) om = 1.0 / (1.0 - om / (4.0 * sg * sg));
         const float* xp = q == 0 ? cur : prv;  // step 0 has no previous iterate
         const int gm = gpart ? (q == 0 ? 1 : q == 1 ? 2 : 0) : 0;
-        MG_VEC(cheb_kernel, vec, T.ntiles, st, T, code, r, cur, xp, prv, (float)om, ga, done, gpart, gG, gm);
+        MG_VEC(cheb_kernel, vec, T.ntiles, st, T, code, r, cur, xp, prv, (float)om, ga, done, gpart, gG, gm,
+               znf ? 1 : 0);
         float* t = cur;
         cur = prv;
         prv = t;
@@ -1622,6 +1630,6 @@ extern "C" int bs_mg_chebyshev(const unsigned char* code, const float* r, float*
     if (steps < 0 || (steps & 1) || !(lo > 0.0 && lo < 2.0)) return (int)hipErrorInvalidValue;
     const int v = vec_for(nz, {r, x, tmp, code});
     const Tiles T = make_tiles(nx, ny, nz, v);
-    chebyshev((hipStream_t)stream, T, v, code, r, x, tmp, steps, lo, nullptr);
+    chebyshev((hipStream_t)stream, T, v, code, r, x, tmp, steps, lo, nullptr, nullptr, 0, false);
     return (int)hipGetLastError();
 }
