@@ -74,7 +74,7 @@ TRAFFIC_JACOBI_PAIR = 14.207
 # The learned iteration's fine-level passes, measured the same way over
 # `bench.py --legs passes` (reads = FETCH_SIZE x 1.99965 from max_abs_kernel,
 # writes = WRITE_SIZE; dispatches that returned at once left out).
-TRAFFIC_PASSES = {"fcg_spmv_dir_kernel": 28.26, "pre2_restrict_kernel<true>": 22.58, "prolong_post_kernel": 16.34}
+TRAFFIC_PASSES = {"fcg_spmv_dir_kernel": 27.40, "pre2_restrict_kernel<true>": 22.59, "prolong_post_kernel": 16.35}
 
 
 # The same flexible CG and V-cycle with non-learned solves of the 4x-coarsened
