@@ -68,38 +68,46 @@ __global__ __launch_bounds__(256) void conduction_sweep_kernel(Tiles T, float* _
                                                                const float* __restrict__ cp_arr, CondParams P) {
     __shared__ CondTable tb;
     build_table(tb, k_arr, rho_arr, cp_arr, P);
-    __syncthreads();
-    int i, j, k0;
-    int64_t c0;
-    if (!one_tile(T, VEC, i, j, k0, c0)) return;
+    // The sweep's own loads are issued before the table's barrier, so the two
+    // latencies overlap; every thread reaches the barrier.
+    int i = 0, j = 0, k0 = 0;
+    int64_t c0 = 0;
+    const bool valid = one_tile(T, VEC, i, j, k0, c0);
     const int64_t plane = (int64_t)T.ny * T.nz;
     const int ma = P.mat_air, nm = P.n_mat;
     int mc[VEC];
-    ldi<VEC>(mat, c0, mc);
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) mc[e] = ma;
+    if (valid) ldi<VEC>(mat, c0, mc);
     bool any = false;
 #pragma unroll
     for (int e = 0; e < VEC; ++e) any |= mc[e] != ma;
-    if (!any) return;  // all air: nothing is written
-    float tc[VEC], to[VEC];
-    ldf<VEC>(t_iter, c0, tc);
-    ldf<VEC>(t_old, c0, to);
+    const bool work = valid && any;  // all air: nothing is written
     // neighbour columns (absent ones flagged by have_*)
     const bool have_l = i > 0, have_r = i < T.nx - 1, have_d = j > 0, have_u = j < T.ny - 1;
+    float tc[VEC], to[VEC];
     int ml[VEC], mr[VEC], md[VEC], mu[VEC];
     float tl[VEC], tr[VEC], td[VEC], tu[VEC];
 #pragma unroll
     for (int e = 0; e < VEC; ++e) {
+        tc[e] = to[e] = 0.0f;
         ml[e] = mr[e] = md[e] = mu[e] = 0;
         tl[e] = tr[e] = td[e] = tu[e] = 0.0f;
     }
-    if (have_l) { ldi<VEC>(mat, c0 - plane, ml); ldf<VEC>(t_iter, c0 - plane, tl); }
-    if (have_r) { ldi<VEC>(mat, c0 + plane, mr); ldf<VEC>(t_iter, c0 + plane, tr); }
-    if (have_d) { ldi<VEC>(mat, c0 - T.nz, md); ldf<VEC>(t_iter, c0 - T.nz, td); }
-    if (have_u) { ldi<VEC>(mat, c0 + T.nz, mu); ldf<VEC>(t_iter, c0 + T.nz, tu); }
     int mb = 0, mf = 0;
     float tb_ = 0.0f, tf = 0.0f;
-    if (k0 > 0) { mb = mat[c0 - 1]; tb_ = t_iter[c0 - 1]; }
-    if (k0 + VEC < T.nz) { mf = mat[c0 + VEC]; tf = t_iter[c0 + VEC]; }
+    if (work) {
+        ldf<VEC>(t_iter, c0, tc);
+        ldf<VEC>(t_old, c0, to);
+        if (have_l) { ldi<VEC>(mat, c0 - plane, ml); ldf<VEC>(t_iter, c0 - plane, tl); }
+        if (have_r) { ldi<VEC>(mat, c0 + plane, mr); ldf<VEC>(t_iter, c0 + plane, tr); }
+        if (have_d) { ldi<VEC>(mat, c0 - T.nz, md); ldf<VEC>(t_iter, c0 - T.nz, td); }
+        if (have_u) { ldi<VEC>(mat, c0 + T.nz, mu); ldf<VEC>(t_iter, c0 + T.nz, tu); }
+        if (k0 > 0) { mb = mat[c0 - 1]; tb_ = t_iter[c0 - 1]; }
+        if (k0 + VEC < T.nz) { mf = mat[c0 + VEC]; tf = t_iter[c0 + VEC]; }
+    }
+    __syncthreads();
+    if (!work) return;
     float out[VEC];
 #pragma unroll
     for (int e = 0; e < VEC; ++e) {
