@@ -1580,7 +1580,9 @@ extern "C" int bs_mg_prolong_post(const unsigned char* code, const float* z_in, 
                                   const float* z_old, float* z_out, float* tmp, double* part, size_t part_len,
                                   int* groups, int nx, int ny, int nz, int fused, double w, bs_stream_t stream) {
     using namespace bs;
-    if (nx <= 0 || ny <= 0 || nz <= 0 || !groups) return (int)hipErrorInvalidValue;
+    if (!groups) return (int)hipErrorInvalidValue;
+    *groups = 0;
+    if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
     hipStream_t st = (hipStream_t)stream;
     const MLvl LF = mk(nx, ny, nz);
     const MLvl LC = mk((nx + 1) / 2, (ny + 1) / 2, (nz + 1) / 2);

@@ -14,7 +14,12 @@
  * (bs_pressure_solve_pcg_launch, bs_pcg_solve, bs_pcg_solve_noalloc,
  * bs_fcg_solve, bs_refine_solve, bs_fcg_refine_solve) share process-wide
  * polling and graph state and hold one lock for the whole call: calls from
- * several host threads run one after another.
+ * several host threads run one after another.  A grid with a dimension <= 0
+ * is a silent no-op, as in the reference's launchers
+ * (crates/cuda-kernels/src/jacobi_pressure.cu:193, 238;
+ * pressure_solve_pcg.cu:120-123): no buffer is touched except that the
+ * statistics, iteration counts and device maxima (max|div|, max|u|) an entry
+ * point reports read 0.
  */
 #ifndef BOILSIM_HIP_H
 #define BOILSIM_HIP_H
@@ -327,7 +332,8 @@ int bs_mg_smooth_restrict(const unsigned char* code, const float* r, float* z, f
  * null), *groups = G.  fused = 1 runs the single-pass kernel the V-cycle uses
  * (nz % 4 == 0, 16-byte aligned vectors; otherwise an error), fused = 0 the
  * prolongation and the sweep as two kernels through tmp (nx*ny*nz floats).
- * z_out must not alias z_in.  Asynchronous on `stream`. */
+ * z_out must not alias z_in; an empty grid writes *groups = 0 and nothing
+ * else.  Asynchronous on `stream`. */
 int bs_mg_prolong_post(const unsigned char* code, const float* z_in, const float* ec, const float* r,
                        const float* z_old, float* z_out, float* tmp, double* part, size_t part_len, int* groups,
                        int nx, int ny, int nz, int fused, double w, bs_stream_t stream);
