@@ -11,6 +11,18 @@
 
 namespace bs {
 
+// Compute units of the current device (256 on the MI355X if the runtime
+// cannot say).  The i-marching passes size their grids to whole rounds of
+// workgroups per CU: a grid just past a round leaves the device idle but for
+// a few workgroups while that last round runs.
+inline int device_cus() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        return 256;
+    return cus;
+}
+
 // ---------------------------------------------------------------------------
 // Cell classes.  The pressure path only distinguishes fluid (interior
 // unknown), air (Dirichlet p = 0) and anything else (solid, Neumann: the

@@ -406,14 +406,30 @@ __device__ __forceinline__ void write_partials(double (&v)[NV], double* __restri
     }
 }
 
-// Sums G partials per value (written by an earlier kernel) into out[].
+// Sums G partials per value (written by an earlier kernel) into out[].  Each
+// thread adds partials threadIdx.x, threadIdx.x + 256, ... in that order, with
+// eight loads in flight at a time (a running sum over single loads waits for
+// each in turn); past the end it adds +0.0, which leaves a sum that starts at
+// +0.0 unchanged, so the totals are those of the plain strided loop.
 template <int NV>
 __device__ __forceinline__ void sum_partials(const double* __restrict__ partials, unsigned G,
                                              double (&out)[NV]) {
+    constexpr unsigned kBatch = 8;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
+        const double* pk = partials + (size_t)k * G;
         double acc = 0.0;
-        for (unsigned w = threadIdx.x; w < G; w += kThreads) acc += partials[(size_t)k * G + w];
+        for (unsigned w = threadIdx.x; w < G; w += kBatch * kThreads) {
+            double v[kBatch];
+#pragma unroll
+            for (unsigned u = 0; u < kBatch; ++u) {
+                const unsigned q = w + u * kThreads;
+                const double x = pk[q < G ? q : G - 1];
+                v[u] = q < G ? x : 0.0;
+            }
+#pragma unroll
+            for (unsigned u = 0; u < kBatch; ++u) acc += v[u];
+        }
         out[k] = acc;
     }
     block_sums<NV>(out);

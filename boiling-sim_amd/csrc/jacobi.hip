@@ -25,6 +25,8 @@
 #include "boilsim_hip.h"
 #include "common.hpp"
 
+#include <stdlib.h>
+
 namespace bs {
 
 template <int VEC> struct VecF;
@@ -347,13 +349,17 @@ struct PairShape {
     unsigned grid;
 };
 
-// About 4096 workgroups (16 per CU): tiles over (j,k), i split into segments.
+// At most 32 workgroups per CU (four rounds of the 8 a CU holds): tiles over
+// (j,k), i split into segments.  440x440x280 in one run (workgroups, i planes
+// each: us per launch): 7840, 9: 184.0; 10080, 7: 185.2; 11840, 6: 189.5;
+// 17600, 4: 200.6; 4160, 17 (16 per CU rounded up, just past two rounds):
+// 197.0.
 static PairShape pair_shape(int nx, int ny, int nz) {
     PairShape s;
     s.jtiles = (ny + kPairTJ - 1) / kPairTJ;
     s.ktiles = (nz + 4 * kPairKT - 1) / (4 * kPairKT);
     const int cols = s.jtiles * s.ktiles;
-    int nseg = (4096 + cols / 2) / cols;
+    int nseg = 32 * device_cus() / cols;
     nseg = nseg < 1 ? 1 : (nseg > nx ? nx : nseg);
     s.ni = (nx + nseg - 1) / nseg;
     nseg = (nx + s.ni - 1) / s.ni;

@@ -271,15 +271,47 @@ __global__ __launch_bounds__(256) void step_partials_kernel(MLvl L, const uint8_
                                                             const float* __restrict__ e,
                                                             double* __restrict__ part, const int* __restrict__ done) {
     if (done && *done) return;
+    // Four cells per thread and round, every load of the round issued before
+    // any arithmetic: indices past the grid or the edges are clamped to the
+    // cell itself and their terms dropped, so the sums (over the thread's
+    // cells c, c + stride, ... in order; A e as apply_a forms it) are those
+    // of one cell at a time.
+    constexpr int B = 4;
     double re = 0.0, eae = 0.0;
     const int64_t stride = (int64_t)gridDim.x * 256;
-    for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < L.n; c += stride) {
-        if (code[c] < kCodeFluid0) continue;
-        int i, j, k;
-        ijk(L, (uint32_t)c, i, j, k);
-        const double ev = e[c];
-        re += (double)r[c] * ev;
-        eae += ev * (double)apply_a(L, code, e, c, i, j, k);
+    const int64_t plane = (int64_t)L.ny * L.nz;
+    for (int64_t c0 = (int64_t)blockIdx.x * 256 + threadIdx.x; c0 < L.n; c0 += B * stride) {
+        float rv[B], ev[B], av[B];
+        bool use[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int64_t cb = c0 + b * stride;
+            const bool in = cb < L.n;
+            const int64_t c = in ? cb : c0;
+            int i, j, k;
+            ijk(L, (uint32_t)c, i, j, k);
+            const bool ok[6] = {i > 0, i < L.nx - 1, j > 0, j < L.ny - 1, k > 0, k < L.nz - 1};
+            const int64_t nb[6] = {c - plane, c + plane, c - L.nz, c + L.nz, c - 1, c + 1};
+            const uint8_t cc = code[c];
+            ev[b] = e[c];
+            rv[b] = r[c];
+            float s = 0.0f;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const int64_t a = ok[q] ? nb[q] : c;
+                const float t = fz(code[a], e[a]);
+                s += ok[q] ? t : 0.0f;  // s is never -0, so adding +0 leaves it as skipping would
+            }
+            av[b] = lap_mg(cc, ev[b], s);
+            use[b] = in && cc >= kCodeFluid0;
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            if (!use[b]) continue;
+            const double evd = ev[b];
+            re += (double)rv[b] * evd;
+            eae += evd * (double)av[b];
+        }
     }
     double v[2] = {re, eae};
     write_partials<2>(v, part);
@@ -739,14 +771,15 @@ struct PrShape {
     unsigned grid;
 };
 
-// About `target` workgroups, i segments of even length >= 8 (two planes of
-// each segment's smoother are recomputed by its neighbours).
+// At most `target` workgroups (when the columns allow), i segments of even
+// length >= 8 (two planes of each segment's smoother are recomputed by its
+// neighbours).
 static PrShape pr_shape(int nx, int ny, int nz, int target) {
     PrShape s;
     s.jtiles = (ny + kPrTJ - 1) / kPrTJ;
     s.ktiles = (nz + 4 * kPrKT - 1) / (4 * kPrKT);
     const int cols = s.jtiles * s.ktiles;
-    int nseg = (target + cols / 2) / cols;
+    int nseg = target / cols;
     nseg = nseg < 1 ? 1 : nseg;
     int ni = (nx + nseg - 1) / nseg;
     ni = ni < 8 ? 8 : ni;
@@ -1310,7 +1343,7 @@ void scale_to_optimal(hipStream_t st, int nx, int ny, int nz, const uint8_t* cod
 __global__ void zero_u32(unsigned int* p) { if (threadIdx.x == 0) *p = 0u; }
 
 int mg_setup(hipStream_t st, void* ws, int nx, int ny, int nz, const int* mat, int mf, int ma,
-             const uint8_t* fcode_in, double* n_fluid_coarse, bool deep) {
+             const uint8_t* fcode_in, bool deep) {
     MgWs W = carve_mg(ws, nx, ny, nz);
     if (deep && W.L4.n > kBotMax) return (int)hipErrorInvalidValue;
     // The fine codes (given, or formed in a temporary: zt0 is free during
@@ -1335,12 +1368,7 @@ int mg_setup(hipStream_t st, void* ws, int nx, int ny, int nz, const int* mat, i
         coarse_class_kernel<<<grid_for(W.L4.n), 256, 0, st>>>(W.L3, W.L4, W.code3, W.mat4);
         codes_from_mat_kernel<<<grid_for(W.L4.n), 256, 0, st>>>(W.L4, W.mat4, 0, 2, W.code4, nullptr);
     }
-    BS_RETURN_IF_ERR(hipGetLastError());
-    unsigned int h = 0;
-    BS_RETURN_IF_ERR(hipMemcpyAsync(&h, W.nfl, sizeof(h), hipMemcpyDeviceToHost, st));
-    BS_RETURN_IF_ERR(hipStreamSynchronize(st));
-    if (n_fluid_coarse) *n_fluid_coarse = (double)h;
-    return hipSuccess;
+    return (int)hipGetLastError();
 }
 
 // Partial count of the last pass: one per workgroup of the single-pass
@@ -1387,6 +1415,12 @@ static void restrict_level(hipStream_t st, const Tiles& TF, int vf, const MLvl& 
         restrict_kernel<<<grid_for(LC.n), 256, 0, st>>>(LF, LC, fcode, r, z, rc, done);
     }
 }
+
+// Workgroups of the fine level's single pre-smoothing pass: at most 16 per
+// CU, two rounds of the 8 each CU holds (440x440x280: 4070 workgroups of
+// 20-plane segments, 258 us, against 6845 of 12 planes, 3.34 rounds, 268 us;
+// 2035 of 40 planes 260 us, 8140 of 10 planes 271 us).
+static int pr_target() { return 16 * device_cus(); }
 
 // Two smoothing sweeps from zero on level LF and the restricted residual on
 // LC: one pass when the shape allows it (fused != 0), else pre2_kernel and
@@ -1443,12 +1477,12 @@ static void chebyshev(hipStream_t st, const Tiles& T, int vec, const uint8_t* co
 
 unsigned mg_fold_groups(int nx, int ny, int nz) {
     const MLvl L = mk(nx, ny, nz);
-    return pr_fits(L, 4) ? pr_shape(nx, ny, nz, 7680).grid : 0u;
+    return pr_fits(L, 4) ? pr_shape(nx, ny, nz, pr_target()).grid : 0u;
 }
 
 int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* code0, const float* r, float* z,
              const float* z_old, double* dots_part, int coarse_kind, const float* params, const void* packed,
-             double n_fluid_coarse, const int* done, const MgFold* fold) {
+             const int* done, const MgFold* fold) {
     MgWs W = carve_mg(ws, nx, ny, nz);
     const int v0 = fold ? vec_for(W.L0.nz, {r, z, z_old, code0, W.zt0, fold->ap, fold->r_out})
                         : vec_for(W.L0.nz, {r, z, z_old, code0, W.zt0});
@@ -1465,7 +1499,7 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
     const bool fused_post = v0 == 4;
     float* zpre = fused_post ? W.zt0 : z;
     if (fold) pass_mark(st, 1, 0);
-    int rc = pre_restrict(st, T0, v0, W.L0, W.L1, C1, code0, r, zpre, W.r1, done, 1, 7680, K0, fold);
+    int rc = pre_restrict(st, T0, v0, W.L0, W.L1, C1, code0, r, zpre, W.r1, done, 1, pr_target(), K0, fold);
     if (rc != hipSuccess) return rc;
     if (fold) pass_mark(st, 1, 1);
     if (fold) {
@@ -1518,7 +1552,7 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
         if (rc != hipSuccess) return rc;
         polish();
     } else if (coarse_kind == 1) {
-        rc = unet_apply(st, W.unet, W.L2.nx, W.L2.ny, W.L2.nz, params, packed, W.r2, W.code2, n_fluid_coarse,
+        rc = unet_apply(st, W.unet, W.L2.nx, W.L2.ny, W.L2.nz, params, packed, W.r2, W.code2, W.nfl,
                         nullptr, 0, W.e2, nullptr, nullptr, done);
         if (rc != hipSuccess) return rc;
         // the A-norm-optimal step's products; the polish's first two steps
@@ -1530,7 +1564,7 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
         jacobi_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, W.zt2, done);
         jacobi_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.zt2, W.e2, done);
         residual_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, W.rs2, done);
-        rc = unet_apply(st, W.unet, W.L2.nx, W.L2.ny, W.L2.nz, params, packed, W.rs2, W.code2, n_fluid_coarse,
+        rc = unet_apply(st, W.unet, W.L2.nx, W.L2.ny, W.L2.nz, params, packed, W.rs2, W.code2, W.nfl,
                         nullptr, 0, W.en2, nullptr, nullptr, done);
         if (rc != hipSuccess) return rc;
         optimal_step(st, W.L2, W.code2, W.rs2, W.en2, W.part, done);

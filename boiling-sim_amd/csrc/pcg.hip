@@ -35,6 +35,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <atomic>
 #include <mutex>
 
 namespace bs {
@@ -57,14 +58,41 @@ struct PcgCtrl {
     int mode;        // 0: reference cadence (iter >= 8 && iter % 5 == 0), 1: every iteration
     int best_it;     // flexible CG: iteration of the best residual so far
     int fallback;    // flexible CG: stalled or lost positivity
-    int pad;
+    int n_fluid;     // flexible CG with the fine-grid network: fluid cells (it normalises r)
     double best_rr;
     double dots[2];  // flexible CG: r.z_new, r.z_old (reduced)
     double apz;      // flexible CG, fused first pass: Ap.z of the iteration (z its preconditioned residual)
     int fresh;       // flexible CG: the next direction is p = z (first iteration of a solve)
-    int pad2;
+    int polls;       // flexible CG: iterations posted to the host ring since the first direction
 };
 static_assert(sizeof(PcgCtrl) <= 256, "control block must fit 256 bytes");
+
+// What the host polls after each flexible-CG iteration: the last kernel of
+// the iteration stores it into a ring in mapped pinned host memory (slot
+// (seq - 1) mod kPollRing), in place of a copy of the control block per
+// launch unit.
+struct PollRec {
+    double rr;
+    double stop;
+    int iter;
+    int done;
+    int seq;  // 1 + the iteration's index since the first direction
+    int pad;
+};
+constexpr int kPollRing = 8;  // > the iterations in flight (two units of at most two)
+
+__device__ __forceinline__ void post_poll(PcgCtrl* ctrl, PollRec* ring) {
+    const int q = ctrl->polls;
+    ctrl->polls = q + 1;
+    PollRec rec;
+    rec.rr = ctrl->rr;
+    rec.stop = ctrl->stop;
+    rec.iter = ctrl->iter;
+    rec.done = ctrl->done;
+    rec.seq = q + 1;
+    rec.pad = 0;
+    ring[q & (kPollRing - 1)] = rec;
+}
 
 struct PcgBuffers {
     uint8_t* code;
@@ -280,15 +308,20 @@ constexpr int kRollKT = 14, kRollTJ = 14;
 struct RollShape {
     int jtiles, ktiles, ni;
     unsigned grid;
+    int tj = 0;  // j rows per tile (fcg_spmv_dir_kernel)
 };
 
-// About 4096 workgroups: tiles over (j,k), i split into segments.
+// At most 16 workgroups per CU (two rounds of the 8 a CU holds): tiles over
+// (j,k), i split into segments.  440x440x280 in one run (workgroups, i planes
+// each: us per launch): 4000, 18: 120.8; 5920, 12: 123.0; 1920, 37: 124.2;
+// 4160, 17 (about 4096 rounded up, just past two rounds): 128.2; 2080, 34
+// (just past one round): 145.3.
 static RollShape roll_shape(int nx, int ny, int nz) {
     RollShape s;
     s.jtiles = (ny + kRollTJ - 1) / kRollTJ;
     s.ktiles = (nz + 4 * kRollKT - 1) / (4 * kRollKT);
     const int cols = s.jtiles * s.ktiles;
-    int nseg = (4096 + cols / 2) / cols;
+    int nseg = 16 * device_cus() / cols;
     nseg = nseg < 1 ? 1 : (nseg > nx ? nx : nseg);
     s.ni = (nx + nseg - 1) / nseg;
     nseg = (nx + s.ni - 1) / s.ni;
@@ -297,19 +330,25 @@ static RollShape roll_shape(int nx, int ny, int nz) {
 }
 
 // The flexible CG's first pass (fcg_spmv_dir_kernel) takes the same stencil
-// on a taller tile: 62 x 56 (j,k) cells and their one-chunk ring on 1024
+// on a taller tile: up to 62 x 56 (j,k) cells and their one-chunk ring on 1024
 // threads.  The pass is memory-bound and its ring is re-read from L2 or HBM
 // by the neighbouring tiles; the taller tile cuts that overhead from 1.18x
-// to 1.13x of the tile's cells (440x440x280: 339 -> 319 us per pass).  Plain
-// PCG keeps the 14-row tile of pcg_spmv_roll_kernel.
+// to 1.13x of the tile's cells (440x440x280: 339 -> 319 us per pass).  The
+// rows are spread evenly over the j tiles (440: 8 x 55 rather than 7 x 62 +
+// 6), and the grid is at most one workgroup per CU, each walking a long i
+// segment.  Measured in one run at 440x440x280 (tile rows, workgroups, i
+// planes each: us): 55, 240, 74: 289.5; 55, 480, 37: 297; 55, 1000, 18: 301.5;
+// 62, 240, 74: 303; 62, 1040, 17 (the earlier rule, just past two rounds of
+// the two workgroups a CU holds): 320; 55, 280, 63 (just past one round): 424.
+// Plain PCG keeps the 14-row tile of pcg_spmv_roll_kernel.
 constexpr int kDirTJ = 62, kDirThreads = (kDirTJ + 2) * (kRollKT + 2);
 static RollShape dir_shape(int nx, int ny, int nz) {
     RollShape s;
     s.jtiles = (ny + kDirTJ - 1) / kDirTJ;
+    s.tj = (ny + s.jtiles - 1) / s.jtiles;
     s.ktiles = (nz + 4 * kRollKT - 1) / (4 * kRollKT);
     const int cols = s.jtiles * s.ktiles;
-    const int target = 4096 * 256 / kDirThreads;
-    int nseg = (target + cols / 2) / cols;
+    int nseg = device_cus() / cols;
     nseg = nseg < 1 ? 1 : (nseg > nx ? nx : nseg);
     s.ni = (nx + nseg - 1) / nseg;
     nseg = (nx + s.ni - 1) / s.ni;
@@ -458,12 +497,7 @@ __global__ __launch_bounds__(1024) void pap_reduce_kernel(const double* __restri
 // quantities, r_{k+1}.z_k = r_k.z_k - alpha_k (A p_k).z_k (the fused first
 // pass accumulates (A p_k).z_k while it holds both), which spares the last
 // preconditioner pass the z_old stream.
-__global__ __launch_bounds__(1024) void dots_reduce_kernel(const double* __restrict__ partZ, unsigned G,
-                                                           PcgCtrl* ctrl, int init, int apz_mode = 0) {
-    if (ctrl->done) return;
-    double v[2];
-    reduce1024<2>(partZ, G, v);
-    if (threadIdx.x != 0) return;
+__device__ void dots_step(PcgCtrl* ctrl, const double (&v)[2], int init, int apz_mode) {
     ctrl->dots[0] = v[0];
     ctrl->dots[1] = v[1];
     if (!(v[0] > 0.0)) {
@@ -484,6 +518,22 @@ __global__ __launch_bounds__(1024) void dots_reduce_kernel(const double* __restr
     ctrl->beta = beta;
     ctrl->fresh = init;
     if (!init && it1 >= ctrl->max_iter) ctrl->done = 1;
+}
+
+// The iteration's last kernel.  init (the first direction) restarts the
+// count of posted iterations; with a ring, every iteration posts its state
+// to it, a finished solve's no-op iterations included.
+__global__ __launch_bounds__(1024) void dots_reduce_kernel(const double* __restrict__ partZ, unsigned G,
+                                                           PcgCtrl* ctrl, int init, int apz_mode = 0,
+                                                           PollRec* ring = nullptr) {
+    if (!ctrl->done) {
+        double v[2];
+        reduce1024<2>(partZ, G, v);
+        if (threadIdx.x == 0) dots_step(ctrl, v, init, apz_mode);
+    }
+    if (threadIdx.x != 0) return;
+    if (init) ctrl->polls = 0;
+    if (ring) post_poll(ctrl, ring);
 }
 
 // alpha = r.z / p.Ap (0 if p.Ap == 0, axpy_device.cu:82-91); x += alpha p,
@@ -619,7 +669,8 @@ constexpr unsigned kMaxWg = 1024;
 struct HostPoll {
     int* slots = nullptr;      // [2][2]: (done, iter) of the last two batches
     PcgCtrl* final = nullptr;  // full control block after a solve
-    PcgCtrl* ctl = nullptr;    // [2]: the control block after each of the last two batches
+    PollRec* ring = nullptr;   // [kPollRing]: the flexible CG's per-iteration posts (host view)
+    PollRec* ring_dev = nullptr;  // the same memory as kernels address it
 };
 static HostPoll g_poll;
 
@@ -633,10 +684,17 @@ static std::recursive_mutex g_solve_mu;
 static int ensure_poll() {
     if (g_poll.slots) return hipSuccess;
     void* h = nullptr;
-    BS_RETURN_IF_ERR(hipHostMalloc(&h, 1024, hipHostMallocDefault));
+    BS_RETURN_IF_ERR(hipHostMalloc(&h, 2048, hipHostMallocDefault));
+    void* d = nullptr;
+    if (const hipError_t e = hipHostGetDevicePointer(&d, (char*)h + 1024, 0); e != hipSuccess) {
+        (void)hipHostFree(h);
+        return (int)e;
+    }
+    static_assert(sizeof(PollRec) * kPollRing <= 1024, "poll ring must fit its block");
     g_poll.slots = (int*)h;
     g_poll.final = (PcgCtrl*)((char*)h + 256);
-    g_poll.ctl = (PcgCtrl*)((char*)h + 512);
+    g_poll.ring = (PollRec*)((char*)h + 1024);
+    g_poll.ring_dev = (PollRec*)d;
     return hipSuccess;
 }
 
@@ -1156,7 +1214,7 @@ __global__ __launch_bounds__(256) void fcg_direction_kernel(Tiles T, const float
 // p_new goes to another buffer than p_old: the ring cells of a tile belong to
 // its neighbours, whose old values are still read there.
 __global__ __launch_bounds__(kDirThreads) void fcg_spmv_dir_kernel(int nx, int ny, int nz, int ni_seg, int jtiles,
-                                                           int ktiles, const uint8_t* __restrict__ code,
+                                                           int ktiles, int tj, const uint8_t* __restrict__ code,
                                                            const float* __restrict__ z,
                                                            const float* __restrict__ p_old,
                                                            float* __restrict__ p_new, float* __restrict__ x,
@@ -1175,11 +1233,12 @@ __global__ __launch_bounds__(kDirThreads) void fcg_spmv_dir_kernel(int nx, int n
     const int cols = jtiles * ktiles;
     const int seg = (int)(b / cols), col = (int)(b % cols);
     const int jt = col / ktiles, kt = col - (col / ktiles) * ktiles;
-    const int j0 = jt * kDirTJ, k0 = kt * 4 * kRollKT;
+    const int j0 = jt * tj, k0 = kt * 4 * kRollKT;
     const int i0 = seg * ni_seg, i1 = min(nx, i0 + ni_seg);
     const int j = j0 - 1 + r, k = k0 - 4 + 4 * c;
-    const bool jk_in = j >= 0 && j < ny && k >= 0 && k < nz;
-    const bool interior = r >= 1 && r <= kDirTJ && c >= 1 && c <= kRollKT && j < ny && k < nz;
+    // rows past the tile's tj rows and its ring stay idle
+    const bool jk_in = r <= tj + 1 && j >= 0 && j < ny && k >= 0 && k < nz;
+    const bool interior = r >= 1 && r <= tj && c >= 1 && c <= kRollKT && j < ny && k < nz;
     const int64_t pl = (int64_t)ny * nz;
     const int64_t off = (int64_t)j * nz + k;
     // planes outside the grid and chunks outside (j,k) read as air (value 0)
@@ -1372,7 +1431,7 @@ struct FcgBuffers {
     int kind;
     const float* params;
     const void* packed;
-    double n_fluid;  // of the level the network runs on
+    const unsigned* n_fluid = nullptr;  // kind 1: the fine grid's fluid count (device); the V-cycle keeps its own
     unsigned partZ_G;  // workgroups that wrote partZ
     int nx, ny, nz;
 };
@@ -1446,7 +1505,7 @@ static int fcg_precond(hipStream_t st, const FcgBuffers& F, const float* r, floa
     }
     // The V-cycle's last sweep also writes the r.z and r.z_old partials.
     return mg_apply(st, F.pws, F.nx, F.ny, F.nz, B.code, r, z, z_old, F.partZ, coarse, F.params, F.packed,
-                    F.n_fluid, done, r_out ? &fold : nullptr);
+                    done, r_out ? &fold : nullptr);
 }
 
 // One flexible-CG iteration of parity `par` (z_new = z[1-par], z_old = z[par]).
@@ -1460,7 +1519,7 @@ static int fcg_enqueue_iteration(const Tiles& T, int vec, const FcgBuffers& F, i
         const RollShape rs = dir_shape(T.nx, T.ny, T.nz);
         float* pb[2] = {B.p, F.p2};
         pass_mark(st, 0, 0);
-        fcg_spmv_dir_kernel<<<rs.grid, kDirThreads, 0, st>>>(T.nx, T.ny, T.nz, rs.ni, rs.jtiles, rs.ktiles, B.code, F.z[par],
+        fcg_spmv_dir_kernel<<<rs.grid, kDirThreads, 0, st>>>(T.nx, T.ny, T.nz, rs.ni, rs.jtiles, rs.ktiles, rs.tj, B.code, F.z[par],
                                                      pb[par], pb[par ^ 1], B.x, B.ap, B.partT, B.ctrl);
         pass_mark(st, 0, 1);
         fcg_pap_reduce_kernel<<<1, 1024, 0, st>>>(B.partT, rs.grid, B.ctrl);
@@ -1469,7 +1528,7 @@ static int fcg_enqueue_iteration(const Tiles& T, int vec, const FcgBuffers& F, i
             float* rb[2] = {B.r, F.r2};
             const int rc = fcg_precond(st, F, rb[par], F.z[par ^ 1], nullptr, rb[par ^ 1]);
             if (rc != hipSuccess) return rc;
-            dots_reduce_kernel<<<1, 1024, 0, st>>>(F.partZ, F.partZ_G, B.ctrl, 0, 1);
+            dots_reduce_kernel<<<1, 1024, 0, st>>>(F.partZ, F.partZ_G, B.ctrl, 0, 1, g_poll.ring_dev);
             return (int)hipGetLastError();
         }
         BS_VEC_LAUNCH(fcg_rupdate_kernel, vec, B.nwg, st, T, B.ap, B.r, B.partB, B.ctrl);
@@ -1481,7 +1540,7 @@ static int fcg_enqueue_iteration(const Tiles& T, int vec, const FcgBuffers& F, i
     // fused: r.z_old from the first pass's Ap.z instead of a z_old stream
     const int rc = fcg_precond(st, F, B.r, F.z[par ^ 1], F.fused ? nullptr : F.z[par]);
     if (rc != hipSuccess) return rc;
-    dots_reduce_kernel<<<1, 1024, 0, st>>>(F.partZ, F.partZ_G, B.ctrl, 0, F.fused ? 1 : 0);
+    dots_reduce_kernel<<<1, 1024, 0, st>>>(F.partZ, F.partZ_G, B.ctrl, 0, F.fused ? 1 : 0, g_poll.ring_dev);
     if (!F.fused) BS_VEC_LAUNCH(fcg_direction_kernel, vec, B.nwg, st, T, F.z[par ^ 1], B.p, B.ctrl, 0);
     return (int)hipGetLastError();
 }
@@ -1522,7 +1581,7 @@ struct UnitGraphKey {
     const float* x;
     const float* params;
     const void* packed;
-    double n_fluid;
+    const unsigned* n_fluid;
     int nx, ny, nz, kind, vec, fused, fold;
     int unit;  // 0: the (even, odd) pair; 1 + par: the single iteration of parity par; 3: the start
     unsigned nwg;
@@ -1598,8 +1657,12 @@ static int fcg_begin(const Tiles& T, int vec, const FcgBuffers& F, hipStream_t s
 }
 
 // Runs iterations from the device counter (even on entry) in graph-launched
-// units until done, polling the flag one unit behind.
-static int fcg_iterate(const Tiles& T, int vec, const FcgBuffers& F, int max_iter, hipStream_t st) {
+// units until done, polling the flag one unit behind, then applies the
+// pending step to x.  With `copy_final` the final control block is copied to
+// g_poll.final and the stream synchronised; without, the caller does that
+// after queueing more work.
+static int fcg_iterate(const Tiles& T, int vec, const FcgBuffers& F, int max_iter, hipStream_t st,
+                       bool copy_final = true) {
     const PcgBuffers& B = F.B;
     BS_RETURN_IF_ERR((hipError_t)ensure_poll());
     const bool eager = fcg_eager();
@@ -1614,8 +1677,11 @@ static int fcg_iterate(const Tiles& T, int vec, const FcgBuffers& F, int max_ite
         return (int)e;
     }
     // Units (an iteration pair, or a single iteration) are queued two deep,
-    // each followed by a copy of the control block into a pinned slot, and
-    // polled oldest first.  A unit is milliseconds of GPU work, so the host
+    // each followed by an event, and polled oldest first: once a unit's
+    // event has completed, its last iteration's post in the pinned ring
+    // (written by that iteration's last kernel) is visible.  Posts count
+    // from the first direction, so the unit that ends after `it` iterations
+    // of this call posted sequence number `it`.  A unit is milliseconds of GPU work, so the host
     // stays ahead.  From the residual's reduction per iteration between the
     // last two polls: once the units in flight are predicted to reach the
     // stopping test, no further unit is queued until that is confirmed;
@@ -1626,6 +1692,7 @@ static int fcg_iterate(const Tiles& T, int vec, const FcgBuffers& F, int max_ite
     // unit.
     int it = 0, head = 0, inflight = 0;
     int unit_its[2] = {0, 0};
+    int unit_end[2] = {0, 0};
     double rr_prev = -1.0, rr_last = -1.0, stop = -1.0;
     int it_prev = 0, it_last = 0;
     bool finished = false;
@@ -1645,7 +1712,7 @@ static int fcg_iterate(const Tiles& T, int vec, const FcgBuffers& F, int max_ite
             } else if (rated) {
                 // singles once two more iterations are predicted to suffice:
                 // a pair could then end with a no-op half (an iteration's
-                // launches), a superfluous single costs only its poll copy
+                // launches), a superfluous single costs only its poll
                 double pred = rr_last;
                 for (int k = 0; k < its_inflight + 2; ++k) pred *= q1;
                 if (pred < stop) n = 1;
@@ -1657,11 +1724,10 @@ static int fcg_iterate(const Tiles& T, int vec, const FcgBuffers& F, int max_ite
             } else {
                 rc = hipGraphLaunch(n == 2 ? exec[0] : exec[1 + (it & 1)], st);
             }
-            if (rc == hipSuccess)
-                rc = hipMemcpyAsync(g_poll.ctl + slot, B.ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, st);
             if (rc == hipSuccess) rc = hipEventRecord(ev[slot], st);
             if (rc == hipSuccess) {
                 unit_its[slot] = n;
+                unit_end[slot] = it + n;
                 ++inflight;
                 its_inflight += n;
                 it += n;
@@ -1670,7 +1736,18 @@ static int fcg_iterate(const Tiles& T, int vec, const FcgBuffers& F, int max_ite
         if (rc != hipSuccess || inflight == 0) break;
         rc = hipEventSynchronize(ev[head]);
         if (rc != hipSuccess) break;
-        const PcgCtrl& h = g_poll.ctl[head];
+        std::atomic_thread_fence(std::memory_order_acquire);
+        const volatile PollRec& post = g_poll.ring[(unit_end[head] - 1) & (kPollRing - 1)];
+        PollRec h;
+        h.rr = post.rr;
+        h.stop = post.stop;
+        h.iter = post.iter;
+        h.done = post.done;
+        h.seq = post.seq;
+        if (h.seq != unit_end[head]) {  // every iteration posts once: a missing post is a bug
+            rc = (int)hipErrorUnknown;
+            break;
+        }
         head ^= 1;
         --inflight;
         if (h.done) finished = true;
@@ -1684,7 +1761,7 @@ static int fcg_iterate(const Tiles& T, int vec, const FcgBuffers& F, int max_ite
         BS_VEC_LAUNCH(fcg_finish_kernel, vec, B.nwg, st, T, B.p, F.p2, B.x, B.ctrl);
         rc = (int)hipGetLastError();
     }
-    if (rc == hipSuccess) {
+    if (rc == hipSuccess && copy_final) {
         rc = hipMemcpyAsync(g_poll.final, B.ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, st);
         if (rc == hipSuccess) rc = hipStreamSynchronize(st);
     }
@@ -1770,25 +1847,24 @@ extern "C" int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx
     BS_RETURN_IF_ERR(hipGetLastError());
     if (precond == 1) {
         // The network normalises r by its RMS over the fluid cells of this grid.
-        count_fluid_kernel<<<256, 256, 0, st>>>((int64_t)n, B.code, &B.ctrl->pad);
-        BS_RETURN_IF_ERR(hipMemcpyAsync(g_poll.final, B.ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, st));
-        BS_RETURN_IF_ERR(hipStreamSynchronize(st));
-        F.n_fluid = (double)(unsigned)g_poll.final->pad;
+        count_fluid_kernel<<<256, 256, 0, st>>>((int64_t)n, B.code, &B.ctrl->n_fluid);
+        F.n_fluid = (const unsigned*)&B.ctrl->n_fluid;
     }
     if (precond >= 2) {
-        const int rc = mg_setup(st, F.pws, nx, ny, nz, mat, mat_fluid, mat_air, B.code, &F.n_fluid, precond == 6);
+        const int rc = mg_setup(st, F.pws, nx, ny, nz, mat, mat_fluid, mat_air, B.code, precond == 6);
         if (rc != hipSuccess) return rc;
     }
     fcg_prepare_kernel<<<1, 64, 0, st>>>(B.ctrl, 0);
     int rc = fcg_begin(T, vec, F, st);
     if (rc != hipSuccess) return rc;
-    rc = fcg_iterate(T, vec, F, max_iter, st);
+    // The true residual is queued behind the last iteration before the host
+    // waits: the flexible CG's counters arrive with it.
+    rc = fcg_iterate(T, vec, F, max_iter, st, false);
     if (rc != hipSuccess) return rc;
-    PcgCtrl h = *g_poll.final;
-    const int fcg_iters = h.iter;
-    const int fell_back = h.fallback;
+    PcgCtrl h;
+    int fcg_iters = -1, fell_back = 0;
     // True residual; plain PCG (residual replacement) while it is above tol.
-    int restarts = 0, it0 = h.iter;
+    int restarts = 0, it0 = 0;
     for (;;) {
         BS_VEC_LAUNCH(pcg_residual_kernel, vec, B.nwg, st, T, B.code, B.b, B.x, nullptr, B.p, B.partS, 0);
         pcg_residual_finalize<<<1, 256, 0, st>>>(B.partS, B.nwg, B.ctrl, 0, 0);
@@ -1796,6 +1872,11 @@ extern "C" int bs_fcg_solve(float* x, const float* div_u, const int* mat, int nx
         BS_RETURN_IF_ERR(hipMemcpyAsync(g_poll.final, B.ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, st));
         BS_RETURN_IF_ERR(hipStreamSynchronize(st));
         h = *g_poll.final;
+        if (fcg_iters < 0) {
+            fcg_iters = h.iter;
+            fell_back = h.fallback;
+            it0 = h.iter;
+        }
         const double rel_true = h.bsq > 0.0 ? sqrt(h.rr_true / h.bsq) : 0.0;
         if (stats) {
             stats->relres_true = rel_true;
@@ -2277,12 +2358,9 @@ extern "C" int bs_fcg_refine_solve(double* x, float* x32, const float* div_u, co
     F.fold = fcg_fold(F);
     F.fold_G = F.fold ? mg_fold_groups(nx, ny, nz) : 0u;
     if (precond == 1) {
-        RefCtrl h;
-        BS_RETURN_IF_ERR(hipMemcpyAsync(&h, R.rc, sizeof(h), hipMemcpyDeviceToHost, st));
-        BS_RETURN_IF_ERR(hipStreamSynchronize(st));
-        F.n_fluid = (double)h.n_fluid;
+        F.n_fluid = &R.rc->n_fluid;
     } else {
-        rc = mg_setup(st, F.pws, nx, ny, nz, mat, mat_fluid, mat_air, B.code, &F.n_fluid, precond == 6);
+        rc = mg_setup(st, F.pws, nx, ny, nz, mat, mat_fluid, mat_air, B.code, precond == 6);
         if (rc != hipSuccess) return rc;
     }
     // After an inner flexible CG stalls (fallback), the remaining steps run

@@ -13,11 +13,12 @@ constexpr unsigned kUNetPartWg = 1024;  // workgroups of the U-Net's reduction p
 // Workspace of one U-Net application on an nx*ny*nz grid (inference only).
 size_t unet_ws_bytes(int nx, int ny, int nz);
 
-// z = N(r) on fluid cells.  rr_part (G partials of r.r) may be null; rz_part
+// z = N(r) on fluid cells; n_fluid points to the grid's fluid-cell count in
+// device memory.  rr_part (G partials of r.r) may be null; rz_part
 // (2 x kUNetPartWg partials of r.z and r.z_old) is written when not null.
 // Every kernel returns at once when *done != 0.
 int unet_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const float* params, const void* packed,
-               const float* r, const uint8_t* code, double n_fluid, const double* rr_part, unsigned G,
+               const float* r, const uint8_t* code, const unsigned* n_fluid, const double* rr_part, unsigned G,
                float* z, const float* z_old, double* rz_part, const int* done);
 
 // Multigrid-augmented preconditioner: a V-cycle on the fine grid and its 2x
@@ -26,12 +27,12 @@ int unet_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const float* pa
 // two more geometric levels and a one-workgroup PCG bottom solve (4).
 size_t mg_ws_bytes(int nx, int ny, int nz);
 // Builds the coarse classes and codes from the fine mask (or from its fine
-// class codes fcode when not null); returns the fluid count of the coarsest
-// level through *n_fluid_coarse (synchronises once).
+// class codes fcode when not null) and the coarsest level's fluid count, all
+// on the device (asynchronous on st).
 // deep: also the two levels below the 4x one (coarse kind 4; an error when
 // the lowest has more than 16384 cells).
 int mg_setup(hipStream_t st, void* ws, int nx, int ny, int nz, const int* mat, int mf, int ma, const uint8_t* fcode,
-             double* n_fluid_coarse, bool deep = false);
+             bool deep = false);
 // The flexible CG's residual update folded into the V-cycle's first fine
 // pass: the pass reads r_old (mg_apply's r) and Ap, works on and writes
 // r_out = r_old - (*alpha) Ap (r_out must not alias r_old) and one r.r
@@ -53,7 +54,7 @@ unsigned mg_fold_groups(int nx, int ny, int nz);
 // into dots_part, mg_dots_groups(nx,ny,nz) workgroups of 2 values.
 int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* code0, const float* r, float* z,
              const float* z_old, double* dots_part, int coarse_kind, const float* params, const void* packed,
-             double n_fluid_coarse, const int* done, const MgFold* fold = nullptr);
+             const int* done, const MgFold* fold = nullptr);
 unsigned mg_dots_groups(int nx, int ny, int nz);
 
 // e *= (r.e)/(e.A e) (0 when not positive): the A-norm-optimal multiple of a

@@ -155,6 +155,7 @@ struct ConvExtra {
     const double* rr_part = nullptr;
     unsigned G = 0;
     double n_fluid = 0.0;
+    const unsigned* n_fluid_dev = nullptr;  // when set, the fluid count is read here instead
     float* rms_out = nullptr;
     __bf16* pool_out = nullptr;
     int cnx = 0, cny = 0, cnz = 0;
@@ -188,7 +189,8 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(
     if constexpr (FEAT) {
         double v[1] = {0.0};
         if (X.rr_part) sum_partials<1>(X.rr_part, X.G, v);
-        const double rms = X.rr_part ? sqrt(v[0] / (X.n_fluid > 0 ? X.n_fluid : 1.0)) : 1.0;
+        const double nf = X.n_fluid_dev ? (double)*X.n_fluid_dev : X.n_fluid;
+        const double rms = X.rr_part ? sqrt(v[0] / (nf > 0 ? nf : 1.0)) : 1.0;
         inv = rms > 0.0 ? (float)(1.0 / rms) : 0.0f;
         if (blockIdx.x == 0 && threadIdx.x == 0 && X.rms_out) *X.rms_out = (float)rms;
     }
@@ -480,12 +482,14 @@ __global__ __launch_bounds__(256) void wgrad_reduce_kernel(const float* __restri
 __global__ __launch_bounds__(256) void features_kernel(Lvl L, const float* __restrict__ r,
                                                        const uint8_t* __restrict__ code,
                                                        const double* __restrict__ rr_partials, unsigned G,
-                                                       double n_fluid, float* __restrict__ rms_out,
+                                                       double n_fluid, const unsigned* __restrict__ n_fluid_dev,
+                                                       float* __restrict__ rms_out,
                                                        __bf16* __restrict__ x0, const int* __restrict__ done) {
     if (done && *done) return;
     double v[1] = {0.0};
     if (rr_partials) sum_partials<1>(rr_partials, G, v);
-    const double rms = rr_partials ? sqrt(v[0] / (n_fluid > 0 ? n_fluid : 1.0)) : 1.0;
+    const double nf = n_fluid_dev ? (double)*n_fluid_dev : n_fluid;
+    const double rms = rr_partials ? sqrt(v[0] / (nf > 0 ? nf : 1.0)) : 1.0;
     const float inv = rms > 0.0 ? (float)(1.0 / rms) : 0.0f;
     if (blockIdx.x == 0 && threadIdx.x == 0 && rms_out) *rms_out = (float)rms;
     const int64_t stride = (int64_t)gridDim.x * 256;
@@ -918,9 +922,12 @@ static void conv_wgrad(hipStream_t st, const Lvl& L, const __bf16* in, const __b
 
 // Forward pass; r -> z (FP32, fluid cells).  rr_partials/G: r.r partials
 // already on the device (the Krylov loop has them); when null they are formed
-// here.  training != 0 also keeps the pre-skip activations for backward.
+// here.  The fluid count that normalises r is n_fluid_dev's value when that is
+// not null (no host round trip), else n_fluid.  training != 0 also keeps the
+// pre-skip activations for backward.
 static int unet_forward(hipStream_t st, UNetWs& W, const float* params, const __bf16* packed, const float* r,
-                        const uint8_t* code, double n_fluid, const double* rr_partials, unsigned G, float* z,
+                        const uint8_t* code, double n_fluid, const unsigned* n_fluid_dev,
+                        const double* rr_partials, unsigned G, float* z,
                         const float* z_old, double* rz_partials, const int* done, int training) {
     const unsigned eg1 = elem_grid(W.L1.n);
     if (!rr_partials) {
@@ -929,7 +936,8 @@ static int unet_forward(hipStream_t st, UNetWs& W, const float* params, const __
         G = kPartWg;
     }
     if (training) {
-        features_kernel<<<eg1, 256, 0, st>>>(W.L1, r, code, rr_partials, G, n_fluid, W.rms, W.x0, done);
+        features_kernel<<<eg1, 256, 0, st>>>(W.L1, r, code, rr_partials, G, n_fluid, n_fluid_dev, W.rms, W.x0,
+                                             done);
         conv_fwd<8, 8>(st, W.L1, W.x0, packed, 0, params, 1, W.h1, nullptr, nullptr, done);
         pool_kernel<8><<<elem_grid(W.L2.n), 256, 0, st>>>(W.L1, W.L2, W.h1, W.p1, done);
         conv_fwd<8, 16>(st, W.L2, W.p1, packed, 1, params, 1, W.h2, nullptr, nullptr, done);
@@ -938,7 +946,8 @@ static int unet_forward(hipStream_t st, UNetWs& W, const float* params, const __
         // the same values without the feature tensor and the two pooling
         // passes (GPU test: bit-identical to the training forward)
         ConvExtra X1;
-        X1.r = r; X1.code = code; X1.rr_part = rr_partials; X1.G = G; X1.n_fluid = n_fluid; X1.rms_out = W.rms;
+        X1.r = r; X1.code = code; X1.rr_part = rr_partials; X1.G = G; X1.n_fluid = n_fluid;
+        X1.n_fluid_dev = n_fluid_dev; X1.rms_out = W.rms;
         X1.pool_out = W.p1; X1.cnx = W.L2.nx; X1.cny = W.L2.ny; X1.cnz = W.L2.nz;
         conv_fwd_x<8, 8, true, true>(st, W.L1, nullptr, packed, 0, params, 1, W.h1, done, X1);
         ConvExtra X2;
@@ -996,7 +1005,7 @@ extern "C" int bs_unet_forward(const float* params, const void* packed, const fl
     if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
     UNetWs W = carve_unet(workspace, nx, ny, nz, training);
     if (workspace_bytes < W.bytes) return (int)hipErrorInvalidValue;
-    return unet_forward(stream, W, params, (const __bf16*)packed, r, code, n_fluid, nullptr, 0, z, nullptr,
+    return unet_forward(stream, W, params, (const __bf16*)packed, r, code, n_fluid, nullptr, nullptr, 0, z, nullptr,
                         nullptr, nullptr, training);
 }
 
@@ -1088,10 +1097,10 @@ static_assert(kUNetPartWg == kPartWg, "partial counts must agree");
 size_t unet_ws_bytes(int nx, int ny, int nz) { return carve_unet(nullptr, nx, ny, nz, 0).bytes; }
 
 int unet_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const float* params, const void* packed,
-               const float* r, const uint8_t* code, double n_fluid, const double* rr_part, unsigned G,
+               const float* r, const uint8_t* code, const unsigned* n_fluid, const double* rr_part, unsigned G,
                float* z, const float* z_old, double* rz_part, const int* done) {
     UNetWs W = carve_unet(ws, nx, ny, nz, 0);
-    return unet_forward(st, W, params, (const __bf16*)packed, r, code, n_fluid, rr_part, G, z, z_old, rz_part,
+    return unet_forward(st, W, params, (const __bf16*)packed, r, code, 0.0, n_fluid, rr_part, G, z, z_old, rz_part,
                         done, 0);
 }
 
