@@ -1506,9 +1506,11 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
         r = fold->r_out;  // the rest of the cycle works on the updated residual
         if (fold->after) fold->after(st, fold->after_arg);
     }
-    // level 1 (the single pass walks its i segments plane by plane: with the
-    // few workgroups this level fills, that chain is slower than two kernels)
-    rc = pre_restrict(st, T1, v1, W.L1, W.L2, C2, W.code1, W.r1, W.z1, W.r2, done, 0, 0, K1);
+    // level 1: the same single pass on at most 4 workgroups per CU (220x220x140:
+    // 912 of 14 planes; the learned projection 18.92 ms against 19.09 with
+    // pre2_kernel + restrict4_kernel, 19.05 with 456 workgroups, 18.95 with
+    // 1596; the restriction sums in another order, the iterations are the same)
+    rc = pre_restrict(st, T1, v1, W.L1, W.L2, C2, W.code1, W.r1, W.z1, W.r2, done, 1, 4 * device_cus(), K1);
     if (rc != hipSuccess) return rc;
     // level 2: coarse solve.  coarse_kind 0: one damped-Jacobi sweep from
     // zero, then the polish; 1: the network (A-norm-optimal step), then the
