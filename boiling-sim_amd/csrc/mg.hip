@@ -744,10 +744,10 @@ __global__ __launch_bounds__(256) void prolong4_kernel(Tiles F, Tiles CV, const 
 
 // ---------------------------------------------------------------------------
 // Pre-smoothing and restriction of one level in a single pass (nx, ny even,
-// nz % 4 == 0, 16-byte aligned vectors).  A workgroup owns a 12 x 56 (j,k)
-// tile and walks down an even-length i segment; each of its 256 threads holds
-// one 4-cell k chunk of the tile or of its ring (two rows in j, one chunk in
-// k).  At plane q it forms the two-step smoother z = c2 (z1 + c3 D^-1 (r - A
+// nz % 4 == 0, 16-byte aligned vectors).  A workgroup owns a tj x 56 (j,k)
+// tile (tj even, at most SR - 4) and walks down an even-length i segment;
+// each of its SR x 16 threads holds one 4-cell k chunk of the tile or of its
+// ring (two rows in j, one chunk in k).  At plane q it forms the two-step smoother z = c2 (z1 + c3 D^-1 (r - A
 // z1)), z1 = c1 D^-1 r, over the tile and a one-cell ring (pre2_kernel's
 // arithmetic, so z is bit-identical), then the residual r - A z at plane q-1 over the
 // tile, which it sums over the 2x2x2 children of each coarse cell (times 0.5,
@@ -762,21 +762,45 @@ __global__ __launch_bounds__(256) void prolong4_kernel(Tiles F, Tiles CV, const 
 // registers (the pass is issue-bound, not HBM-bound, so VALU work is what
 // these save).  The march through a segment is a chain of per-plane
 // latencies, so the pass pays off only where the level fills the chip with
-// several rounds of workgroups (the fine level).
+// workgroups (the fine level and its 2x coarsening).  On the fine level the
+// tile is 56 x 56 on 1024 threads (of which the ring holds a quarter,
+// against a third of 256 threads on 12 x 56) and the grid one round of one
+// workgroup per CU on long segments: 440x440x280, 240 workgroups of 38
+// planes, 219 us per pass, against 257 us for 4070 workgroups of 256
+// threads on 20 planes (and 242 / 243 / 248 us for 2 / 3 / 4 rounds of the
+// taller tile, 234 us for 44 x 56 tiles on 768 threads).  The 2x level
+// keeps the 256-thread tile (the taller one there: the same time).
 // ---------------------------------------------------------------------------
-constexpr int kPrTJ = 12, kPrKT = 14;
+// Workgroups of `fn` (nt threads each) that one CU holds at once, from the
+// runtime's occupancy calculator (the built kernel's register and LDS use).
+template <typename F>
+static int resident_wgs(F fn, int nt) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, nt, 0) != hipSuccess || n < 1) n = 1;
+    return n;
+}
+
+constexpr int kPrKT = 14;
+// Thread rows (tile rows + the 4 ring rows) of the two workgroup shapes: 256
+// threads (the 2x level) and kPrBigSR * 16 threads (the fine level, where the
+// taller tile spends fewer threads and L2 reads on the ring).
+constexpr int kPrSmallSR = 16, kPrBigSR = 64;
 
 struct PrShape {
-    int jtiles, ktiles, ni;
+    int jtiles, ktiles, ni, tj;
     unsigned grid;
 };
 
 // At most `target` workgroups (when the columns allow), i segments of even
 // length >= 8 (two planes of each segment's smoother are recomputed by its
-// neighbours).
-static PrShape pr_shape(int nx, int ny, int nz, int target) {
+// neighbours); the rows spread evenly over the j tiles, an even number per
+// tile (up to sr - 4).
+static PrShape pr_shape(int nx, int ny, int nz, int target, int sr) {
     PrShape s;
-    s.jtiles = (ny + kPrTJ - 1) / kPrTJ;
+    const int tmax = sr - 4;
+    s.jtiles = (ny + tmax - 1) / tmax;
+    s.tj = (ny + s.jtiles - 1) / s.jtiles;
+    s.tj += s.tj & 1;
     s.ktiles = (nz + 4 * kPrKT - 1) / (4 * kPrKT);
     const int cols = s.jtiles * s.ktiles;
     int nseg = target / cols;
@@ -799,9 +823,10 @@ __device__ __forceinline__ uint8_t byte_of(uint32_t w, int e) { return (uint8_t)
 // update kernel's float operations), writes r to r_out on the tile (r_out
 // must not alias r_old: the ring cells belong to neighbouring tiles, which
 // still read their old values) and one partial of r.r per workgroup.
-template <bool FOLD>
-__global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int nz, int ni_seg, int jtiles,
-                                                            int ktiles, const uint8_t* __restrict__ code,
+template <bool FOLD, int SR>
+__global__ __launch_bounds__(SR * (kPrKT + 2)) void pre2_restrict_kernel(int nx, int ny, int nz, int ni_seg,
+                                                                         int jtiles, int ktiles, int tj,
+                                                                         const uint8_t* __restrict__ code,
                                                             const float* __restrict__ r, float* __restrict__ z,
                                                             float* __restrict__ rc, const int* __restrict__ done,
                                                             Pre2Coef K, const float* __restrict__ ap,
@@ -809,8 +834,8 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
                                                             float* __restrict__ r_out, double* __restrict__ rr_part) {
     if (done && *done) return;
     const float af = FOLD ? (float)*alpha_p : 0.0f;
-    constexpr int CH = kPrKT + 2, SR = kPrTJ + 4, W = 4 * CH;
-    static_assert(CH * SR == 256, "one thread per chunk of the tile and its ring");
+    constexpr int CH = kPrKT + 2, W = 4 * CH, NT = SR * CH;
+    static_assert(SR % 4 == 0, "a wave holds whole row pairs");
     __shared__ __align__(16) float sR[2][SR * W + 8];
     __shared__ __align__(16) float sZ[2][SR * W + 8];
     const int tid = threadIdx.x;
@@ -819,12 +844,13 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
     const int cols = jtiles * ktiles;
     const int seg = (int)(b / cols), col = (int)(b % cols);
     const int jt = col / ktiles, kt = col - (col / ktiles) * ktiles;
-    const int j0 = jt * kPrTJ, k0 = kt * 4 * kPrKT;
+    const int j0 = jt * tj, k0 = kt * 4 * kPrKT;
     const int i0 = seg * ni_seg, i1 = min(nx, i0 + ni_seg);
     const int j = j0 - 2 + row, k = k0 - 4 + 4 * ch;
-    const bool jk_in = j >= 0 && j < ny && k >= 0 && k < nz;
-    const bool smooth_row = row >= 1 && row <= kPrTJ + 2;
-    const bool inner = row >= 2 && row < kPrTJ + 2 && ch >= 1 && ch <= kPrKT && j < ny && k < nz;
+    // rows past the tile's tj rows and its ring stay idle (air)
+    const bool jk_in = row < tj + 4 && j >= 0 && j < ny && k >= 0 && k < nz;
+    const bool smooth_row = row >= 1 && row <= tj + 2;
+    const bool inner = row >= 2 && row < tj + 2 && ch >= 1 && ch <= kPrKT && j < ny && k < nz;
     const int64_t pl = (int64_t)ny * nz;
     const int64_t off = (int64_t)j * nz + k;
     // planes outside the grid and chunks outside (j,k) read as air (code 0, value 0)
@@ -961,7 +987,7 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
     }
     if (FOLD) {
         double v[1] = {rr};
-        write_partials<1>(v, rr_part);
+        write_partials<1, NT>(v, rr_part);
     }
 }
 
@@ -969,33 +995,47 @@ __global__ __launch_bounds__(256) void pre2_restrict_kernel(int nx, int ny, int 
 // Prolongation and post-smoothing of the fine level in one pass (nz % 4 == 0,
 // 16-byte aligned vectors): zt = z_in + P ec on fluid cells (z_in elsewhere),
 // then z_out = zt + w D^-1 (r - A zt) on fluid cells (0 elsewhere) with the
-// partials of r.z_out and r.z_old.  A workgroup owns a 14 x 56 (j,k) tile and
-// walks down an i segment; each of its 256 threads holds one 4-cell k chunk
-// of the tile or of its one-cell ring.  At plane q it forms zt over the tile
-// and ring (prolong4_kernel's float operations), at plane q-1 the sweep over
-// the tile (smooth_dots_kernel's), so z_out is bit-identical to the two
-// kernels'; only the dot products' summation order differs.  The coarse
-// values come from LDS: each coarse plane's 9 x 34 window under the tile is
-// loaded once into a ring of three (a fine plane reads coarse planes q/2 and
-// q/2 -/+ 1); zt planes are double-buffered.  zt is held at +0 on non-fluid
-// cells (the sweep never reads it there), so the sweep's A is lap_mg with no
-// neighbour codes.
+// partials of r.z_out and r.z_old.  A workgroup owns a tj x 56 (j,k) tile
+// (tj even, at most SR - 2) and walks down an i segment; each of its SR x 16
+// threads holds one 4-cell k chunk of the tile or of its one-cell ring.  At
+// plane q it forms zt over the tile and ring (prolong4_kernel's float
+// operations), at plane q-1 the sweep over the tile (smooth_dots_kernel's),
+// so z_out is bit-identical to the two kernels'; only the dot products'
+// summation order differs.  The coarse values come from LDS: each coarse
+// plane's (tj/2 + 2) x 34 window under the tile is loaded once into a ring of
+// three (a fine plane reads coarse planes q/2 and q/2 -/+ 1); zt planes are
+// double-buffered.  zt is held at +0 on non-fluid cells (the sweep never
+// reads it there), so the sweep's A is lap_mg with no neighbour codes.  On
+// the fine level the tile is 44 x 56 on 768 threads, two workgroups per CU in
+// one round (440x440x280: 500 workgroups of 44 planes, 162 us per pass,
+// against 184 us for 7040 workgroups of 256 threads on 14 x 56 tiles; 172 us
+// for two rounds of the taller tile, 197 us for 1.4 rounds, 188 / 168 us for
+// one round of 62 x 56 on 1024 / 30 x 56 on 512 threads).
 // ---------------------------------------------------------------------------
-constexpr int kPpTJ = 14, kPpKT = 14;
-constexpr int kPpER = kPpTJ / 2 + 2, kPpEK = 2 * kPpKT + 6;  // coarse window: 9 rows x 34
-constexpr int kPpEN = kPpER * kPpEK;
+constexpr int kPpKT = 14;
+constexpr int kPpEK = 2 * kPpKT + 6;  // coarse window columns
+// Thread rows (tile rows + the 2 ring rows) of the two workgroup shapes: 256
+// threads and kPpBigSR * 16 (the V-cycle's fine level).
+constexpr int kPpSmallSR = 16, kPpBigSR = 48;
+// coarse window rows under a tile of up to sr - 2 rows (an even number)
+__host__ __device__ constexpr int pp_er(int sr) { return (sr - 2) / 2 + 2; }
 
 struct PpShape {
-    int jtiles, ktiles, ni;
+    int jtiles, ktiles, ni, tj;
     unsigned grid;
 };
 
-static PpShape pp_shape(int nx, int ny, int nz) {
+// Rows spread evenly over the j tiles, an even number per tile (up to sr - 2);
+// at most `target` workgroups of i segments.
+static PpShape pp_shape(int nx, int ny, int nz, int sr, int target) {
     PpShape s;
-    s.jtiles = (ny + kPpTJ - 1) / kPpTJ;
+    const int tmax = sr - 2;
+    s.jtiles = (ny + tmax - 1) / tmax;
+    s.tj = (ny + s.jtiles - 1) / s.jtiles;
+    s.tj += s.tj & 1;
     s.ktiles = (nz + 4 * kPpKT - 1) / (4 * kPpKT);
     const int cols = s.jtiles * s.ktiles;
-    int nseg = (7680 + cols / 2) / cols;
+    int nseg = target / cols;
     nseg = nseg < 1 ? 1 : nseg;
     int ni = (nx + nseg - 1) / nseg;
     ni = ni < 4 ? 4 : ni;
@@ -1005,29 +1045,32 @@ static PpShape pp_shape(int nx, int ny, int nz) {
     return s;
 }
 
-__global__ __launch_bounds__(256) void prolong_post_kernel(int nx, int ny, int nz, int ni_seg, int jtiles, int ktiles,
-                                                           const uint8_t* __restrict__ code,
+template <int SR>
+__global__ __launch_bounds__(SR * (kPpKT + 2)) void prolong_post_kernel(int nx, int ny, int nz, int ni_seg,
+                                                                        int jtiles, int ktiles, int tj,
+                                                                        const uint8_t* __restrict__ code,
                                                            const float* __restrict__ z_in,
                                                            const float* __restrict__ ec, const float* __restrict__ r,
                                                            const float* __restrict__ z_old, float* __restrict__ z_out,
                                                            double* __restrict__ part, float w,
                                                            const int* __restrict__ done) {
     if (done && *done) return;
-    constexpr int CH = kPpKT + 2, SR = kPpTJ + 2, W = 4 * CH;
-    static_assert(CH * SR == 256, "one thread per chunk of the tile and its ring");
+    constexpr int CH = kPpKT + 2, W = 4 * CH, NT = SR * CH;
+    constexpr int EN = pp_er(SR) * kPpEK;
     __shared__ __align__(16) float sZ[2][SR * W + 8];
-    __shared__ float sE[3][kPpEN];
+    __shared__ float sE[3][EN];
     const int tid = threadIdx.x;
     const int row = tid / CH, ch = tid - (tid / CH) * CH;
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
     const int cols = jtiles * ktiles;
     const int seg = (int)(b / cols), col = (int)(b % cols);
     const int jt = col / ktiles, kt = col - (col / ktiles) * ktiles;
-    const int j0 = jt * kPpTJ, k0 = kt * 4 * kPpKT;
+    const int j0 = jt * tj, k0 = kt * 4 * kPpKT;
     const int i0 = seg * ni_seg, i1 = min(nx, i0 + ni_seg);
     const int j = j0 - 1 + row, k = k0 - 4 + 4 * ch;
-    const bool jk_in = j >= 0 && j < ny && k >= 0 && k < nz;
-    const bool inner = row >= 1 && row <= kPpTJ && ch >= 1 && ch <= kPpKT && j < ny && k < nz;
+    // rows past the tile's tj rows and its ring stay idle (air)
+    const bool jk_in = row < tj + 2 && j >= 0 && j < ny && k >= 0 && k < nz;
+    const bool inner = row >= 1 && row <= tj && ch >= 1 && ch <= kPpKT && j < ny && k < nz;
     const int cnx = (nx + 1) / 2, cny = (ny + 1) / 2, cnz = (nz + 1) / 2;
     const int Jb = j0 / 2 - 1, Kb = k0 / 2 - 3;  // coarse window origin
     const int64_t pl = (int64_t)ny * nz, cpl = (int64_t)cny * cnz;
@@ -1045,7 +1088,7 @@ __global__ __launch_bounds__(256) void prolong_post_kernel(int nx, int ny, int n
     auto load_coarse = [&](int m) {  // coarse plane m's window into its ring slot
         if (m < 0 || m >= cnx) return;
         float* E = sE[m % 3];
-        for (int t = tid; t < kPpEN; t += 256) {
+        for (int t = tid; t < EN; t += NT) {
             const int a = t / kPpEK, c = t - a * kPpEK;
             const int J = Jb + a, K = Kb + c;
             E[t] = (J >= 0 && J < cny && K >= 0 && K < cnz) ? ec[(int64_t)m * cpl + (int64_t)J * cnz + K] : 0.0f;
@@ -1152,7 +1195,16 @@ __global__ __launch_bounds__(256) void prolong_post_kernel(int nx, int ny, int n
         for (int e = 0; e < 4; ++e) { Tmm[e] = Tm[e]; Tm[e] = Tc[e]; }
     }
     double v[2] = {da, db};
-    write_partials<2>(v, part);
+    write_partials<2, NT>(v, part);
+}
+
+// The fine level's taller tiles fill the chip in one round: as many
+// workgroups as the CUs hold at once, each walking a long i segment (fewer,
+// longer segments recompute fewer planes; a partial second round costs more
+// than it hides).  The 256-thread shape keeps 30 per CU.
+static int pp_target(int sr) {
+    static const int occ = resident_wgs(prolong_post_kernel<kPpBigSR>, kPpBigSR * (kPpKT + 2));
+    return (sr == kPpBigSR ? occ : 30) * device_cus();
 }
 
 // z_out = z_in + P ec from level LC to its refinement LF (tiles TF, vector
@@ -1376,7 +1428,7 @@ int mg_setup(hipStream_t st, void* ws, int nx, int ny, int nz, const int* mat, i
 // V-cycle's fine buffers all come from 256-byte aligned workspace, so the
 // vector width depends on nz only).
 unsigned mg_dots_groups(int nx, int ny, int nz) {
-    return (nz & 3) == 0 ? pp_shape(nx, ny, nz).grid : make_tiles(nx, ny, nz, vec_for(nz, {})).ntiles;
+    return (nz & 3) == 0 ? pp_shape(nx, ny, nz, kPpBigSR, pp_target(kPpBigSR)).grid : make_tiles(nx, ny, nz, vec_for(nz, {})).ntiles;
 }
 
 #define MG_VEC(KERNEL, VEC, GRID, ST, ...)                                          \
@@ -1420,24 +1472,35 @@ static void restrict_level(hipStream_t st, const Tiles& TF, int vf, const MLvl& 
 // CU, two rounds of the 8 each CU holds (440x440x280: 4070 workgroups of
 // 20-plane segments, 258 us, against 6845 of 12 planes, 3.34 rounds, 268 us;
 // 2035 of 40 planes 260 us, 8140 of 10 planes 271 us).
-static int pr_target() { return 16 * device_cus(); }
+static int pr_target() {
+    static const int occ = resident_wgs(pre2_restrict_kernel<true, kPrBigSR>, kPrBigSR * (kPrKT + 2));
+    return occ * device_cus();
+}
 
 // Two smoothing sweeps from zero on level LF and the restricted residual on
 // LC: one pass when the shape allows it (fused != 0), else pre2_kernel and
 // the restriction kernels.
 static int pre_restrict(hipStream_t st, const Tiles& TF, int vf, const MLvl& LF, const MLvl& LC, const Tiles& TC1,
                         const uint8_t* code, const float* r, float* z, float* rc, const int* done, int fused,
-                        int target_wg, Pre2Coef K, const MgFold* fold = nullptr) {
+                        int target_wg, Pre2Coef K, const MgFold* fold = nullptr, bool big = false) {
     if (fused && pr_fits(LF, vf)) {
-        const PrShape S = pr_shape(LF.nx, LF.ny, LF.nz, target_wg);
-        if (fold)
-            pre2_restrict_kernel<true><<<S.grid, 256, 0, st>>>(LF.nx, LF.ny, LF.nz, S.ni, S.jtiles, S.ktiles, code,
-                                                               r, z, rc, done, K, fold->ap, fold->alpha, fold->r_out,
-                                                               fold->rr_part);
+        const int sr = big ? kPrBigSR : kPrSmallSR;
+        const PrShape S = pr_shape(LF.nx, LF.ny, LF.nz, target_wg, sr);
+        const unsigned nt = (unsigned)sr * (kPrKT + 2);
+        if (fold && big)
+            pre2_restrict_kernel<true, kPrBigSR><<<S.grid, nt, 0, st>>>(
+                LF.nx, LF.ny, LF.nz, S.ni, S.jtiles, S.ktiles, S.tj, code, r, z, rc, done, K, fold->ap, fold->alpha,
+                fold->r_out, fold->rr_part);
+        else if (fold)
+            return (int)hipErrorInvalidValue;
+        else if (big)
+            pre2_restrict_kernel<false, kPrBigSR><<<S.grid, nt, 0, st>>>(
+                LF.nx, LF.ny, LF.nz, S.ni, S.jtiles, S.ktiles, S.tj, code, r, z, rc, done, K, nullptr, nullptr,
+                nullptr, nullptr);
         else
-            pre2_restrict_kernel<false><<<S.grid, 256, 0, st>>>(LF.nx, LF.ny, LF.nz, S.ni, S.jtiles, S.ktiles, code,
-                                                                r, z, rc, done, K, nullptr, nullptr, nullptr,
-                                                                nullptr);
+            pre2_restrict_kernel<false, kPrSmallSR><<<S.grid, nt, 0, st>>>(
+                LF.nx, LF.ny, LF.nz, S.ni, S.jtiles, S.ktiles, S.tj, code, r, z, rc, done, K, nullptr, nullptr,
+                nullptr, nullptr);
     } else if (fold) {
         return (int)hipErrorInvalidValue;
     } else {
@@ -1477,7 +1540,7 @@ static void chebyshev(hipStream_t st, const Tiles& T, int vec, const uint8_t* co
 
 unsigned mg_fold_groups(int nx, int ny, int nz) {
     const MLvl L = mk(nx, ny, nz);
-    return pr_fits(L, 4) ? pr_shape(nx, ny, nz, pr_target()).grid : 0u;
+    return pr_fits(L, 4) ? pr_shape(nx, ny, nz, pr_target(), kPrBigSR).grid : 0u;
 }
 
 int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* code0, const float* r, float* z,
@@ -1499,7 +1562,7 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
     const bool fused_post = v0 == 4;
     float* zpre = fused_post ? W.zt0 : z;
     if (fold) pass_mark(st, 1, 0);
-    int rc = pre_restrict(st, T0, v0, W.L0, W.L1, C1, code0, r, zpre, W.r1, done, 1, pr_target(), K0, fold);
+    int rc = pre_restrict(st, T0, v0, W.L0, W.L1, C1, code0, r, zpre, W.r1, done, 1, pr_target(), K0, fold, true);
     if (rc != hipSuccess) return rc;
     if (fold) pass_mark(st, 1, 1);
     if (fold) {
@@ -1555,7 +1618,7 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
         polish();
     } else if (coarse_kind == 1) {
         rc = unet_apply(st, W.unet, W.L2.nx, W.L2.ny, W.L2.nz, params, packed, W.r2, W.code2, W.nfl,
-                        nullptr, 0, W.e2, nullptr, nullptr, done);
+                        nullptr, 0, W.e2, done);
         if (rc != hipSuccess) return rc;
         // the A-norm-optimal step's products; the polish's first two steps
         // apply the scaling (optimal_step's second pass folded in)
@@ -1567,7 +1630,7 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
         jacobi_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.zt2, W.e2, done);
         residual_kernel<<<g2, 256, 0, st>>>(W.L2, W.code2, W.r2, W.e2, W.rs2, done);
         rc = unet_apply(st, W.unet, W.L2.nx, W.L2.ny, W.L2.nz, params, packed, W.rs2, W.code2, W.nfl,
-                        nullptr, 0, W.en2, nullptr, nullptr, done);
+                        nullptr, 0, W.en2, done);
         if (rc != hipSuccess) return rc;
         optimal_step(st, W.L2, W.code2, W.rs2, W.en2, W.part, done);
         add_kernel<<<g2, 256, 0, st>>>(W.L2.n, W.en2, W.e2, done);
@@ -1579,11 +1642,11 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
     chebyshev(st, T1, v1, W.code1, W.r1, W.z1, W.zt1, kL1ChebSteps, kL1ChebLo, done);
     // fine: prolongation, then one sweep fused with r.z, r.z_old
     if (fused_post) {
-        const PpShape P = pp_shape(nx, ny, nz);
+        const PpShape P = pp_shape(nx, ny, nz, kPpBigSR, pp_target(kPpBigSR));
         if (P.grid != mg_dots_groups(nx, ny, nz)) return (int)hipErrorInvalidValue;
         pass_mark(st, 2, 0);
-        prolong_post_kernel<<<P.grid, 256, 0, st>>>(nx, ny, nz, P.ni, P.jtiles, P.ktiles, code0, W.zt0, W.z1, r, z_old,
-                                                    z, dots_part, kPostW, done);
+        prolong_post_kernel<kPpBigSR><<<P.grid, kPpBigSR * (kPpKT + 2), 0, st>>>(
+            nx, ny, nz, P.ni, P.jtiles, P.ktiles, P.tj, code0, W.zt0, W.z1, r, z_old, z, dots_part, kPostW, done);
         pass_mark(st, 2, 1);
         return (int)hipGetLastError();
     }
@@ -1601,7 +1664,8 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
 // Jacobi (w = 0.8) for lo <= 0, the two-step Chebyshev polynomial over
 // [lo, 2] otherwise — and rc[I,J,K] = 0.5 x the sum of r - A z over the 8
 // children of coarse cell (I,J,K) (coarse dims (n+1)/2).  fused = 1 runs the
-// single-pass kernel and is an error unless nx, ny are even, nz % 4 == 0 and
+// single-pass kernel on 256-thread tiles and fused = 2 on the fine level's
+// taller tiles; both are an error unless nx, ny are even, nz % 4 == 0 and
 // the vectors are 16-byte aligned; fused = 0 runs the two-kernel path.
 // The V-cycle's last pass on the fine level (diagnostic entry, no counterpart
 // in the reference): zt = z_in + P ec on fluid cells (cell-centred trilinear
@@ -1609,7 +1673,8 @@ int mg_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const uint8_t* co
 // z_out = zt + w D^-1 (r - A zt) on fluid cells (0 elsewhere), and per
 // workgroup the partials of r.z_out (part[0..G)) and r.z_old (part[G..2G),
 // 0 if z_old is null); *groups = G.  fused = 1 runs the single-pass kernel
-// (nz % 4 == 0 and 16-byte aligned vectors, otherwise an error), fused = 0
+// on 256-thread tiles, fused = 2 on the fine level's taller tiles (both need
+// nz % 4 == 0 and 16-byte aligned vectors, otherwise an error), fused = 0
 // prolong + smooth_dots through tmp (nx*ny*nz floats).  z_out must not alias
 // z_in.
 extern "C" int bs_mg_prolong_post(const unsigned char* code, const float* z_in, const float* ec, const float* r,
@@ -1625,11 +1690,16 @@ extern "C" int bs_mg_prolong_post(const unsigned char* code, const float* z_in, 
     const int vf = vec_for(nz, {z_in, r, z_old, z_out, tmp, code});
     if (fused) {
         if (vf != 4) return (int)hipErrorInvalidValue;
-        const PpShape P = pp_shape(nx, ny, nz);
+        const int sr = fused == 2 ? kPpBigSR : kPpSmallSR;
+        const PpShape P = pp_shape(nx, ny, nz, sr, pp_target(sr));
         if (part_len < 2 * (size_t)P.grid) return (int)hipErrorInvalidValue;
         *groups = (int)P.grid;
-        prolong_post_kernel<<<P.grid, 256, 0, st>>>(nx, ny, nz, P.ni, P.jtiles, P.ktiles, code, z_in, ec, r, z_old,
-                                                    z_out, part, (float)w, nullptr);
+        if (fused == 2)
+            prolong_post_kernel<kPpBigSR><<<P.grid, sr * (kPpKT + 2), 0, st>>>(
+                nx, ny, nz, P.ni, P.jtiles, P.ktiles, P.tj, code, z_in, ec, r, z_old, z_out, part, (float)w, nullptr);
+        else
+            prolong_post_kernel<kPpSmallSR><<<P.grid, sr * (kPpKT + 2), 0, st>>>(
+                nx, ny, nz, P.ni, P.jtiles, P.ktiles, P.tj, code, z_in, ec, r, z_old, z_out, part, (float)w, nullptr);
         return (int)hipGetLastError();
     }
     const Tiles TF = make_tiles(nx, ny, nz, vf);
@@ -1653,8 +1723,9 @@ extern "C" int bs_mg_smooth_restrict(const unsigned char* code, const float* r, 
     const Tiles TF = make_tiles(nx, ny, nz, vf);
     const Tiles TC1 = make_tiles(LC.nx, LC.ny, LC.nz, 1);
     if (lo >= 2.0) return (int)hipErrorInvalidValue;
-    return pre_restrict((hipStream_t)stream, TF, vf, LF, LC, TC1, code, r, z, rc, nullptr, fused, 4096,
-                        lo > 0.0 ? pre2_chebyshev(lo) : Pre2Coef{kOmega, 1.0f, kOmega});
+    return pre_restrict((hipStream_t)stream, TF, vf, LF, LC, TC1, code, r, z, rc, nullptr, fused,
+                        fused == 2 ? pr_target() : 4096, lo > 0.0 ? pre2_chebyshev(lo) : Pre2Coef{kOmega, 1.0f, kOmega},
+                        nullptr, fused == 2);
 }
 
 // `steps` Chebyshev steps (even) over [lo, 2] of D^-1 A on A x = r from x,

@@ -1484,7 +1484,7 @@ static int fcg_precond(hipStream_t st, const FcgBuffers& F, const float* r, floa
     if (F.kind == 1) {
         // z = gamma N(r) with the A-norm-optimal gamma, so r.z >= 0 always.
         const int rc = unet_apply(st, F.pws, F.nx, F.ny, F.nz, F.params, F.packed, r, B.code, F.n_fluid, nullptr, 0,
-                                  z, nullptr, nullptr, done);
+                                  z, done);
         if (rc != hipSuccess) return rc;
         scale_to_optimal(st, F.nx, F.ny, F.nz, B.code, r, z, F.partZ, done);
         fcg_dots_kernel<<<kUNetPartWg, 256, 0, st>>>(n, r, z, z_old, F.partZ, B.ctrl);

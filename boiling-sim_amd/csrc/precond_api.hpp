@@ -14,12 +14,11 @@ constexpr unsigned kUNetPartWg = 1024;  // workgroups of the U-Net's reduction p
 size_t unet_ws_bytes(int nx, int ny, int nz);
 
 // z = N(r) on fluid cells; n_fluid points to the grid's fluid-cell count in
-// device memory.  rr_part (G partials of r.r) may be null; rz_part
-// (2 x kUNetPartWg partials of r.z and r.z_old) is written when not null.
-// Every kernel returns at once when *done != 0.
+// device memory.  rr_part (G partials of r.r) may be null.  Every kernel
+// returns at once when *done != 0.
 int unet_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const float* params, const void* packed,
                const float* r, const uint8_t* code, const unsigned* n_fluid, const double* rr_part, unsigned G,
-               float* z, const float* z_old, double* rz_part, const int* done);
+               float* z, const int* done);
 
 // Multigrid-augmented preconditioner: a V-cycle on the fine grid and its 2x
 // coarsening, with the 4x-coarsened level solved by the U-Net (coarse_kind 1,

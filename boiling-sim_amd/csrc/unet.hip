@@ -143,20 +143,15 @@ inline ConvTiles conv_tiles(const Lvl& L) {
 // Inference-only extras of the first encoder convolutions.  FEAT: the input
 // is not a tensor but the network's features of the residual, formed while
 // staging the halo exactly as features_kernel forms them (r/rms on fluid
-// cells, the one-hot class; rms from the r.r partials, summed by every
-// workgroup in the same order), so the feature tensor never goes to memory;
-// workgroup 0 also stores rms.  POOL: the 2x2x2 average of the block's
+// cells, the one-hot class), so the feature tensor never goes to memory.
+// POOL: the 2x2x2 average of the block's
 // (bf16-rounded) outputs into the coarser grid, pool_kernel's float
 // operations: the blocks are even-aligned, so every coarse cell's children
 // lie in one block.
 struct ConvExtra {
     const float* r = nullptr;
     const uint8_t* code = nullptr;
-    const double* rr_part = nullptr;
-    unsigned G = 0;
-    double n_fluid = 0.0;
-    const unsigned* n_fluid_dev = nullptr;  // when set, the fluid count is read here instead
-    float* rms_out = nullptr;
+    const float* inv_rms = nullptr;  // 1/rms of r (rms_kernel)
     __bf16* pool_out = nullptr;
     int cnx = 0, cny = 0, cnz = 0;
 };
@@ -186,14 +181,7 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(
     const int i0 = ti * kTI, j0 = tj * kTJ, k0 = tk * kTK;
     const int64_t plane = (int64_t)L.ny * L.nz;
     float inv = 0.0f;
-    if constexpr (FEAT) {
-        double v[1] = {0.0};
-        if (X.rr_part) sum_partials<1>(X.rr_part, X.G, v);
-        const double nf = X.n_fluid_dev ? (double)*X.n_fluid_dev : X.n_fluid;
-        const double rms = X.rr_part ? sqrt(v[0] / (nf > 0 ? nf : 1.0)) : 1.0;
-        inv = rms > 0.0 ? (float)(1.0 / rms) : 0.0f;
-        if (blockIdx.x == 0 && threadIdx.x == 0 && X.rms_out) *X.rms_out = (float)rms;
-    }
+    if constexpr (FEAT) inv = *X.inv_rms;
 
     // stage the halo block: element v = (cell h, group q), consecutive
     // threads on consecutive k of one row
@@ -329,49 +317,56 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(
     }
 }
 
-// Final 8 -> 1 convolution (VALU, FP32): y = sum W*d1 + b; z = scale*y on
-// fluid cells.  When r is given, also partials of r.z and (if z_old is given)
-// r.z_old for the flexible-CG beta.
-__global__ __launch_bounds__(256) void conv_out_kernel(Lvl L, const __bf16* __restrict__ in,
+// Final 8 -> 1 convolution (VALU, FP32): y = b + the taps inside the grid, in
+// tap order, each the 8 fmaf of its channels in channel order; z = scale * y
+// on fluid cells, 0 elsewhere.  A workgroup owns one 4 x 4 x 16 block of the
+// convolutions' tiling and stages its input with the one-cell halo in LDS
+// (the layout of conv_mfma_kernel with one channel group), so the 27 taps of
+// a cell are read from LDS instead of 27 gathers from L1/L2; thread (w, row,
+// c) computes cell (i0 + w, j0 + row, k0 + c).
+__global__ __launch_bounds__(256) void conv_out_kernel(Lvl L, ConvTiles T, const __bf16* __restrict__ in,
                                                        const float* __restrict__ w, const float* __restrict__ bias,
                                                        const float* __restrict__ scale_ptr,
                                                        const uint8_t* __restrict__ code,
                                                        float* __restrict__ y_out, float* __restrict__ z_out,
-                                                       const float* __restrict__ r,
-                                                       const float* __restrict__ z_old,
-                                                       double* __restrict__ partials, const int* __restrict__ done) {
+                                                       const int* __restrict__ done) {
     if (done && *done) return;
-    const float scale = scale_ptr ? *scale_ptr : 1.0f;
-    double rz = 0.0, rzo = 0.0;
+    __shared__ bf16x8 tile[kHalo];
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const int tk = (int)(b % T.tk);
+    const uint32_t rr = b / T.tk;
+    const int tj = (int)(rr % T.tj), ti = (int)(rr / T.tj);
+    const int i0 = ti * kTI, j0 = tj * kTJ, k0 = tk * kTK;
     const int64_t plane = (int64_t)L.ny * L.nz;
-    const int64_t stride = (int64_t)gridDim.x * 256;
-    for (int64_t cell = (int64_t)blockIdx.x * 256 + threadIdx.x; cell < L.n; cell += stride) {
-        int i, j, k;
-        cell_ijk(L, (uint32_t)cell, i, j, k);
-        float acc = bias[0];
-#pragma unroll
-        for (int t = 0; t < 27; ++t) {
-            const int di = t / 9 - 1, dj = (t / 3) % 3 - 1, dk = t % 3 - 1;
-            if ((unsigned)(i + di) >= (unsigned)L.nx || (unsigned)(j + dj) >= (unsigned)L.ny ||
-                (unsigned)(k + dk) >= (unsigned)L.nz)
-                continue;
-            const int64_t nb = cell + di * plane + dj * L.nz + dk;
-            const bf16x8 x = *reinterpret_cast<const bf16x8*>(in + nb * 8);
-#pragma unroll
-            for (int ci = 0; ci < 8; ++ci) acc = fmaf((float)x[ci], w[t * 8 + ci], acc);
-        }
-        if (y_out) y_out[cell] = acc;
-        const float z = code[cell] >= kCodeFluid0 ? acc * scale : 0.0f;
-        if (z_out) z_out[cell] = z;
-        if (r) {
-            rz += (double)r[cell] * (double)z;
-            if (z_old) rzo += (double)r[cell] * (double)z_old[cell];
-        }
+    for (int h = threadIdx.x; h < kHalo; h += 256) {
+        const int hk = h % kHK, hr = h / kHK;
+        const int hj = hr % kHJ, hi = hr / kHJ;
+        const int gi = i0 - 1 + hi, gj = j0 - 1 + hj, gk = k0 - 1 + hk;
+        bf16x8 val = bf16x8{};
+        if ((unsigned)gi < (unsigned)L.nx && (unsigned)gj < (unsigned)L.ny && (unsigned)gk < (unsigned)L.nz)
+            val = *reinterpret_cast<const bf16x8*>(in + ((int64_t)gi * plane + (int64_t)gj * L.nz + gk) * 8);
+        tile[h] = val;
     }
-    if (partials) {
-        double v[2] = {rz, rzo};
-        write_partials<2>(v, partials);
+    __syncthreads();
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int row = lane >> 4, c = lane & 15;
+    const int i = i0 + wv, j = j0 + row, k = k0 + c;
+    if (i >= L.nx || j >= L.ny || k >= L.nz) return;
+    const float scale = scale_ptr ? *scale_ptr : 1.0f;
+    float acc = bias[0];
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+        const int di = t / 9 - 1, dj = (t / 3) % 3 - 1, dk = t % 3 - 1;
+        if ((unsigned)(i + di) >= (unsigned)L.nx || (unsigned)(j + dj) >= (unsigned)L.ny ||
+            (unsigned)(k + dk) >= (unsigned)L.nz)
+            continue;
+        const bf16x8 x = tile[((wv + 1 + di) * kHJ + row + 1 + dj) * kHK + c + 1 + dk];
+#pragma unroll
+        for (int ci = 0; ci < 8; ++ci) acc = fmaf((float)x[ci], w[t * 8 + ci], acc);
     }
+    const int64_t cell = (int64_t)i * plane + (int64_t)j * L.nz + k;
+    if (y_out) y_out[cell] = acc;
+    if (z_out) z_out[cell] = code[cell] >= kCodeFluid0 ? acc * scale : 0.0f;
 }
 
 // Input gradient of the final conv: g_d1[cell][ci] = sum_t gy[cell - off(t)] W[t][ci].
@@ -477,21 +472,31 @@ __global__ __launch_bounds__(256) void wgrad_reduce_kernel(const float* __restri
 // Elementwise / resampling kernels (channels-last BF16)
 // ---------------------------------------------------------------------------
 
-// Features of the residual: r/rms, one-hot class; rms = sqrt(r.r / n_fluid)
-// from the r.r partials (or 1 when partials is null and rms_in is null).
-__global__ __launch_bounds__(256) void features_kernel(Lvl L, const float* __restrict__ r,
-                                                       const uint8_t* __restrict__ code,
-                                                       const double* __restrict__ rr_partials, unsigned G,
-                                                       double n_fluid, const unsigned* __restrict__ n_fluid_dev,
-                                                       float* __restrict__ rms_out,
-                                                       __bf16* __restrict__ x0, const int* __restrict__ done) {
+// rms = sqrt(r.r / n_fluid) from the r.r partials (1 when partials is null)
+// into rms[0], and its reciprocal (0 when rms is 0) into rms[1].  One
+// workgroup, so the network's kernels read the two values instead of each
+// summing the partials again.
+__global__ __launch_bounds__(256) void rms_kernel(const double* __restrict__ rr_partials, unsigned G, double n_fluid,
+                                                  const unsigned* __restrict__ n_fluid_dev, float* __restrict__ rms,
+                                                  const int* __restrict__ done) {
     if (done && *done) return;
     double v[1] = {0.0};
     if (rr_partials) sum_partials<1>(rr_partials, G, v);
     const double nf = n_fluid_dev ? (double)*n_fluid_dev : n_fluid;
-    const double rms = rr_partials ? sqrt(v[0] / (nf > 0 ? nf : 1.0)) : 1.0;
-    const float inv = rms > 0.0 ? (float)(1.0 / rms) : 0.0f;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && rms_out) *rms_out = (float)rms;
+    const double rv = rr_partials ? sqrt(v[0] / (nf > 0 ? nf : 1.0)) : 1.0;
+    if (threadIdx.x == 0) {
+        rms[0] = (float)rv;
+        rms[1] = rv > 0.0 ? (float)(1.0 / rv) : 0.0f;
+    }
+}
+
+// Features of the residual: r/rms (inv_rms = 1/rms from rms_kernel), one-hot class.
+__global__ __launch_bounds__(256) void features_kernel(Lvl L, const float* __restrict__ r,
+                                                       const uint8_t* __restrict__ code,
+                                                       const float* __restrict__ inv_rms,
+                                                       __bf16* __restrict__ x0, const int* __restrict__ done) {
+    if (done && *done) return;
+    const float inv = *inv_rms;
     const int64_t stride = (int64_t)gridDim.x * 256;
     for (int64_t cell = (int64_t)blockIdx.x * 256 + threadIdx.x; cell < L.n; cell += stride) {
         const uint8_t cc = code[cell];
@@ -927,17 +932,16 @@ static void conv_wgrad(hipStream_t st, const Lvl& L, const __bf16* in, const __b
 // pre-skip activations for backward.
 static int unet_forward(hipStream_t st, UNetWs& W, const float* params, const __bf16* packed, const float* r,
                         const uint8_t* code, double n_fluid, const unsigned* n_fluid_dev,
-                        const double* rr_partials, unsigned G, float* z,
-                        const float* z_old, double* rz_partials, const int* done, int training) {
+                        const double* rr_partials, unsigned G, float* z, const int* done, int training) {
     const unsigned eg1 = elem_grid(W.L1.n);
     if (!rr_partials) {
         rr_partials_kernel<<<kPartWg, 256, 0, st>>>(W.L1.n, r, W.part);
         rr_partials = W.part;
         G = kPartWg;
     }
+    rms_kernel<<<1, 256, 0, st>>>(rr_partials, G, n_fluid, n_fluid_dev, W.rms, done);
     if (training) {
-        features_kernel<<<eg1, 256, 0, st>>>(W.L1, r, code, rr_partials, G, n_fluid, n_fluid_dev, W.rms, W.x0,
-                                             done);
+        features_kernel<<<eg1, 256, 0, st>>>(W.L1, r, code, W.rms + 1, W.x0, done);
         conv_fwd<8, 8>(st, W.L1, W.x0, packed, 0, params, 1, W.h1, nullptr, nullptr, done);
         pool_kernel<8><<<elem_grid(W.L2.n), 256, 0, st>>>(W.L1, W.L2, W.h1, W.p1, done);
         conv_fwd<8, 16>(st, W.L2, W.p1, packed, 1, params, 1, W.h2, nullptr, nullptr, done);
@@ -946,8 +950,7 @@ static int unet_forward(hipStream_t st, UNetWs& W, const float* params, const __
         // the same values without the feature tensor and the two pooling
         // passes (GPU test: bit-identical to the training forward)
         ConvExtra X1;
-        X1.r = r; X1.code = code; X1.rr_part = rr_partials; X1.G = G; X1.n_fluid = n_fluid;
-        X1.n_fluid_dev = n_fluid_dev; X1.rms_out = W.rms;
+        X1.r = r; X1.code = code; X1.inv_rms = W.rms + 1;
         X1.pool_out = W.p1; X1.cnx = W.L2.nx; X1.cny = W.L2.ny; X1.cnz = W.L2.nz;
         conv_fwd_x<8, 8, true, true>(st, W.L1, nullptr, packed, 0, params, 1, W.h1, done, X1);
         ConvExtra X2;
@@ -968,9 +971,9 @@ static int unet_forward(hipStream_t st, UNetWs& W, const float* params, const __
         conv_fwd<16, 8>(st, W.L1, W.s2, packed, 5, params, 1, nullptr, W.h1, W.s1, done, &W.L2);
     }
     conv_fwd<8, 8>(st, W.L1, W.s1, packed, 6, params, 1, W.d1, nullptr, nullptr, done);
-    conv_out_kernel<<<rz_partials ? (unsigned)kPartWg : eg1, 256, 0, st>>>(
-        W.L1, W.d1, params + conv_w_off(7), params + conv_b_off(7), W.rms, code, W.y, z,
-        rz_partials ? r : nullptr, z_old, rz_partials, done);
+    const ConvTiles T1 = conv_tiles(W.L1);
+    conv_out_kernel<<<T1.ti * T1.tj * T1.tk, 256, 0, st>>>(W.L1, T1, W.d1, params + conv_w_off(7),
+                                                          params + conv_b_off(7), W.rms, code, W.y, z, done);
     return (int)hipGetLastError();
 }
 
@@ -1006,7 +1009,7 @@ extern "C" int bs_unet_forward(const float* params, const void* packed, const fl
     UNetWs W = carve_unet(workspace, nx, ny, nz, training);
     if (workspace_bytes < W.bytes) return (int)hipErrorInvalidValue;
     return unet_forward(stream, W, params, (const __bf16*)packed, r, code, n_fluid, nullptr, nullptr, 0, z, nullptr,
-                        nullptr, nullptr, training);
+                        training);
 }
 
 // After bs_unet_forward(training=1) on (r, ...): loss and gradients for the
@@ -1022,8 +1025,10 @@ extern "C" int bs_unet_backward(const float* params, const void* packed_v, const
     const __bf16* packed = (const __bf16*)packed_v;
     const unsigned eg1 = elem_grid(W.L1.n);
     // z (recomputed from d1) -> d = z - e -> A d; e.r normaliser.
-    conv_out_kernel<<<eg1, 256, 0, st>>>(W.L1, W.d1, params + conv_w_off(7), params + conv_b_off(7), W.rms, code,
-                                         nullptr, W.gy, nullptr, nullptr, nullptr, nullptr);
+    const ConvTiles T1 = conv_tiles(W.L1);
+    conv_out_kernel<<<T1.ti * T1.tj * T1.tk, 256, 0, st>>>(W.L1, T1, W.d1, params + conv_w_off(7),
+                                                          params + conv_b_off(7), W.rms, code, nullptr, W.gy,
+                                                          nullptr);
     sub_kernel<<<eg1, 256, 0, st>>>(W.L1.n, W.gy, e, W.gy);
     const Tiles T = make_tiles(nx, ny, nz, 1);
     const unsigned nwg = T.ntiles < 2048 ? T.ntiles : 2048;
@@ -1098,9 +1103,9 @@ size_t unet_ws_bytes(int nx, int ny, int nz) { return carve_unet(nullptr, nx, ny
 
 int unet_apply(hipStream_t st, void* ws, int nx, int ny, int nz, const float* params, const void* packed,
                const float* r, const uint8_t* code, const unsigned* n_fluid, const double* rr_part, unsigned G,
-               float* z, const float* z_old, double* rz_part, const int* done) {
+               float* z, const int* done) {
     UNetWs W = carve_unet(ws, nx, ny, nz, 0);
-    return unet_forward(st, W, params, (const __bf16*)packed, r, code, 0.0, n_fluid, rr_part, G, z, z_old, rz_part,
+    return unet_forward(st, W, params, (const __bf16*)packed, r, code, 0.0, n_fluid, rr_part, G, z,
                         done, 0);
 }
 

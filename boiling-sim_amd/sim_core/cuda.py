@@ -45,6 +45,18 @@ def _i32(name: str, v) -> int:
     return v
 
 
+def _fused(v) -> int:
+    """Kernel-shape selector of the V-cycle diagnostic entries: False/0 the
+    split kernels, True/1 the single pass on 256-thread tiles, 2 the single
+    pass on the fine level's taller tiles."""
+    if v is True or v is False:
+        return int(v)
+    v = int(v)
+    if v not in (0, 1, 2):
+        raise ValueError(f"fused={v} is not 0, 1 or 2")
+    return v
+
+
 class SimCore:
     """Handle on one HIP device (reference: ``SimCore(device_id)``)."""
 
@@ -340,9 +352,10 @@ class SimCore:
         """The V-cycle's first pass on one level: z = two smoothing steps from
         zero (damped Jacobi for lo <= 0, Chebyshev over [lo, 2] otherwise), rc =
         0.5 x the children's sum of r - A z (see bs_mg_smooth_restrict); fused
-        selects the single-pass kernel."""
+        selects the single-pass kernel (True or 1: 256-thread tiles, 2: the fine
+        level's taller tiles)."""
         _lib.call("bs_mg_smooth_restrict", _ptr("code_ptr", code_ptr), _ptr("r_ptr", r_ptr), _ptr("z_ptr", z_ptr),
-                  _ptr("rc_ptr", rc_ptr), _usize("nx", nx), _usize("ny", ny), _usize("nz", nz), 1 if fused else 0,
+                  _ptr("rc_ptr", rc_ptr), _usize("nx", nx), _usize("ny", ny), _usize("nz", nz), _fused(fused),
                   float(lo), _ptr("stream", stream))
 
     def mg_chebyshev(self, code_ptr, r_ptr, x_ptr, tmp_ptr, nx, ny, nz, steps, lo, stream=0) -> None:
@@ -354,13 +367,14 @@ class SimCore:
 
     def mg_prolong_post(self, code_ptr, z_in_ptr, ec_ptr, r_ptr, z_old_ptr, z_out_ptr, tmp_ptr, part_ptr, part_len,
                         nx, ny, nz, fused=True, w=1.0, stream=0) -> int:
-        """The V-cycle's last fine-level pass (see bs_mg_prolong_post); returns
-        the number G of workgroup partials written per dot product."""
+        """The V-cycle's last fine-level pass (see bs_mg_prolong_post; fused as
+        for mg_smooth_restrict); returns the number G of workgroup partials
+        written per dot product."""
         g = ctypes.c_int(0)
         _lib.call("bs_mg_prolong_post", _ptr("code_ptr", code_ptr), _ptr("z_in_ptr", z_in_ptr),
                   _ptr("ec_ptr", ec_ptr), _ptr("r_ptr", r_ptr), z_old_ptr or None, _ptr("z_out_ptr", z_out_ptr),
                   _ptr("tmp_ptr", tmp_ptr), _ptr("part_ptr", part_ptr), int(part_len), ctypes.byref(g),
-                  _usize("nx", nx), _usize("ny", ny), _usize("nz", nz), 1 if fused else 0, float(w),
+                  _usize("nx", nx), _usize("ny", ny), _usize("nz", nz), _fused(fused), float(w),
                   _ptr("stream", stream))
         return int(g.value)
 

@@ -319,7 +319,9 @@ int bs_fcg_pass_timing(int enable, double* us_median, double* us_mean, int* coun
  * of r - A z over the fluid children of coarse cell (I,J,K), coarse dims
  * ((nx+1)/2, (ny+1)/2, (nz+1)/2).  fused = 1 runs the single-pass kernel the
  * V-cycle uses where the shape allows (nx, ny even, nz % 4 == 0, 16-byte
- * aligned vectors; otherwise an error), fused = 0 the two-kernel path.
+ * aligned vectors; otherwise an error) on the 256-thread tiles of its 2x
+ * level, fused = 2 the same kernel on the taller tiles of its fine level,
+ * fused = 0 the two-kernel path.
  * Asynchronous on `stream`. */
 int bs_mg_smooth_restrict(const unsigned char* code, const float* r, float* z, float* rc, int nx, int ny, int nz,
                           int fused, double lo, bs_stream_t stream);
@@ -330,7 +332,8 @@ int bs_mg_smooth_restrict(const unsigned char* code, const float* r, float* z, f
  * w D^-1 (r - A zt) on fluid cells (0 elsewhere); part[0..G) receives per
  * workgroup partials of r.z_out and part[G..2G) of r.z_old (z_old may be
  * null), *groups = G.  fused = 1 runs the single-pass kernel the V-cycle uses
- * (nz % 4 == 0, 16-byte aligned vectors; otherwise an error), fused = 0 the
+ * (nz % 4 == 0, 16-byte aligned vectors; otherwise an error) on 256-thread
+ * tiles, fused = 2 on the taller tiles of the V-cycle's fine level, fused = 0 the
  * prolongation and the sweep as two kernels through tmp (nx*ny*nz floats).
  * z_out must not alias z_in; an empty grid writes *groups = 0 and nothing
  * else.  Asynchronous on `stream`. */

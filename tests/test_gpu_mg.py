@@ -91,8 +91,9 @@ def _mask(kind, shape, rng):
     return m
 
 
-# (mask, shape): partial j tiles (12 rows) and k tiles (56 cells), several i
-# segments with a short last one, fluid on the domain faces (random masks).
+# (mask, shape): partial j tiles (12 rows; up to 60 on the taller tiles) and
+# k tiles (56 cells), several i segments with a short last one, fluid on the
+# domain faces (random masks).
 CASES = [
     ("pot", (64, 64, 48)),
     ("pot", (90, 30, 60)),
@@ -101,6 +102,7 @@ CASES = [
     ("random", (8, 12, 4)),
     ("random", (98, 24, 16)),
     ("pot", (128, 36, 64)),
+    ("random", (20, 130, 16)),  # three of the fine level's taller j tiles, the last partial
 ]
 
 
@@ -116,21 +118,24 @@ def test_fused_pass_matches_two_kernels(simcore, torch_gpu, kind, shape, lo):
     cshape = tuple((n + 1) // 2 for n in shape)
     cd, rd = dev(torch, code), dev(torch, r)
     out = {}
-    for fused in (1, 0):
+    for fused in (2, 1, 0):
         z = torch.full(shape, np.nan, dtype=torch.float32, device="cuda")
         rc = torch.full(cshape, np.nan, dtype=torch.float32, device="cuda")
         simcore.mg_smooth_restrict(cd.data_ptr(), rd.data_ptr(), z.data_ptr(), rc.data_ptr(), *shape, fused, lo)
         torch.cuda.synchronize()
         out[fused] = (host(z), host(rc))
-    (zf, rcf), (zu, rcu) = out[1], out[0]
-    print(f"[{kind} {shape}] fluid {int((code >= 2).sum())}, max|rc| {np.abs(rcu).max():.3e}, "
-          f"max|rc_f - rc_u| {np.abs(rcf - rcu).max():.3e}")
-    # every cell written, z bit-identical (the same float operations)
-    assert np.isfinite(zf).all() and np.isfinite(rcf).all()
-    assert np.array_equal(zf.view(np.uint32), zu.view(np.uint32))
-    # the restriction sums the same FP32 terms in another order
-    scale = np.abs(rcu).max()
-    np.testing.assert_allclose(rcf, rcu, rtol=0, atol=2e-6 * scale)
+    (zu, rcu) = out[0]
+    for fused in (1, 2):  # 256-thread tiles, the fine level's taller tiles
+        zf, rcf = out[fused]
+        print(f"[{kind} {shape} fused {fused}] fluid {int((code >= 2).sum())}, max|rc| {np.abs(rcu).max():.3e}, "
+              f"max|rc_f - rc_u| {np.abs(rcf - rcu).max():.3e}")
+        # every cell written, z bit-identical (the same float operations)
+        assert np.isfinite(zf).all() and np.isfinite(rcf).all()
+        assert np.array_equal(zf.view(np.uint32), zu.view(np.uint32))
+        # the restriction sums the same FP32 terms in another order
+        scale = np.abs(rcu).max()
+        np.testing.assert_allclose(rcf, rcu, rtol=0, atol=2e-6 * scale)
+    zf, rcf = out[2]
     # both against the FP64 restatement
     z64, rc64 = _reference(code, r, lo)
     np.testing.assert_allclose(zu, z64, rtol=0, atol=1e-6 * np.abs(z64).max())
@@ -251,6 +256,7 @@ PP_CASES = [
     ("random", (8, 12, 4)),
     ("random", (98, 24, 16)),
     ("random", (5, 1, 8)),
+    ("random", (20, 130, 16)),  # three of the fine level's taller j tiles, the last partial
 ]
 
 
@@ -272,7 +278,7 @@ def test_prolong_post_matches_two_kernels(simcore, torch_gpu, kind, shape, with_
     cd, zid, ecd, rd, zod = held
     n = int(np.prod(shape))
     out = {}
-    for fused in (1, 0):
+    for fused in (2, 1, 0):
         z = torch.full(shape, np.nan, dtype=torch.float32, device="cuda")
         tmp = torch.full(shape, np.nan, dtype=torch.float32, device="cuda")
         part = torch.full((2 * n + 64,), np.nan, dtype=torch.float64, device="cuda")
@@ -283,10 +289,12 @@ def test_prolong_post_matches_two_kernels(simcore, torch_gpu, kind, shape, with_
         p = host(part)
         out[fused] = (host(z), p[:g].sum(), p[g:2 * g].sum())
     (zf, af, bf), (zu, au, bu) = out[1], out[0]
-    print(f"[{kind} {shape} z_old {with_old}] fluid {int(fl.sum())}, r.z {af:.6e} / {au:.6e}, "
-          f"r.z_old {bf:.6e} / {bu:.6e}")
-    assert np.isfinite(zf).all()
+    zt2, at2, bt2 = out[2]  # the fine level's taller tiles
+    print(f"[{kind} {shape} z_old {with_old}] fluid {int(fl.sum())}, r.z {af:.6e} / {at2:.6e} / {au:.6e}, "
+          f"r.z_old {bf:.6e} / {bt2:.6e} / {bu:.6e}")
+    assert np.isfinite(zf).all() and np.isfinite(zt2).all()
     assert np.array_equal(zf.view(np.uint32), zu.view(np.uint32))
+    assert np.array_equal(zt2.view(np.uint32), zu.view(np.uint32))
     # FP64 restatement: zt = z_in + P ec, one undamped Jacobi sweep from zt
     zt = z_in.astype(np.float64) + _prolong_reference(code, ec)
     z64 = np.where(fl, zt + _dinv(code) * (r - _apply(code, zt)), 0.0)
@@ -296,9 +304,9 @@ def test_prolong_post_matches_two_kernels(simcore, torch_gpu, kind, shape, with_
     b64 = float((r.astype(np.float64) * z_old.astype(np.float64)).sum()) if with_old else 0.0
     scale_a = float(np.abs(r.astype(np.float64) * zf).sum())
     scale_b = float(np.abs(r.astype(np.float64) * z_old).sum())
-    for got in (af, au):
+    for got in (af, at2, au):
         assert abs(got - a64) <= 1e-11 * scale_a
-    for got in (bf, bu):
+    for got in (bf, bt2, bu):
         assert abs(got - b64) <= 1e-11 * scale_b
 
 
