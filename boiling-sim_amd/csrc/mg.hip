@@ -765,11 +765,14 @@ __global__ __launch_bounds__(256) void prolong4_kernel(Tiles F, Tiles CV, const 
 // workgroups (the fine level and its 2x coarsening).  On the fine level the
 // tile is 56 x 56 on 1024 threads (of which the ring holds a quarter,
 // against a third of 256 threads on 12 x 56) and the grid one round of one
-// workgroup per CU on long segments: 440x440x280, 240 workgroups of 38
+// workgroup per CU on long segments: 440x440x280, 240 workgroups of 74
 // planes, 219 us per pass, against 257 us for 4070 workgroups of 256
 // threads on 20 planes (and 242 / 243 / 248 us for 2 / 3 / 4 rounds of the
 // taller tile, 234 us for 44 x 56 tiles on 768 threads).  The 2x level
-// keeps the 256-thread tile (the taller one there: the same time).
+// keeps the 256-thread tile (the taller one there: the same time).  All
+// columns of a segment walk the same planes together, which keeps HBM reads
+// within few planes at a time: equal shares of (column, plane) pairs in
+// column order, which would use all 256 CUs, ran at 435 us.
 // ---------------------------------------------------------------------------
 // Workgroups of `fn` (nt threads each) that one CU holds at once, from the
 // runtime's occupancy calculator (the built kernel's register and LDS use).

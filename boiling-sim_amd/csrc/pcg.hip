@@ -339,8 +339,10 @@ static RollShape roll_shape(int nx, int ny, int nz) {
 // segment.  Measured in one run at 440x440x280 (tile rows, workgroups, i
 // planes each: us): 55, 240, 74: 289.5; 55, 480, 37: 297; 55, 1000, 18: 301.5;
 // 62, 240, 74: 303; 62, 1040, 17 (the earlier rule, just past two rounds of
-// the two workgroups a CU holds): 320; 55, 280, 63 (just past one round): 424.
-// Plain PCG keeps the 14-row tile of pcg_spmv_roll_kernel.
+// the two workgroups a CU holds): 320; 55, 280, 63 (just past one round): 424;
+// 256 equal shares of the (column, plane) pairs in column order, which
+// leaves no CU idle but has the workgroups on 256 different planes at once:
+// 524.  Plain PCG keeps the 14-row tile of pcg_spmv_roll_kernel.
 constexpr int kDirTJ = 62, kDirThreads = (kDirTJ + 2) * (kRollKT + 2);
 static RollShape dir_shape(int nx, int ny, int nz) {
     RollShape s;
