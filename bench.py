@@ -58,11 +58,11 @@ B_JACOBI_REF = 16.0
 B_PCG_ITER = 42.0
 # The learned flexible-CG iteration's three fine-level passes:
 #   fcg_spmv_dir_kernel: code, z, p_old, x read; p, Ap, x written = 25
-#   pre2_restrict_kernel<true>: code, r_old, Ap read; r, z written, the
+#   pre2_restrict_kernel<true, 64>: code, r_old, Ap read; r, z written, the
 #       restricted residual (1/8 of a field) written = 17.5
-#   prolong_post_kernel: code, z_in, r, the coarse correction (1/8) read;
+#   prolong_post_kernel<48>: code, z_in, r, the coarse correction (1/8) read;
 #       z written = 13.5
-PASS_BYTES = {"fcg_spmv_dir_kernel": 25.0, "pre2_restrict_kernel<true>": 17.5, "prolong_post_kernel": 13.5}
+PASS_BYTES = {"fcg_spmv_dir_kernel": 25.0, "pre2_restrict_kernel<true, 64>": 17.5, "prolong_post_kernel<48>": 13.5}
 B_LEARNED_FINE = sum(PASS_BYTES.values())  # 56
 # HBM bytes per cell one launch actually moves at 440x440x280, measured with
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (one counter per run) and
@@ -73,8 +73,10 @@ B_LEARNED_FINE = sum(PASS_BYTES.values())  # 56
 TRAFFIC_JACOBI_PAIR = 14.207
 # The learned iteration's fine-level passes, measured the same way over
 # `bench.py --legs passes` (reads = FETCH_SIZE x 1.99965 from max_abs_kernel,
-# writes = WRITE_SIZE; dispatches that returned at once left out).
-TRAFFIC_PASSES = {"fcg_spmv_dir_kernel": 27.40, "pre2_restrict_kernel<true>": 22.59, "prolong_post_kernel": 16.35}
+# writes = WRITE_SIZE; means over the 45 launches that ran): read + written
+# KiB per launch 709,727 + 647,096, 517,636 + 450,094 and 602,806 + 216,228.
+TRAFFIC_PASSES = {"fcg_spmv_dir_kernel": 25.63, "pre2_restrict_kernel<true, 64>": 18.28,
+                  "prolong_post_kernel<48>": 15.47}
 
 
 # The same flexible CG and V-cycle with non-learned solves of the 4x-coarsened
@@ -329,7 +331,7 @@ def advection_leg(core, torch, gmat, mat, ux0, uy0, uz0, cfg, nx, ny, nz, n, rep
 def pass_leg(core, torch, projection, cal_src, n, reps=3):
     """The learned solve's fine-level passes timed inside `reps` learned
     projections run with pass timing on (bs_fcg_pass_timing: the solves
-    launch eagerly and bracket fcg_spmv_dir_kernel, pre2_restrict_kernel<true>
+    launch eagerly and bracket fcg_spmv_dir_kernel, pre2_restrict_kernel<true, 64>
     and prolong_post_kernel with HIP events on the solve's stream).  A plain
     device copy of one field runs first: its FETCH_SIZE / WRITE_SIZE counters
     calibrate a counter run over this leg (bench.py --legs passes)."""

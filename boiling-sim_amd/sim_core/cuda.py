@@ -330,7 +330,7 @@ class SimCore:
         return st.as_dict()
 
     # Fine-level passes of the learned flexible CG, in bs_fcg_pass_timing's order.
-    TIMED_PASSES = ("fcg_spmv_dir_kernel", "pre2_restrict_kernel<true>", "prolong_post_kernel")
+    TIMED_PASSES = ("fcg_spmv_dir_kernel", "pre2_restrict_kernel<true, 64>", "prolong_post_kernel<48>")
 
     @staticmethod
     def fcg_pass_timing_start() -> None:
