@@ -26,11 +26,11 @@ struct CondParams {
     int mat_air, n_mat;
 };
 
-// The pair table: g_tab[m][n] for a non-air neighbour n, and for an air
-// neighbour g_air[m] with its term g_air[m]*T_amb.
+// The pair table: g[m][n] for a non-air neighbour n, and in column kMaxMat
+// the air neighbour's g_air[m]; t_air[m] = g_air[m]*T_amb is the air face's
+// term.  One table read per face, the air case a select.
 struct CondTable {
-    float g[kMaxMat][kMaxMat];
-    float g_air[kMaxMat];
+    float g[kMaxMat][kMaxMat + 1];
     float t_air[kMaxMat];
 };
 
@@ -49,7 +49,7 @@ __device__ __forceinline__ void build_table(CondTable& tb, const float* __restri
     for (int m = threadIdx.x; m < nm; m += blockDim.x) {
         const float inv_rho_cp = 1.0f / (rho_arr[m] * cp_arr[m]);
         const float g = ((P.dt * P.h) * inv_rho_cp) / P.dx;
-        tb.g_air[m] = g;
+        tb.g[m][kMaxMat] = g;
         tb.t_air[m] = g * P.t_amb;
     }
 }
@@ -108,6 +108,7 @@ __global__ __launch_bounds__(256) void conduction_sweep_kernel(Tiles T, float* _
     }
     __syncthreads();
     if (!work) return;
+    const bool inner = have_l && have_r && have_d && have_u && k0 > 0 && k0 + VEC < T.nz;
     float out[VEC];
 #pragma unroll
     for (int e = 0; e < VEC; ++e) {
@@ -115,25 +116,34 @@ __global__ __launch_bounds__(256) void conduction_sweep_kernel(Tiles T, float* _
         out[e] = 0.0f;
         if (m == ma) continue;
         const int ms = mid(m, nm);
+        const float* grow = tb.g[ms];
+        const float tair = tb.t_air[ms];
         float rhs = 0.0f, diag = 0.0f;
+        // one table read per face, the air case a select
         auto face = [&](int mn, float tn) {
-            float g, term;
-            if (mn == ma) {
-                g = tb.g_air[ms];
-                term = tb.t_air[ms];
-            } else {
-                g = tb.g[ms][mid(mn, nm)];
-                term = g * tn;
-            }
+            const bool air = mn == ma;
+            const float g = grow[air ? kMaxMat : mid(mn, nm)];
+            const float term = air ? tair : g * tn;
             rhs = rhs + term;
             diag = diag + g;
         };
-        if (have_l) face(ml[e], tl[e]);
-        if (have_r) face(mr[e], tr[e]);
-        if (have_d) face(md[e], td[e]);
-        if (have_u) face(mu[e], tu[e]);
-        if (k0 + e > 0) face(e > 0 ? mc[e - 1] : mb, e > 0 ? tc[e - 1] : tb_);
-        if (k0 + e < T.nz - 1) face(e < VEC - 1 ? mc[e + 1] : mf, e < VEC - 1 ? tc[e + 1] : tf);
+        const int mb_e = e > 0 ? mc[e - 1] : mb, mf_e = e < VEC - 1 ? mc[e + 1] : mf;
+        const float tb_e = e > 0 ? tc[e - 1] : tb_, tf_e = e < VEC - 1 ? tc[e + 1] : tf;
+        if (inner) {  // no face of the chunk lies outside the grid
+            face(ml[e], tl[e]);
+            face(mr[e], tr[e]);
+            face(md[e], td[e]);
+            face(mu[e], tu[e]);
+            face(mb_e, tb_e);
+            face(mf_e, tf_e);
+        } else {
+            if (have_l) face(ml[e], tl[e]);
+            if (have_r) face(mr[e], tr[e]);
+            if (have_d) face(md[e], td[e]);
+            if (have_u) face(mu[e], tu[e]);
+            if (k0 + e > 0) face(mb_e, tb_e);
+            if (k0 + e < T.nz - 1) face(mf_e, tf_e);
+        }
         out[e] = (to[e] + rhs) / (1.0f + diag);
     }
     bool all = true;
