@@ -131,6 +131,24 @@ constexpr int kTI = 4, kTJ = 4, kTK = 16;
 constexpr int kHI = kTI + 2, kHJ = kTJ + 2, kHK = kTK + 2;
 constexpr int kHalo = kHI * kHJ * kHK;
 
+// LDS slot (within one channel group's halo block) of tap t = 9 (di + 1) +
+// 3 (dj + 1) + (dk + 1) of the cell at (w, 0, c) of the block, relative to
+// that cell's own slot w kHJ kHK + c: halo cell (w + 1 + di, 1 + dj,
+// c + 1 + dk).  t >= 27 (the K padding) maps to tap 26, which keeps the
+// discarded read inside the tile.
+__host__ __device__ constexpr int tap_lds(int t) {
+    return t >= 27 ? tap_lds(26) : ((t / 9) * kHJ + (t / 3) % 3) * kHK + t % 3;
+}
+
+// tap_lds of taps t0 .. t0 + n - 1 (n <= 4) packed one per byte (every slot
+// is below 256), so a lane picks its tap's slot with one shift and mask.
+__host__ __device__ constexpr uint32_t tap_bytes(int t0, int n) {
+    uint32_t w = 0;
+    for (int u = 0; u < n; ++u) w |= (uint32_t)tap_lds(t0 + u) << (8 * u);
+    return w;
+}
+static_assert(tap_lds(26) < 256, "tap slots fit a byte");
+
 struct ConvTiles {
     uint32_t ti, tj, tk;  // blocks per axis
 };
@@ -166,7 +184,7 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(
     const float* __restrict__ bias, int relu, __bf16* __restrict__ out, const __bf16* __restrict__ skip,
     __bf16* __restrict__ out_sum, const int* __restrict__ done, int cny = 0, int cnz = 0, ConvExtra X = ConvExtra{}) {
     if (done && *done) return;
-    static_assert(CIN % 8 == 0, "channels come in groups of 8");
+    static_assert(CIN == 8 || CIN == 16 || CIN == 32, "8, 16 or 32 input channels");
     static_assert(!FEAT || CIN == 8, "the features are 8 channels");
     constexpr int KS = ksteps_of(CIN);
     constexpr int MT = mtiles_of(COUT);
@@ -183,32 +201,44 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(
     float inv = 0.0f;
     if constexpr (FEAT) inv = *X.inv_rms;
 
-    // stage the halo block: element v = (cell h, group q), consecutive
-    // threads on consecutive k of one row
-    for (int v = threadIdx.x; v < NG * kHalo; v += 256) {
-        const int h = v / NG, q = v - (v / NG) * NG;
-        const int hk = h % kHK, hr = h / kHK;
-        const int hj = hr % kHJ, hi = hr / kHJ;
-        const int gi = i0 - 1 + hi, gj = j0 - 1 + hj, gk = k0 - 1 + hk;
-        bf16x8 val = bf16x8{};
-        if ((unsigned)gi < (unsigned)L.nx && (unsigned)gj < (unsigned)L.ny && (unsigned)gk < (unsigned)L.nz) {
-            if constexpr (FEAT) {
-                const int64_t cell = (int64_t)gi * plane + (int64_t)gj * L.nz + gk;
-                const uint8_t cc = X.code[cell];
-                __bf16 f[8];
-                f[0] = (__bf16)(cc >= kCodeFluid0 ? X.r[cell] * inv : 0.0f);
-                f[1] = (__bf16)(cc >= kCodeFluid0 ? 1.0f : 0.0f);
-                f[2] = (__bf16)(cc == kCodeAir ? 1.0f : 0.0f);
-                f[3] = (__bf16)(cc == kCodeSolid ? 1.0f : 0.0f);
-                f[4] = f[5] = f[6] = f[7] = (__bf16)0.0f;
-                val = *reinterpret_cast<const bf16x8*>(f);
-            } else {
-                const int64_t src = UPIN ? ((int64_t)(gi >> 1) * cny + (gj >> 1)) * cnz + (gk >> 1)
-                                         : (int64_t)gi * plane + (int64_t)gj * L.nz + gk;
-                val = *reinterpret_cast<const bf16x8*>(in + src * CIN + q * 8);
+    // stage the halo block: thread t holds one (k, channel group) slot of a
+    // halo row, (hk, q) = (col / NG, col % NG) with col = t mod (kHK NG), so
+    // consecutive threads read consecutive 16-byte pieces of one row, and
+    // walks the kHI x kHJ halo rows R at a time from row t / (kHK NG)
+    // (threads past R kHK NG stay idle); the per-row work is one small
+    // division.
+    {
+        constexpr int COLS = kHK * NG, R = 256 / COLS, ROWS = kHI * kHJ;
+        const int col = (int)threadIdx.x % COLS, row0 = (int)threadIdx.x / COLS;
+        const int hk = col / NG, q = col - (col / NG) * NG;
+        const int gk = k0 - 1 + hk;
+        const bool k_in = row0 < R && (unsigned)gk < (unsigned)L.nz;
+#pragma unroll
+        for (int ps = 0; ps < (ROWS + R - 1) / R; ++ps) {
+            const int hr = row0 + ps * R;
+            if (row0 >= R || hr >= ROWS) break;
+            const int hi = hr / kHJ, hj = hr - (hr / kHJ) * kHJ;
+            const int gi = i0 - 1 + hi, gj = j0 - 1 + hj;
+            bf16x8 val = bf16x8{};
+            if (k_in && (unsigned)gi < (unsigned)L.nx && (unsigned)gj < (unsigned)L.ny) {
+                if constexpr (FEAT) {
+                    const int64_t cell = (int64_t)gi * plane + (int64_t)gj * L.nz + gk;
+                    const uint8_t cc = X.code[cell];
+                    __bf16 f[8];
+                    f[0] = (__bf16)(cc >= kCodeFluid0 ? X.r[cell] * inv : 0.0f);
+                    f[1] = (__bf16)(cc >= kCodeFluid0 ? 1.0f : 0.0f);
+                    f[2] = (__bf16)(cc == kCodeAir ? 1.0f : 0.0f);
+                    f[3] = (__bf16)(cc == kCodeSolid ? 1.0f : 0.0f);
+                    f[4] = f[5] = f[6] = f[7] = (__bf16)0.0f;
+                    val = *reinterpret_cast<const bf16x8*>(f);
+                } else {
+                    const int64_t src = UPIN ? ((int64_t)(gi >> 1) * cny + (gj >> 1)) * cnz + (gk >> 1)
+                                             : (int64_t)gi * plane + (int64_t)gj * L.nz + gk;
+                    val = *reinterpret_cast<const bf16x8*>(in + src * CIN + q * 8);
+                }
             }
+            tile[q * kHalo + hr * kHK + hk] = val;
         }
-        tile[q * kHalo + h] = val;
     }
     __syncthreads();
 
@@ -221,21 +251,33 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(
 #pragma unroll
         for (int nt = 0; nt < kTJ; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-#pragma unroll 2
+    // Fully unrolled, so each k-step's taps are constants: lane group g of
+    // k-step s takes channels 32 s + 8 g .. + 7 of the K axis, i.e. channel
+    // group q of tap t = (32 s + 8 g) / CIN (t = 27 on the padding of the
+    // last k-step, where B = 0).  Per step that is a select among at most four
+    // constant LDS offsets instead of the tap's division and decomposition.
+    const int lane_org = w * kHJ * kHK + c;
+#pragma unroll
     for (int s = 0; s < KS; ++s) {
         bf16x8 a[MT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) a[mt] = wfrag[((size_t)s * MT + mt) * 64 + lane];
-        const int kk = s * 32 + g * 8;
-        const int t = kk / CIN;  // 27 on the padding of the last k-step (B = 0 there)
-        const int q = (kk - t * CIN) / 8;
-        const int ta = t < 27 ? t : 26;  // keeps the (discarded) LDS read inside the tile
-        const int di = ta / 9 - 1, dj = (ta / 3) % 3 - 1, dk = ta % 3 - 1;
-        // halo cell of tile row nt: (w+1+di, nt+1+dj, c+1+dk)
-        const int hbase = q * kHalo + ((w + 1 + di) * kHJ + 1 + dj) * kHK + c + 1 + dk;
+        int off;
+        bool live;
+        if constexpr (CIN == 8) {  // t = 4 s + g, q = 0
+            off = (int)((tap_bytes(4 * s, 4) >> (8 * g)) & 0xFFu);
+            live = 4 * s + g < 27;
+        } else if constexpr (CIN == 16) {  // t = 2 s + g / 2, q = g % 2
+            off = (g & 1) * kHalo + (int)((tap_bytes(2 * s, 2) >> (8 * (g >> 1))) & 0xFFu);
+            live = 2 * s + (g >> 1) < 27;
+        } else {  // t = s, q = g
+            off = g * kHalo + tap_lds(s);
+            live = true;
+        }
+        const int hbase = lane_org + off;
 #pragma unroll
         for (int nt = 0; nt < kTJ; ++nt) {
-            const bf16x8 bv = t < 27 ? tile[hbase + nt * kHK] : bf16x8{};
+            const bf16x8 bv = live ? tile[hbase + nt * kHK] : bf16x8{};
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt)
                 acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], bv, acc[mt][nt], 0, 0, 0);
@@ -338,14 +380,22 @@ __global__ __launch_bounds__(256) void conv_out_kernel(Lvl L, ConvTiles T, const
     const int tj = (int)(rr % T.tj), ti = (int)(rr / T.tj);
     const int i0 = ti * kTI, j0 = tj * kTJ, k0 = tk * kTK;
     const int64_t plane = (int64_t)L.ny * L.nz;
-    for (int h = threadIdx.x; h < kHalo; h += 256) {
-        const int hk = h % kHK, hr = h / kHK;
-        const int hj = hr % kHJ, hi = hr / kHJ;
-        const int gi = i0 - 1 + hi, gj = j0 - 1 + hj, gk = k0 - 1 + hk;
-        bf16x8 val = bf16x8{};
-        if ((unsigned)gi < (unsigned)L.nx && (unsigned)gj < (unsigned)L.ny && (unsigned)gk < (unsigned)L.nz)
-            val = *reinterpret_cast<const bf16x8*>(in + ((int64_t)gi * plane + (int64_t)gj * L.nz + gk) * 8);
-        tile[h] = val;
+    {  // conv_mfma_kernel's staging with one channel group
+        constexpr int R = 256 / kHK, ROWS = kHI * kHJ;
+        const int hk = (int)threadIdx.x % kHK, row0 = (int)threadIdx.x / kHK;
+        const int gk = k0 - 1 + hk;
+        const bool k_in = row0 < R && (unsigned)gk < (unsigned)L.nz;
+#pragma unroll
+        for (int ps = 0; ps < (ROWS + R - 1) / R; ++ps) {
+            const int hr = row0 + ps * R;
+            if (row0 >= R || hr >= ROWS) break;
+            const int hi = hr / kHJ, hj = hr - (hr / kHJ) * kHJ;
+            const int gi = i0 - 1 + hi, gj = j0 - 1 + hj;
+            bf16x8 val = bf16x8{};
+            if (k_in && (unsigned)gi < (unsigned)L.nx && (unsigned)gj < (unsigned)L.ny)
+                val = *reinterpret_cast<const bf16x8*>(in + ((int64_t)gi * plane + (int64_t)gj * L.nz + gk) * 8);
+            tile[hr * kHK + hk] = val;
+        }
     }
     __syncthreads();
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
