@@ -1207,6 +1207,14 @@ __global__ __launch_bounds__(256) void fcg_direction_kernel(Tiles T, const float
     }
 }
 
+// nbr_term where the value is exactly +0 on every non-fluid cell: an air
+// neighbour's v is then the +0 nbr_term would return, so only the solid
+// case needs a select.  The directions p = z + beta p hold +0 there: every
+// preconditioner output z is +0 off the fluid (the V-cycle's last pass and
+// the network store 0.0f there), p starts as z, and +0 + beta * (+0) is +0
+// for either sign of beta; planes and chunks outside the grid read as +0.
+__device__ __forceinline__ float nbr_term_z(uint8_t c, float v, float ps) { return c == kCodeSolid ? ps : v; }
+
 // The flexible-CG iteration's first pass (nz % 4 == 0, 16-byte aligned
 // vectors), pcg_spmv_roll_kernel's stencil on the taller dir_shape tile: the direction
 // p_new = z + beta p_old (p_new = z when ctrl->fresh), Ap = A p_new with the
@@ -1304,12 +1312,12 @@ __global__ __launch_bounds__(kDirThreads) void fcg_spmv_dir_kernel(int nx, int n
                 const float v_b = e > 0 ? vc[e - 1] : vb;
                 const uint8_t c_f = e < 3 ? (uint8_t)((Kc >> (8 * (e + 1))) & 0xFF) : cf;
                 const float v_f = e < 3 ? vc[e + 1] : vf;
-                float s = nbr_term((uint8_t)((Km >> (8 * e)) & 0xFF), vl[e], ps) +
-                          nbr_term((uint8_t)((Kn >> (8 * e)) & 0xFF), vr[e], ps);
-                s = s + nbr_term((uint8_t)((kd >> (8 * e)) & 0xFF), vd[e], ps);
-                s = s + nbr_term((uint8_t)((ku >> (8 * e)) & 0xFF), vu[e], ps);
-                s = s + nbr_term(c_b, v_b, ps);
-                s = s + nbr_term(c_f, v_f, ps);
+                float s = nbr_term_z((uint8_t)((Km >> (8 * e)) & 0xFF), vl[e], ps) +
+                          nbr_term_z((uint8_t)((Kn >> (8 * e)) & 0xFF), vr[e], ps);
+                s = s + nbr_term_z((uint8_t)((kd >> (8 * e)) & 0xFF), vd[e], ps);
+                s = s + nbr_term_z((uint8_t)((ku >> (8 * e)) & 0xFF), vu[e], ps);
+                s = s + nbr_term_z(c_b, v_b, ps);
+                s = s + nbr_term_z(c_f, v_f, ps);
                 out[e] = 6.0f * ps - s;
                 pap += (double)ps * (double)out[e];
             }
